@@ -1,0 +1,64 @@
+"""Shared builders for the parity tests: the golden-fixture specs, mirrored for the oracle
+and for the product. Test infrastructure only."""
+import math
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+kB = 1.380649e-23
+
+# Same specs as tests/golden/make_golden.py (kept in sync by test_golden_specs_match).
+CASES = {
+    "neel_pinwheel": dict(lattice="IP_Pinwheel", a=1e-6, n=24, T=300, moment=2.37e-16,
+                          energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=math.pi / 8))],
+                          params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=np.inf), nsteps=200, seed=11),
+    "neel_oop_parabolic": dict(lattice="OOP_Square", a=1e-6, n=20, T=300, moment=1.1e-15,
+                               energies=[("dipolar", {}), ("zeeman", dict(magnitude=2e-5)), ("exchange", dict(J_kT=0.5))],
+                               params=dict(UPDATE_SCHEME="Néel", ENERGY_BARRIER_METHOD="parabolic"),
+                               update_kw=dict(t_max=np.inf), nsteps=200, seed=12),
+    "neel_kagome_tmax": dict(lattice="IP_Kagome", a=1e-6, n=20, T=300, moment=2.37e-16,
+                             energies=[("dipolar", {}), ("zeeman", dict(magnitude=5e-4, angle=0.3)), ("exchange", dict(J_kT=0.3))],
+                             params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=1e-3), nsteps=120, seed=13),
+    "metro_oop_sum": dict(lattice="OOP_Square", a=1e-6, n=32, T=300, moment=1.1e-15,
+                          energies=[("dipolar", {})], params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0),
+                          update_kw=dict(r=4e-6), nsteps=150, seed=21),
+    "metro_oop_conv": dict(lattice="OOP_Square", a=1e-6, n=32, T=600, moment=1.1e-15,
+                           energies=[("dipolar", {}), ("exchange", dict(J_kT=0.2))],
+                           params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=5,
+                                       SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=2),
+                           update_kw=dict(r=3e-6), nsteps=150, seed=22),
+    "metro_ipsq_pbc": dict(lattice="IP_Square", a=1e-6, n=24, T=200, moment=2.37e-16,
+                           energies=[("dipolar", dict(cutoff_range=4e-6)), ("zeeman", dict(magnitude=2e-3, angle=1.0))],
+                           params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
+                                       SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0, METROPOLIS_TIME=True),
+                           kw=dict(PBC=True), update_kw=dict(Q=0.05), nsteps=150, seed=23),
+    "metro_pinwheel": dict(lattice="IP_Pinwheel", a=1e-6, n=24, T=300, moment=2.37e-16,
+                           energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=math.pi / 8))],
+                           params=dict(UPDATE_SCHEME="Metropolis"), update_kw=dict(r=2.5e-6), nsteps=150, seed=24),
+}
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False))
+
+
+def oracle_magnets(name, fx=None):
+    """OracleMagnets for a fixture case, with the fixture's pattern and E_B."""
+    from oracle import model
+    spec = CASES[name]
+    fx = load(name) if fx is None else fx
+    kT = kB * spec["T"]
+    ens = []
+    for kind, kw in spec["energies"]:
+        if kind == "dipolar":
+            ens.append(model.Dipolar(**kw))
+        elif kind == "zeeman":
+            ens.append(model.Zeeman(**kw))
+        else:
+            ens.append(model.Exchange(J=kw["J_kT"] * kT))
+    params = dict(spec["params"])
+    params["UPDATE_SCHEME"] = "Neel" if params["UPDATE_SCHEME"] in ("Néel", "Neel") else "Metropolis"
+    return model.make_magnets(spec["lattice"], spec["a"], n=spec["n"], T=spec["T"], E_B=fx["E_B"],
+                              moment=spec["moment"], pattern=fx["pattern"], energies=ens, params=params,
+                              seed=spec["seed"], PBC=spec.get("kw", {}).get("PBC", False))
