@@ -1,0 +1,177 @@
+"""Device-resident state of one Magnets instance and the calls into libhotspice_hip.so.
+
+PyTorch is used only for device memory and the current HIP stream; all arithmetic on the
+stepping path happens in the library's gfx950 kernels. Layout in HBM (row-major (ny, nx)):
+  m int8 | flags uint8 | E, Ec, Ep, Epc f32 (combined fields + Kahan compensation) |
+  EB, beta, mu f32 | occ_list int32 | kernels K32/Kpo32 f32 and K64/Kps64/Kpo64 f64 of shape
+  [n_sub][2kwy+1][2kwx+1] | run state hs_state_t[2] | scratch (Neel partials, Metropolis lists).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class DeviceState:
+    def __init__(self, mm, device=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise _lib.HotspiceError("hotspice_amd needs a HIP device (MI355X): torch.cuda.is_available() is False")
+        self.lib = _lib.load()
+        self.torch = torch
+        self.device = torch.device("cuda", 0 if device is None else int(device))
+        ny, nx = mm.ny, mm.nx
+        self.nx, self.ny = nx, ny
+        n = nx * ny
+        dev = self.device
+        self.m = torch.zeros(n, dtype=torch.int8, device=dev)
+        self.flags = torch.zeros(n, dtype=torch.uint8, device=dev)
+        self.E = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.Ec = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.Ep = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.Epc = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.EB = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.beta = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.mu = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.occ_list = torch.zeros(max(1, n), dtype=torch.int32, device=dev)
+        self.state = torch.zeros(2 * _lib.STATE_BYTES, dtype=torch.uint8, device=dev)
+        self.partials = torch.zeros(2 * 1024 * 32, dtype=torch.uint8, device=dev)
+        self.zee_bx = None
+        self.zee_by = None
+        self.kern = None  # dict of kernel tensors + window
+        self.metro_ws = None
+        self.sel_cache = {}
+        self.log_cap = 0
+        self.log_idx = None
+        self.log_dt = None
+        self.inst = _lib.Instance()
+
+    # ---- plumbing ----------------------------------------------------------------------
+    def stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    @staticmethod
+    def ptr(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+    def upload(self, tensor, array):
+        src = self.torch.from_numpy(np.ascontiguousarray(array).reshape(-1))
+        tensor.copy_(src.to(tensor.dtype), non_blocking=False)
+
+    def download(self, tensor):
+        return tensor.cpu().numpy()
+
+    # ---- run state ---------------------------------------------------------------------
+    def read_state(self):
+        raw = self.state[: _lib.STATE_BYTES].cpu().numpy().tobytes()
+        return _lib.State.from_buffer_copy(raw)
+
+    def write_state(self, st):
+        buf = np.frombuffer(bytes(st), dtype=np.uint8)
+        self.state[: _lib.STATE_BYTES].copy_(self.torch.from_numpy(buf.copy()))
+
+    # ---- kernels -------------------------------------------------------------------------
+    def build_kernels(self, geom: _lib.KernelGeom, rrx, rry, in_plane):
+        torch = self.torch
+        dev = self.device
+        kwx, kwy, ns = geom.kwx, geom.kwy, geom.n_sub
+        shape = (ns, 2 * kwy + 1, 2 * kwx + 1)
+        t_rrx = torch.from_numpy(np.ascontiguousarray(rrx, dtype=np.float64)).to(dev)
+        t_rry = torch.from_numpy(np.ascontiguousarray(rry, dtype=np.float64)).to(dev)
+        geom.rrx = t_rrx.data_ptr()
+        geom.rry = t_rry.data_ptr()
+        K64 = torch.empty(shape, dtype=torch.float64, device=dev)
+        K32 = torch.empty(shape, dtype=torch.float32, device=dev)
+        Kps64 = torch.empty(shape, dtype=torch.float64, device=dev) if in_plane else None
+        Kpo64 = torch.empty(shape, dtype=torch.float64, device=dev) if in_plane else None
+        Kpo32 = torch.empty(shape, dtype=torch.float32, device=dev) if in_plane else None
+        _lib.check(self.lib.hs_build_dipolar_kernels(C.byref(geom), self.ptr(K64), self.ptr(Kps64), self.ptr(Kpo64),
+                                                     self.ptr(K32), self.ptr(Kpo32), self.stream()))
+        torch.cuda.current_stream(dev).synchronize()
+        self.kern = dict(K64=K64, K32=K32, Kps64=Kps64, Kpo64=Kpo64, Kpo32=Kpo32, kwx=kwx, kwy=kwy, n_sub=ns,
+                         _rr=(t_rrx, t_rry))
+
+    def drop_kernels(self):
+        self.kern = None
+
+    def kernels_full(self, which, nx, ny):
+        """Kernels zero-padded to the reference's (n_sub, 2ny-1, 2nx-1) shape, FP64 on the host."""
+        if self.kern is None:
+            return None
+        name = {"K": "K64", "Kps": "Kps64", "Kpo": "Kpo64"}[which]
+        t = self.kern[name]
+        ns, kwx, kwy = self.kern["n_sub"], self.kern["kwx"], self.kern["kwy"]
+        out = np.zeros((ns, 2 * ny - 1, 2 * nx - 1))
+        if t is None:
+            return out
+        out[:, ny - 1 - kwy:ny + kwy, nx - 1 - kwx:nx + kwx] = t.cpu().numpy()
+        return out
+
+    # ---- calls -----------------------------------------------------------------------------
+    def recompute(self, which=3):
+        _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.stream()))
+
+    def components(self):
+        out = self.torch.empty((6, self.ny, self.nx), dtype=self.torch.float64, device=self.device)
+        _lib.check(self.lib.hs_field_components(C.byref(self.inst), self.ptr(out), self.stream()))
+        return out.cpu().numpy()
+
+    def energy_total(self):
+        v = C.c_double(0.0)
+        _lib.check(self.lib.hs_energy_total(C.byref(self.inst), C.byref(v), self.stream()))
+        return v.value
+
+    def zeeman_swap(self, Bx0, By0, bx_old=None, by_old=None):
+        _lib.check(self.lib.hs_zeeman_swap(C.byref(self.inst), float(Bx0), float(By0), self.ptr(bx_old),
+                                           self.ptr(by_old), self.stream()))
+
+    def ensure_log(self, n):
+        if self.log_cap < n:
+            cap = max(n, 1024)
+            self.log_idx = self.torch.zeros(cap, dtype=self.torch.int32, device=self.device)
+            self.log_dt = self.torch.zeros(cap, dtype=self.torch.float64, device=self.device)
+            self.log_cap = cap
+
+    def neel_run(self, run: _lib.Run):
+        if run.log:
+            self.ensure_log(run.max_steps)
+        fc = C.c_int32(0)
+        _lib.check(self.lib.hs_neel_run(C.byref(self.inst), C.byref(run), self.ptr(self.state), self.ptr(self.partials),
+                                        self.ptr(self.log_idx) if run.log else C.c_void_p(0),
+                                        self.ptr(self.log_dt) if run.log else C.c_void_p(0), C.byref(fc),
+                                        self.stream()))
+
+    def metropolis_run(self, sel: _lib.Select, run: _lib.Run, log_cells=None, log_count=None):
+        need = int(self.lib.hs_metropolis_ws_bytes(C.byref(self.inst), C.byref(sel)))
+        if need < 0:
+            raise _lib.HotspiceError("bad selection geometry")
+        if self.metro_ws is None or self.metro_ws.numel() < need:
+            self.metro_ws = self.torch.zeros(need, dtype=self.torch.uint8, device=self.device)
+        _lib.check(self.lib.hs_metropolis_run(C.byref(self.inst), C.byref(sel), C.byref(run), self.ptr(self.state),
+                                              self.ptr(self.metro_ws), self.ptr(log_cells), self.ptr(log_count),
+                                              self.stream()))
+
+    def philox_uniform53(self, seed, step, tag, index0, n, open_zero=False):
+        out = self.torch.empty(max(1, n), dtype=self.torch.float64, device=self.device)
+        _lib.check(self.lib.hs_philox_uniform53(seed & 0xFFFFFFFFFFFFFFFF, step, tag, index0, n, int(open_zero),
+                                                self.ptr(out), self.stream()))
+        return out[:n].cpu().numpy()
+
+
+def kernel_window(rrx, rry, nx, ny, cutoff, pbc):
+    """Half widths (kwx, kwy) of the kernel region that can be nonzero under `cutoff`."""
+    if pbc or not math.isfinite(cutoff):
+        return nx - 1, ny - 1
+    lim = cutoff * (1 + 1e-9)  # a margin: entries at the edge that round to zero are harmless
+    cx = np.abs(np.asarray(rrx)) <= lim
+    cy = np.abs(np.asarray(rry)) <= lim
+    kwx = int(np.max(np.abs(np.nonzero(cx)[1] - (nx - 1)))) if cx.any() else 0
+    kwy = int(np.max(np.abs(np.nonzero(cy)[1] - (ny - 1)))) if cy.any() else 0
+    return min(kwx, nx - 1), min(kwy, ny - 1)

@@ -1,0 +1,143 @@
+"""ctypes bindings of libhotspice_hip.so (the C-ABI declared in include/hotspice_hip.h).
+
+The library is loaded from this package's directory; it is built by ``hotspice_amd.build``
+(``__graft_entry__.build()``). There is no fallback: if the library or a HIP device is missing,
+every stepping call raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhotspice_hip.so")
+
+HS_MAX_UC = 64
+HS_MAX_NN = 49
+HS_FLAG_OCC = 1
+HS_FLAG_NEEL = 2
+
+TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME = 1, 2, 3, 4, 5
+
+EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs_philox_uniform53", "hs_build_dipolar_kernels",
+           "hs_recompute_fields", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
+           "hs_metropolis_run", "hs_metropolis_ws_bytes")
+
+
+class Instance(C.Structure):
+    """hs_instance_t"""
+    _fields_ = [
+        ("m", C.c_void_p), ("flags", C.c_void_p), ("E", C.c_void_p), ("Ec", C.c_void_p),
+        ("Ep", C.c_void_p), ("Epc", C.c_void_p), ("EB", C.c_void_p), ("beta", C.c_void_p), ("mu", C.c_void_p),
+        ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
+        ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p),
+        ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
+        ("J", C.c_double), ("m_perp_factor", C.c_double),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),
+        ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
+        ("use_perp", C.c_int32), ("barrier", C.c_int32), ("nn_w", C.c_int32), ("nn_h", C.c_int32),
+        ("n_occ", C.c_int32), ("has_zeeman", C.c_int32), ("uniform_beta", C.c_int32),
+        ("sub_of", C.c_int32 * HS_MAX_UC), ("ox_uc", C.c_float * HS_MAX_UC), ("oy_uc", C.c_float * HS_MAX_UC),
+        ("nn_mask", C.c_float * HS_MAX_NN),
+    ]
+
+
+class KernelGeom(C.Structure):
+    """hs_kernel_geom_t"""
+    _fields_ = [
+        ("rrx", C.c_void_p), ("rry", C.c_void_p), ("I", C.c_double), ("cutoff", C.c_double), ("decay", C.c_double),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),
+        ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
+        ("site_x", C.c_int32 * HS_MAX_UC), ("site_y", C.c_int32 * HS_MAX_UC),
+        ("shift_x", C.c_int32 * HS_MAX_UC), ("shift_y", C.c_int32 * HS_MAX_UC),
+        ("uc_ox", C.c_float * HS_MAX_UC), ("uc_oy", C.c_float * HS_MAX_UC), ("uc_o", C.c_float * HS_MAX_UC),
+    ]
+
+
+class Run(C.Structure):
+    """hs_run_t"""
+    _fields_ = [("t_max", C.c_double), ("mc_max", C.c_double), ("attempt_freq", C.c_double),
+                ("max_steps", C.c_int64), ("log", C.c_int32), ("nblocks", C.c_int32), ("begin", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class State(C.Structure):
+    """hs_state_t"""
+    _fields_ = [("t", C.c_double), ("t0", C.c_double), ("attempts", C.c_int64), ("attempts0", C.c_int64),
+                ("switches", C.c_int64), ("step", C.c_int64), ("steps_done", C.c_int64), ("done", C.c_int32),
+                ("err", C.c_int32)]
+
+
+class Select(C.Structure):
+    """hs_select_t"""
+    _fields_ = [("pat_off", C.c_void_p), ("pat_cells", C.c_void_p), ("Rx", C.c_int32), ("Ry", C.c_int32),
+                ("snx", C.c_int32), ("sny", C.c_int32), ("ncols", C.c_int32), ("nsc", C.c_int32),
+                ("scheme", C.c_int32), ("cutoff", C.c_int32), ("reduced_k", C.c_int32),
+                ("metropolis_time", C.c_int32)]
+
+
+STATE_BYTES = C.sizeof(State)
+
+_lib = None
+
+
+class HotspiceError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads the library (once). Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HotspiceError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+    sig = {
+        "hs_version": ([], C.c_int),
+        "hs_abi_layout": ([C.POINTER(i64), i32], C.c_int),
+        "hs_last_error": ([], C.c_char_p),
+        "hs_device_info": ([C.POINTER(i32), C.c_char_p, i32], C.c_int),
+        "hs_philox_uniform53": ([u64, u64, C.c_uint32, i64, i64, i32, vp, vp], C.c_int),
+        "hs_build_dipolar_kernels": ([C.POINTER(KernelGeom), vp, vp, vp, vp, vp, vp], C.c_int),
+        "hs_recompute_fields": ([C.POINTER(Instance), i32, vp], C.c_int),
+        "hs_field_components": ([C.POINTER(Instance), vp, vp], C.c_int),
+        "hs_energy_total": ([C.POINTER(Instance), C.POINTER(dbl), vp], C.c_int),
+        "hs_zeeman_swap": ([C.POINTER(Instance), dbl, dbl, vp, vp, vp], C.c_int),
+        "hs_neel_run": ([C.POINTER(Instance), C.POINTER(Run), vp, vp, vp, vp, C.POINTER(i32), vp], C.c_int),
+        "hs_metropolis_run": ([C.POINTER(Instance), C.POINTER(Select), C.POINTER(Run), vp, vp, vp, vp, vp], C.c_int),
+        "hs_metropolis_ws_bytes": ([C.POINTER(Instance), C.POINTER(Select)], C.c_int64),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().hs_last_error().decode(errors="replace")
+        raise HotspiceError(f"libhotspice_hip error {rc}: {msg}")
+    return rc
+
+
+def exported_symbols():
+    """Names declared in include/hotspice_hip.h that the loaded library exports."""
+    lib = load()
+    return [n for n in EXPORTS if hasattr(lib, n)]
+
+
+def abi_layout_python():
+    """The same 15 numbers as hs_abi_layout, from the ctypes mirrors."""
+    return [C.sizeof(Instance), Instance.seed.offset, Instance.nx.offset, Instance.sub_of.offset,
+            Instance.nn_mask.offset, C.sizeof(KernelGeom), KernelGeom.nx.offset, KernelGeom.uc_o.offset,
+            C.sizeof(Run), Run.max_steps.offset, Run.begin.offset, C.sizeof(State), State.done.offset,
+            C.sizeof(Select), Select.metropolis_time.offset]
+
+
+def abi_layout_library():
+    lib = load()
+    out = (C.c_int64 * 15)()
+    n = lib.hs_abi_layout(out, 15)
+    return list(out)[:n]

@@ -1,0 +1,959 @@
+"""Magnets, SimParams and Scheme: the reference's stepping API over the HIP stepping kernels.
+
+Mirrors hotspice/core.py (SimParams 29-53, Magnets 56-980): same constructor, fields, setters,
+update()/progress() entry points and return values. The state (spins, combined local fields,
+barriers, temperatures, moments) lives on the GPU; update() and progress() run entirely in
+libhotspice_hip.so (hs_neel_run / hs_metropolis_run). Host arrays such as ``m`` are mirrors:
+reading downloads, writing (or editing the returned array and calling update_energy())
+uploads. There is no CPU fallback: without the library and a HIP device, Magnets raises.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from abc import ABC, abstractmethod
+from dataclasses import dataclass, field
+from enum import Enum, auto
+from typing import Literal
+
+import numpy as np
+
+from . import _lib
+from ._device import DeviceState, kernel_window
+from .energies import DipolarEnergy, Energy, ExchangeEnergy, ZeemanEnergy
+from .utils import as_2D_array, check_repetition, full_obj_name, lower_than
+
+kB = 1.380649e-23
+
+
+class Scheme(Enum):
+    """core.py:29-32"""
+    NEEL = auto()
+    METROPOLIS = auto()
+    WOLFF = auto()
+
+
+@dataclass(slots=True)
+class SimParams:
+    """core.py:34-53 (same fields, defaults and validation)."""
+    SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF: int = 20
+    REDUCED_KERNEL_SIZE: int = 20
+    UPDATE_SCHEME: Scheme | str = Scheme.NEEL
+    MULTISAMPLING_SCHEME: Literal['single', 'grid', 'Poisson', 'cluster'] = 'grid'
+    ENERGY_BARRIER_METHOD: Literal['simple', 'parabolic'] = 'simple'
+    METROPOLIS_TIME: bool = False
+
+    def __post_init__(self):
+        self.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF = int(self.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF)
+        self.REDUCED_KERNEL_SIZE = int(self.REDUCED_KERNEL_SIZE)
+        if not isinstance(self.UPDATE_SCHEME, Scheme):
+            allowed = {'Néel': Scheme.NEEL, 'Neel': Scheme.NEEL, 'Metropolis': Scheme.METROPOLIS,
+                       'Glauber': Scheme.METROPOLIS, 'Wolff': Scheme.WOLFF}
+            if self.UPDATE_SCHEME not in allowed:
+                raise ValueError(f"UPDATE_SCHEME='{self.UPDATE_SCHEME}' is invalid: allowed values are {allowed.keys()}.")
+            self.UPDATE_SCHEME = allowed[self.UPDATE_SCHEME]
+        if self.MULTISAMPLING_SCHEME not in (allowed := ['single', 'grid', 'Poisson', 'cluster']):
+            raise ValueError(f"MULTISAMPLING_SCHEME='{self.MULTISAMPLING_SCHEME}' is invalid: allowed values are {allowed}.")
+
+
+@dataclass(slots=True)
+class Unitcell:
+    x: int
+    y: int
+
+
+@dataclass
+class History:
+    """core.py:982-1000"""
+    E: list = field(default_factory=list)
+    T: list = field(default_factory=list)
+    t: list = field(default_factory=list)
+    m: list = field(default_factory=list)
+
+    def entry(self, mm):
+        self.E.append(mm.E_tot)
+        self.T.append(mm.T_avg)
+        self.t.append(mm.t if mm.t != 0 else mm.switches)
+        self.m.append(mm.m_avg)
+
+    def clear(self):
+        self.E.clear()
+        self.T.clear()
+        self.t.clear()
+        self.m.clear()
+
+
+class Magnets(ABC):
+    """Device-backed counterpart of hotspice.core.Magnets (core.py:56-980).
+
+    Build-side additions (keyword-only, not in the reference):
+      seed            Philox key of this instance (default: random); trajectories are a pure
+                      function of (seed, state), so runs are reproducible.
+      device          HIP device index (default 0).
+      recompute_every Neel switches (or Metropolis steps) between FP64 full recomputes of the
+                      FP32 fields, which reset incremental drift (default 1024; None disables).
+    """
+
+    def __init__(self, nx: int, ny: int, dx, dy, *, T=300, E_B=0., moment=None, Msat=800e3, V=2e-22,
+                 pattern: str = None, energies=None, PBC: bool = False, m_perp_factor: float = None,
+                 E_B_std: float = None, major_axis: float = 0, minor_axis: float = None, angle: float = 0.,
+                 params: SimParams = None, in_plane: bool = False, seed: int = None, device: int = None,
+                 recompute_every: int | None = 1024):
+        self.seed = int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
+        self.params = SimParams() if params is None else params
+        self.recompute_every = recompute_every
+        self._energies = list()
+        self.in_plane = in_plane
+        self.nx, self.ny = int(nx), int(ny)
+        self.ixx, self.iyy = np.meshgrid(np.arange(0, self.nx), np.arange(0, self.ny))
+        self._set_dx(dx)
+        self._set_dy(dy)
+        self.occupation = self._get_occupation().astype(bool).astype(int)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            self._occupation_inf = 1 / self.occupation
+        self.nonzero = np.nonzero(self.occupation)
+        self._nonzero_array = np.asarray(self.nonzero).reshape(2, -1)
+        self.n = int(np.sum(self.occupation))
+        if self.n == 0:
+            raise ValueError(f"Can not initialize {full_obj_name(self)} of size {self.nx}x{self.ny}, as this does not contain any spins.")
+        if self.in_plane:
+            self._initialize_ip(angle=angle)
+        self.unitcell = self._get_unitcell()
+        self._PBC = False
+        self.PBC = PBC
+        self._m_host = None
+        self._m_ref = None
+        self._occ_ref = None
+        self._dev = DeviceState(self, device)
+        self._m_perp_factor = 0.0
+        self._calc_r_cache = {}
+        self._since_recompute = 0
+        self._sel_key = None
+        self.initialize_m(self._get_groundstate() if pattern is None else pattern, update_energy=False)
+        self.T = T
+        self.E_B = E_B
+        if E_B_std is not None:
+            self.E_B = self.E_B * self.rng.normal(1, E_B_std, size=self.E_B.shape)
+        self.moment = Msat * V if moment is None else moment
+        self.m_perp_factor = in_plane if m_perp_factor is None else m_perp_factor
+        self.major_axis = major_axis
+        self.minor_axis = major_axis if minor_axis is None else minor_axis
+        self.history = History()
+        self._dev.write_state(_lib.State())
+        self._zeros = np.zeros_like(self.m)
+        if energies is None:
+            energies = (DipolarEnergy(),)
+        elif isinstance(energies, Energy):
+            energies = (energies,)
+        else:
+            energies = tuple(energies)
+        self._upload_flags()
+        self._device_params_changed()
+        for energy in energies:
+            self.add_energy(energy, _recompute=False)
+        self._device_recompute()
+
+    # ---- geometry ------------------------------------------------------------------------------
+    def _set_dx(self, value):
+        value = np.asarray(value, dtype=float).reshape(-1)
+        self._dx = np.tile(value, math.ceil(self.nx / value.size))[:self.nx]
+        self.x = np.zeros(self.nx)
+        self.x[1:] = np.cumsum(self._dx)[:-1]
+        self.xx, _ = np.meshgrid(self.x, np.empty(self.ny))
+        self.x_min, self.x_max = float(self.x[0]), float(self.x[-1])
+
+    def _set_dy(self, value):
+        value = np.asarray(value, dtype=float).reshape(-1)
+        self._dy = np.tile(value, math.ceil(self.ny / value.size))[:self.ny]
+        self.y = np.zeros(self.ny)
+        self.y[1:] = np.cumsum(self._dy)[:-1]
+        _, self.yy = np.meshgrid(np.empty(self.nx), self.y)
+        self.y_min, self.y_max = float(self.y[0]), float(self.y[-1])
+
+    @property
+    def dx(self):
+        return self._dx
+
+    @dx.setter
+    def dx(self, value):
+        self._set_dx(value)
+        for e in self._energies:
+            e.initialize(self)
+        self._device_params_changed()
+        self._device_recompute()
+
+    @property
+    def dy(self):
+        return self._dy
+
+    @dy.setter
+    def dy(self, value):
+        self._set_dy(value)
+        for e in self._energies:
+            e.initialize(self)
+        self._device_params_changed()
+        self._device_recompute()
+
+    def _get_unitcell(self, max_cell=100):
+        """core.py:146-161: smallest (ux, uy) in the same triangular search order."""
+        dx, dy = self.dx.reshape(1, -1), self.dy.reshape(-1, 1)
+        arr = self.orientation if self.in_plane else self.occupation
+        for n in range(1, min(self.nx + self.ny + 1, max_cell + 1)):
+            for uy in range(max(0, n - self.nx) + 1, min(n, self.ny) + 1):
+                ux = n - uy + 1
+                if check_repetition(arr, ux, uy) and check_repetition(dx, ux, 1) and check_repetition(dy, 1, uy):
+                    return Unitcell(ux, uy)
+        warnings.warn(f"Could not detect a reasonably sized unit cell. Defaulting to entire domain {self.nx}x{self.ny}.", stacklevel=2)
+        return Unitcell(self.nx, self.ny)
+
+    def _initialize_ip(self, angle: float = 0.):
+        """core.py:182-193"""
+        self.original_angles = self._get_angles() + angle
+        self.angles = self.original_angles * self.occupation
+        self.orientation = np.zeros(self.ixx.shape + (2,))
+        self.orientation[:, :, 0] = np.cos(self.angles) * self.occupation
+        self.orientation[:, :, 1] = np.sin(self.angles) * self.occupation
+
+    @property
+    def PBC(self):
+        return self._PBC
+
+    @PBC.setter
+    def PBC(self, value: bool):
+        """core.py:361-375"""
+        if value and (self.nx % self.unitcell.x != 0 or self.ny % self.unitcell.y != 0):
+            warnings.warn(f"PBC can not be enabled, because there is a non-integer number of unit cells along the x- "
+                          f"and/or y-axis (unit cell {self.unitcell.x}x{self.unitcell.y}).", stacklevel=2)
+            self._PBC = False
+            return
+        self._PBC = bool(value)
+        if hasattr(self, "_dev"):
+            for e in self._energies:
+                e.initialize(self)
+            self._device_params_changed()
+            self._device_recompute()
+
+    @property
+    def shape(self):
+        return (self.ny, self.nx)
+
+    # ---- spins ---------------------------------------------------------------------------------
+    def _get_m_uniform(self, angle=0):
+        """core.py:138-144"""
+        angle += 1e-6
+        if self.in_plane:
+            return self.occupation * (2 * ((self.orientation[:, :, 0] * math.cos(angle) + self.orientation[:, :, 1] * math.sin(angle)) >= 0) - 1)
+        return np.ones_like(self.xx) * ((math.floor(angle / math.pi - .5) % 2) * 2 - 1)
+
+    def initialize_m(self, pattern='random', *, angle: float = 0, update_energy: bool = True):
+        """core.py:163-180"""
+        if isinstance(pattern, str):
+            self._set_m(pattern)
+            m = self._m_pattern
+        else:
+            try:
+                m = as_2D_array(pattern, self.shape)
+            except (ValueError, TypeError):
+                raise ValueError("Argument <pattern> could not be parsed as a valid array for <self.m>.")
+        m = np.sign(np.asarray(m, dtype=float))
+        m = np.multiply(m, self.occupation)
+        m[np.where(self._get_m_uniform() != self._get_m_uniform(angle))] *= -1
+        self.m = m
+        if update_energy:
+            self.update_energy()
+
+    def _set_m(self, pattern: str):
+        """core.py:924-952 (subclasses add lattice patterns); stores the result in self._m_pattern."""
+        match str(pattern).strip().lower():
+            case 'uniform':
+                self._m_pattern = self._get_m_uniform()
+            case 'vortex':
+                m = np.ones_like(self.xx)
+                if self.in_plane:
+                    diff = (self.xx - (self.x_max - self.x_min) / 2) + 1j * (self.yy - (self.y_max - self.y_min) / 2)
+                    desired = diff * (-1j + 1e-6)
+                    dot = desired.real * self.orientation[:, :, 0] + desired.imag * self.orientation[:, :, 1]
+                    dot[dot == 0] = 1
+                    occ = np.where(self.occupation == 1)
+                    m[occ] = np.sign(dot)[occ]
+                else:
+                    m[np.where(self.xx >= (self.x_max - self.x_min) / 2)] *= -1
+                self._m_pattern = m
+            case str(unknown):
+                self._m_pattern = self.rng.integers(0, 2, size=self.xx.shape) * 2 - 1
+                if unknown != 'random':
+                    warnings.warn(f"Pattern '{unknown}'' not recognized, defaulting to 'random'.", stacklevel=2)
+
+    @property
+    def m(self) -> np.ndarray:
+        """(ny, nx) float64 host mirror of the device spins (-1/0/+1)."""
+        if self._m_host is None:
+            self._m_host = self._dev.download(self._dev.m).reshape(self.ny, self.nx).astype(float)
+            self._m_ref = self._m_host.copy()
+        return self._m_host
+
+    @m.setter
+    def m(self, value):
+        self._m_host = np.array(as_2D_array(value, self.shape), dtype=float)
+        self._m_ref = None
+        self._sync_m()
+
+    def _sync_m(self):
+        """Uploads the host mirror if it was assigned or edited in place since the last sync."""
+        if self._m_host is None:
+            return
+        if self._m_ref is not None and np.array_equal(self._m_host, self._m_ref):
+            return
+        self._dev.upload(self._dev.m, np.sign(self._m_host).astype(np.int8))
+        self._m_ref = self._m_host.copy()
+
+    def _after_device_step(self):
+        self._m_host = None
+        self._m_ref = None
+
+    def _upload_flags(self):
+        occ = np.asarray(self.occupation).astype(bool)
+        neel = np.isfinite(self._occupation_inf)
+        flags = occ.astype(np.uint8) * _lib.HS_FLAG_OCC + neel.astype(np.uint8) * _lib.HS_FLAG_NEEL
+        self._dev.upload(self._dev.flags, flags)
+        occ_list = np.flatnonzero(np.isfinite(self._occupation_inf)).astype(np.int32)
+        self._dev.upload(self._dev.occ_list[:occ_list.size], occ_list)
+        self._occ_ref = np.array(self.occupation, copy=True)
+
+    def _sync_occupation(self):
+        if self._occ_ref is None or not np.array_equal(self._occ_ref, self.occupation):
+            self._upload_flags()
+            self._sel_key = None
+
+    # ---- fields --------------------------------------------------------------------------------
+    @property
+    def T(self):
+        return self._T
+
+    @T.setter
+    def T(self, value):
+        """core.py:331-334"""
+        self._T = as_2D_array(value, self.shape)
+        self.kBT = kB * self._T
+        with np.errstate(divide='ignore'):
+            self.beta = 1 / self.kBT
+        self._T_any_zero = bool(np.any(self._T == 0))
+        self._dev.upload(self._dev.beta, self.beta.astype(np.float32))
+        self._uniform_beta = bool(np.all(self.beta == self.beta.reshape(-1)[0]))
+        if hasattr(self, "_energies"):
+            self._device_params_changed()
+
+    @property
+    def E_B(self):
+        return self._E_B
+
+    @E_B.setter
+    def E_B(self, value):
+        """core.py:340-343"""
+        self._E_B = np.array(as_2D_array(value, self.shape), dtype=float)
+        with np.errstate(divide='ignore'):
+            self._E_B_inv_over_four = .25 / self._E_B
+        self._dev.upload(self._dev.EB, self._E_B.astype(np.float32))
+        self._EB_ref = self._E_B.copy()
+
+    def _sync_EB(self):
+        if not np.array_equal(self._E_B, self._EB_ref):
+            self.E_B = self._E_B
+
+    @property
+    def moment(self):
+        return self._moment
+
+    @moment.setter
+    def moment(self, value):
+        """core.py:349-351"""
+        self._moment = np.array(as_2D_array(value, self.shape), dtype=float)
+        self._momentSq = self._moment * self._moment
+        self._dev.upload(self._dev.mu, self._moment.astype(np.float32))
+
+    @property
+    def USE_PERP_ENERGY(self):
+        return bool(self.m_perp_factor) and self.in_plane
+
+    @property
+    def m_perp_factor(self):
+        return self._m_perp_factor
+
+    @m_perp_factor.setter
+    def m_perp_factor(self, value):
+        """core.py:279-287"""
+        if not isinstance(value, (int, float, bool)):
+            raise ValueError("m_perp_factor must be scalar. It is not currently supported as a 2D array.")
+        if value and not self.in_plane:
+            warnings.warn("You set a nonzero `m_perp_energy`, but this has no effect for an OOP ASI.")
+        used = self.USE_PERP_ENERGY
+        self._m_perp_factor = float(value)
+        if hasattr(self, "_energies") and self._energies:
+            self._device_params_changed()
+            if not used and self.USE_PERP_ENERGY:
+                self._device_recompute()
+
+    # ---- counters (device run state) ------------------------------------------------------------
+    def _state(self):
+        return self._dev.read_state()
+
+    @property
+    def t(self):
+        return float(self._state().t)
+
+    @t.setter
+    def t(self, value):
+        st = self._state()
+        st.t = float(value)
+        self._dev.write_state(st)
+
+    @property
+    def switches(self):
+        return int(self._state().switches)
+
+    @switches.setter
+    def switches(self, value):
+        st = self._state()
+        st.switches = int(value)
+        self._dev.write_state(st)
+
+    @property
+    def attempted_switches(self):
+        return int(self._state().attempts)
+
+    @attempted_switches.setter
+    def attempted_switches(self, value):
+        st = self._state()
+        st.attempts = int(value)
+        self._dev.write_state(st)
+
+    @property
+    def MCsteps(self):
+        return self.attempted_switches / self.n
+
+    def reset_stats(self):
+        st = self._state()
+        st.t, st.switches, st.attempts = 0.0, 0, 0
+        self._dev.write_state(st)
+
+    # ---- energies ------------------------------------------------------------------------------
+    def add_energy(self, energy: Energy, exist_ok=False, verbose=True, _recompute=True):
+        """core.py:195-209"""
+        if not isinstance(energy, (ZeemanEnergy, DipolarEnergy, ExchangeEnergy)):
+            raise NotImplementedError(f"{type(energy).__name__} has no device implementation; supported energies are "
+                                      "ZeemanEnergy, DipolarEnergy and ExchangeEnergy.")
+        for i, e in enumerate(self._energies):
+            if type(e) is type(energy):
+                if exist_ok:
+                    return e
+                if verbose:
+                    warnings.warn(f"An instance of {type(energy).__name__} was already included in the simulation, and has now been overwritten.", stacklevel=2)
+                energy.initialize(self)
+                self._energies[i] = energy
+                self._device_params_changed()
+                if _recompute:
+                    self._device_recompute()
+                return
+        energy.initialize(self)
+        self._energies.append(energy)
+        self._device_params_changed()
+        if _recompute:
+            self._device_recompute()
+        return energy
+
+    def remove_energy(self, name: str, verbose=True):
+        """core.py:211-221"""
+        name = name.lower().replace('energy', '')
+        for i, e in enumerate(self._energies):
+            if name == e.shortname:
+                self._energies.pop(i)
+                if isinstance(e, DipolarEnergy):
+                    self._dev.drop_kernels()
+                self._device_params_changed()
+                self._device_recompute()
+                return
+        if verbose:
+            warnings.warn(f"There is no '{name}' energy associated with this Magnets object. Valid energies are: {[e.shortname for e in self._energies]}", stacklevel=2)
+
+    def has_energy(self, name: str):
+        name = name.lower().replace('energy', '')
+        return any(name == e.shortname for e in self._energies)
+
+    def get_energy(self, name: str, verbose=True):
+        """core.py:229-239"""
+        name = name.lower().replace('energy', '')
+        for e in self._energies:
+            if name == e.shortname:
+                return e
+        if verbose:
+            warnings.warn(f"There is no '{name}' energy associated with this Magnets object. Valid energies are: {[e.shortname for e in self._energies]}", stacklevel=2)
+        return None
+
+    def update_energy(self, index=None):
+        """core.py:241-253. Host edits of m / occupation are uploaded, then the combined device
+        field is recomputed in FP64 (index=None) -- also for an explicit index, where the
+        recompute equals the reference's incremental update up to rounding."""
+        self._sync_occupation()
+        self._sync_m()
+        self._sync_EB()
+        self._device_recompute()
+
+    def switch_energy(self, indices2D=None):
+        """core.py:255-260: -2*sum(E) at the given indices (or everywhere)."""
+        E = self.E
+        return -2 * E if indices2D is None else -2 * E[indices2D[0], indices2D[1]]
+
+    def perp_energy(self, indices2D=None):
+        """core.py:262-271"""
+        if not self.USE_PERP_ENERGY:
+            return self._zeros if indices2D is None else np.zeros(np.asarray(indices2D).shape[1])
+        Ep = self.E_perp
+        return Ep if indices2D is None else Ep[indices2D[0], indices2D[1]]
+
+    @property
+    def E(self) -> np.ndarray:
+        """Sum of the energies' E (core.py:305-307), FP64 from the device's compensated field."""
+        return self._dev.components()[4]
+
+    @property
+    def E_perp(self) -> np.ndarray:
+        """core.py:309-312"""
+        return self._dev.components()[5] * self.m_perp_factor
+
+    @property
+    def E_tot(self) -> float:
+        """core.py:426-428"""
+        return float(self._dev.energy_total()) if self._energies else 0.0
+
+    def _energy_component(self, energy, perp=False):
+        comp = self._dev.components()
+        if isinstance(energy, ZeemanEnergy):
+            return comp[1] if perp else comp[0]
+        if isinstance(energy, ExchangeEnergy):
+            return comp[3] if perp else comp[2]
+        total = comp[5] if perp else comp[4]
+        zee = comp[1] if perp else comp[0]
+        exc = comp[3] if perp else comp[2]
+        return total - zee - exc
+
+    # ---- device instance -------------------------------------------------------------------------
+    def _device_build_kernels(self, dip: DipolarEnergy):
+        """Host part of DipolarEnergy._initialize (energies.py:183-214): offsets of the tiled
+        unit-cell positions around each sublattice's centre magnet; the device builds the kernels."""
+        ux, uy, nx, ny = self.unitcell.x, self.unitcell.y, self.nx, self.ny
+        ncx = 2 * math.ceil(nx / ux) + 1
+        ncy = 2 * math.ceil(ny / uy) + 1
+        uc_dx = np.tile(self.dx[:ux], ncx)
+        uc_dy = np.tile(self.dy[:uy], ncy)
+        g = _lib.KernelGeom()
+        sites = []
+        for y in range(uy):
+            for x in range(ux):
+                if self.in_plane:
+                    if self.orientation[y, x, 0] == 0 and self.orientation[y, x, 1] == 0:
+                        continue
+                elif self.occupation[y, x] == 0:
+                    continue
+                sites.append((y, x))
+        if not sites:
+            raise ValueError("No occupied site in the unit cell: no dipolar kernel to build.")
+        if len(sites) > _lib.HS_MAX_UC:
+            raise ValueError("Too many unit-cell sites for the device kernels.")
+        rrx = np.zeros((len(sites), 2 * nx - 1))
+        rry = np.zeros((len(sites), 2 * ny - 1))
+        for n, (y, x) in enumerate(sites):
+            sx = (ux - ((nx - 1) % ux) + x) % ux
+            sy = (uy - ((ny - 1) % uy) + y) % uy
+            xx = np.cumsum(uc_dx[sx:sx + 2 * nx - 1])
+            yy = np.cumsum(uc_dy[sy:sy + 2 * ny - 1])
+            rrx[n] = xx - xx[nx - 1]
+            rry[n] = yy - yy[ny - 1]
+            g.site_x[n], g.site_y[n], g.shift_x[n], g.shift_y[n] = x, y, sx, sy
+        for s in range(uy * ux):
+            y, x = divmod(s, ux)
+            if self.in_plane:
+                g.uc_ox[s] = float(self.orientation[y, x, 0])
+                g.uc_oy[s] = float(self.orientation[y, x, 1])
+            else:
+                g.uc_o[s] = float(self.occupation[y, x])
+        kwx, kwy = kernel_window(rrx, rry, nx, ny, float(dip.cutoff_range), self.PBC)
+        g.I = (self.major_axis ** 2 + self.minor_axis ** 2) / 16
+        g.cutoff = float(dip.cutoff_range)
+        g.decay = float(dip.decay_exponent)
+        g.nx, g.ny, g.ux, g.uy, g.n_sub, g.kwx, g.kwy = nx, ny, ux, uy, len(sites), kwx, kwy
+        g.pbc, g.in_plane = int(self.PBC), int(self.in_plane)
+        self._dev.build_kernels(g, rrx, rry, self.in_plane)
+        self._kernel_sites = sites
+
+    def _device_params_changed(self):
+        """(Re)fills the hs_instance_t the kernels read."""
+        d = self._dev
+        I = d.inst
+        P = d.ptr
+        I.m, I.flags, I.E, I.Ec = P(d.m), P(d.flags), P(d.E), P(d.Ec)
+        I.Ep, I.Epc, I.EB, I.beta, I.mu = P(d.Ep), P(d.Epc), P(d.EB), P(d.beta), P(d.mu)
+        I.occ_list = P(d.occ_list)
+        I.seed = self.seed
+        I.nx, I.ny, I.ux, I.uy = self.nx, self.ny, self.unitcell.x, self.unitcell.y
+        I.pbc, I.in_plane, I.use_perp = int(self.PBC), int(self.in_plane), int(self.USE_PERP_ENERGY)
+        I.m_perp_factor = float(self.m_perp_factor)
+        I.barrier = 0 if self.params.ENERGY_BARRIER_METHOD == 'simple' else 1
+        I.n_occ = int(np.count_nonzero(np.isfinite(self._occupation_inf)))
+        I.uniform_beta = int(getattr(self, "_uniform_beta", True))
+        for s in range(_lib.HS_MAX_UC):
+            I.sub_of[s] = -1
+            I.ox_uc[s] = 0.0
+            I.oy_uc[s] = 0.0
+        for s in range(self.unitcell.x * self.unitcell.y):
+            y, x = divmod(s, self.unitcell.x)
+            if self.in_plane:
+                I.ox_uc[s] = float(self.orientation[y, x, 0])
+                I.oy_uc[s] = float(self.orientation[y, x, 1])
+        dip = self.get_energy('dipolar', verbose=False)
+        if dip is not None and d.kern is not None:
+            k = d.kern
+            I.dip_pf = float(dip.prefactor)
+            I.K32, I.K64 = P(k["K32"]), P(k["K64"])
+            I.Kpo32, I.Kps64 = P(k["Kpo32"]), P(k["Kps64"])
+            I.kwx, I.kwy, I.n_sub = k["kwx"], k["kwy"], k["n_sub"]
+            for n, (y, x) in enumerate(self._kernel_sites):
+                I.sub_of[y * self.unitcell.x + x] = n
+        else:
+            I.dip_pf = 0.0
+            I.K32 = I.K64 = I.Kpo32 = I.Kps64 = None
+            I.kwx = I.kwy = I.n_sub = 0
+        ex = self.get_energy('exchange', verbose=False)
+        if ex is not None and ex.J != 0:
+            nn = np.asarray(ex.local_interaction, dtype=float)
+            if nn.shape[0] % 2 == 0 or nn.shape[1] % 2 == 0 or nn.size > _lib.HS_MAX_NN:
+                raise ValueError(f"Exchange mask of shape {nn.shape} is not supported (odd sides, <= {_lib.HS_MAX_NN} entries).")
+            I.J = float(ex.J)
+            I.nn_h, I.nn_w = nn.shape
+            for q in range(_lib.HS_MAX_NN):
+                I.nn_mask[q] = float(nn.reshape(-1)[q]) if q < nn.size else 0.0
+        else:
+            I.J = 0.0
+            I.nn_h = I.nn_w = 0
+        z = self.get_energy('zeeman', verbose=False)
+        I.has_zeeman = int(z is not None)
+        d.zee_bx = d.zee_by = None
+        I.zee_bx = I.zee_by = None
+        I.zee_Bx = I.zee_By = 0.0
+        if z is not None:
+            bx, by, uniform = z.field_components()
+            if uniform:
+                I.zee_Bx, I.zee_By = float(bx), float(by)
+            else:
+                t = d.torch
+                d.zee_bx = t.from_numpy(np.ascontiguousarray(bx, dtype=np.float32).reshape(-1)).to(d.device)
+                d.zee_by = t.from_numpy(np.ascontiguousarray(by, dtype=np.float32).reshape(-1)).to(d.device)
+                I.zee_bx, I.zee_by = P(d.zee_bx), P(d.zee_by)
+
+    def _device_recompute(self):
+        """FP64 full recompute of the combined fields (Energy.update for every energy)."""
+        if not hasattr(self, "_dev"):
+            return
+        self._dev.recompute(3 if self.USE_PERP_ENERGY else 1)
+        self._since_recompute = 0
+
+    def _device_zeeman_changed(self, z: ZeemanEnergy, old):
+        """ZeemanEnergy.set_field: swap the Zeeman term of the combined device field."""
+        if z not in self._energies:
+            return
+        bx0, by0, uniform0 = old
+        keep = None
+        if uniform0:
+            args = (bx0, by0, None, None)
+        else:
+            t = self._dev.torch
+            keep = (t.from_numpy(np.ascontiguousarray(bx0, dtype=np.float32).reshape(-1)).to(self._dev.device),
+                    t.from_numpy(np.ascontiguousarray(by0, dtype=np.float32).reshape(-1)).to(self._dev.device))
+            args = (0.0, 0.0) + keep
+        self._device_params_changed()
+        self._dev.zeeman_swap(*args)
+        if keep is not None:
+            self._dev.torch.cuda.current_stream(self._dev.device).synchronize()
+
+    # ---- stepping ------------------------------------------------------------------------------
+    def _pre_step(self):
+        self._sync_m()
+        self._sync_EB()
+
+    def update(self, *args, **kwargs):
+        """core.py:604-622: one step of the configured scheme."""
+        if self._T_any_zero:
+            warnings.warn("T=0 somewhere in the domain, so no switch will be simulated to prevent DIV/0 errors.", stacklevel=2)
+            return None
+        match self.params.UPDATE_SCHEME:
+            case Scheme.NEEL:
+                return self._update_Néel(*args, **kwargs)
+            case Scheme.METROPOLIS:
+                return self._update_Metropolis(*args, **kwargs)
+            case Scheme.WOLFF:
+                raise NotImplementedError("The Wolff cluster update has no device implementation.")
+            case other:
+                raise ValueError(f"The Magnets.params object has an invalid value for UPDATE_SCHEME='{other}'.")
+
+    def _update_Néel(self, t_max=1, attempt_freq=1e10):
+        """core.py:679-709: returns the (y, x) divmod tuple of the switched magnet, or None."""
+        self._pre_step()
+        self._maybe_recompute()
+        run = _lib.Run(t_max=float(t_max), mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1,
+                       nblocks=0, begin=1)
+        if not self._uniform_beta:
+            raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
+        self._dev.neel_run(run)
+        self._after_device_step()
+        idx = int(self._dev.log_idx[0].item())
+        st = self._state()
+        if st.err:
+            raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
+        if idx < 0:
+            return None
+        self._since_recompute += 1
+        return divmod(np.int64(idx), np.int64(self.nx))
+
+    def _update_Metropolis(self, idx=None, Q=0.05, r=0, attempt_freq=1e10):
+        """core.py:659-677: returns idx_switch, a (2, a) array of [y; x]."""
+        if idx is not None:
+            raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
+        self._pre_step()
+        sel = self._select_geometry(self.calc_r(Q) if r == 0 else r)
+        cap = max(1, sel.nsc)
+        t = self._dev.torch
+        cells = t.zeros(cap, dtype=t.int32, device=self._dev.device)
+        count = t.zeros(1, dtype=t.int32, device=self._dev.device)
+        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1, nblocks=0,
+                       begin=1)
+        self._dev.metropolis_run(sel, run, cells, count)
+        self._after_device_step()
+        a = int(count.item())
+        st = self._state()
+        if st.err:
+            raise _lib.HotspiceError(f"device Metropolis step failed (err={st.err})")
+        c = cells[:a].cpu().numpy().astype(np.int64)
+        return np.asarray([c // self.nx, c % self.nx])
+
+    def _maybe_recompute(self):
+        if self.recompute_every and self._since_recompute >= self.recompute_every and self._incremental_exact():
+            self._device_recompute()
+
+    def _incremental_exact(self):
+        """True when incremental updates reproduce a full recompute (no kernel truncation)."""
+        if self.params.UPDATE_SCHEME != Scheme.METROPOLIS:
+            return True
+        k = self.params.REDUCED_KERNEL_SIZE
+        kern = self._dev.kern
+        if k == 0 or kern is None:
+            return True
+        return min(k, self.nx - 1) >= kern["kwx"] and min(k, self.ny - 1) >= kern["kwy"]
+
+    def calc_r(self, Q, as_scalar=True):
+        """core.py:720-733 (cached per Q like the reference's lru_cache)."""
+        key = (Q, as_scalar)
+        if key not in self._calc_r_cache:
+            with np.errstate(divide='ignore'):
+                r = (8e-7 * self._momentSq / (Q * self.kBT)) ** (1 / 3)
+            r = np.clip(r, 0, (self.x_max - self.x_min) + (self.y_max - self.y_min))
+            self._calc_r_cache[key] = float(np.max(r)) if as_scalar else r
+        return self._calc_r_cache[key]
+
+    def _select_geometry(self, r):
+        """Supergrid of _select_grid (core.py:505-540) for distance r: sizes and the occupied-cell
+        pattern of one supercell for every unit-cell offset, uploaded once per (r, occupation)."""
+        scheme = self.params.MULTISAMPLING_SCHEME
+        if scheme not in ('grid', 'single'):
+            raise NotImplementedError(f"MULTISAMPLING_SCHEME='{scheme}' has no device implementation.")
+        key = (float(r), scheme, self.params.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF,
+               self.params.REDUCED_KERNEL_SIZE, bool(self.params.METROPOLIS_TIME))
+        if self._sel_key == key:
+            return self._sel
+        sel = _lib.Select()
+        sel.cutoff = self.params.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF
+        sel.reduced_k = self.params.REDUCED_KERNEL_SIZE
+        sel.metropolis_time = int(bool(self.params.METROPOLIS_TIME))
+        t = self._dev.torch
+        if scheme == 'single':
+            sel.scheme = 1
+            sel.nsc = 1
+            self._sel_tensors = ()
+        else:
+            sel.scheme = 0
+            Rx = math.ceil(r / np.min(self.dx)) - 1
+            Ry = math.ceil(r / np.min(self.dy)) - 1
+            Rx = self.unitcell.x * math.ceil(Rx / self.unitcell.x)
+            Ry = self.unitcell.y * math.ceil(Ry / self.unitcell.y)
+            Rx, Ry = min(Rx, self.nx), min(Ry, self.ny)
+            Rx, Ry = max(Rx, 2), max(Ry, 2)
+            snx, sny = self.nx // Rx + 1, self.ny // Ry + 1
+            ncols = len(range(0, min(snx, self.nx), 2))
+            nrows = len(range(0, min(sny, self.ny), 2))
+            offs, cells = [0], []
+            for dy in range(self.unitcell.y):
+                for dx in range(self.unitcell.x):
+                    occ = self.occupation[dy:dy + Ry, dx:dx + Rx]
+                    oy, ox = np.nonzero(occ)
+                    if oy.size == 0:
+                        raise ValueError("A supercell of the selection grid holds no magnet.")
+                    cells.extend(((oy.astype(np.int64) << 16) | ox).tolist())
+                    offs.append(len(cells))
+            t_off = t.tensor(offs, dtype=t.int32, device=self._dev.device)
+            t_cells = t.tensor(cells, dtype=t.int32, device=self._dev.device)
+            self._sel_tensors = (t_off, t_cells)
+            sel.pat_off, sel.pat_cells = t_off.data_ptr(), t_cells.data_ptr()
+            sel.Rx, sel.Ry, sel.snx, sel.sny, sel.ncols, sel.nsc = Rx, Ry, snx, sny, ncols, ncols * nrows
+        self._sel = sel
+        self._sel_key = key
+        return sel
+
+    def progress(self, t_max: float = 1, MCsteps_max: float = 4., stepwise: bool = False, **kwargs):
+        """core.py:736-751. Runs on the device until dt >= t_max or dMCsteps >= MCsteps_max."""
+        if stepwise:
+            return self._progress_stepwise(t_max=t_max, MCsteps_max=MCsteps_max, **kwargs)
+        if self._T_any_zero:
+            warnings.warn("T=0 somewhere in the domain, so no switch will be simulated to prevent DIV/0 errors.", stacklevel=2)
+            # no update() advances t or MCsteps here; the reference would loop forever, so return at once
+            return 0.0, 0.0
+        self._pre_step()
+        scheme = self.params.UPDATE_SCHEME
+        if scheme == Scheme.WOLFF:
+            raise NotImplementedError("The Wolff cluster update has no device implementation.")
+        attempt_freq = kwargs.get('attempt_freq', 1e10)
+        st0 = self._state()
+        t0, a0 = st0.t, st0.attempts
+        begin = True
+        if scheme == Scheme.NEEL:
+            if kwargs.keys() - {'attempt_freq'}:
+                raise TypeError(f"unexpected Neel keywords {sorted(kwargs.keys() - {'attempt_freq'})}")
+            while True:
+                st = self._state()
+                done_mc = (st.attempts - a0)
+                if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
+                    break
+                if math.isfinite(MCsteps_max):
+                    chunk = max(1, int(math.ceil(MCsteps_max * self.n - done_mc)) + 1)
+                else:
+                    chunk = 4096
+                if math.isfinite(t_max):
+                    chunk = min(chunk, 4096)
+                if self.recompute_every:
+                    chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
+                run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
+                               max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
+                if not self._uniform_beta:
+                    raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
+                if begin:
+                    self._set_progress_origin(t0, a0)
+                self._dev.neel_run(run)
+                begin = False
+                st = self._state()
+                if st.err:
+                    raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
+                self._since_recompute += int(st.steps_done)
+                self._maybe_recompute()
+        else:
+            r = kwargs.get('r', 0)
+            Q = kwargs.get('Q', 0.05)
+            if kwargs.get('idx') is not None:
+                raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
+            sel = self._select_geometry(self.calc_r(Q) if r == 0 else r)
+            per_step = max(1.0, 0.25 * sel.nsc)
+            while True:
+                st = self._state()
+                if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
+                    break
+                if math.isfinite(MCsteps_max):
+                    chunk = int(math.ceil((MCsteps_max * self.n - (st.attempts - a0)) / per_step)) + 1
+                    chunk = max(1, min(chunk, 4096))
+                else:
+                    chunk = 1024
+                if self.recompute_every and self._incremental_exact():
+                    chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
+                run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
+                               max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
+                if begin:
+                    self._set_progress_origin(t0, a0)
+                self._dev.metropolis_run(sel, run)
+                begin = False
+                st = self._state()
+                if st.err:
+                    raise _lib.HotspiceError(f"device Metropolis step failed (err={st.err})")
+                self._since_recompute += int(st.steps_done)
+                self._maybe_recompute()
+        self._after_device_step()
+        st = self._state()
+        return st.t - t0, st.attempts / self.n - a0 / self.n
+
+    def _set_progress_origin(self, t0, a0):
+        st = self._state()
+        st.t0, st.attempts0 = t0, a0
+        self._dev.write_state(st)
+
+    def _progress_stepwise(self, t_max: float = 1, MCsteps_max: float = 4., **kwargs):
+        """core.py:753-760 (one device step per yield)."""
+        t_0, MCsteps_0 = self.t, self.MCsteps
+        while lower_than(dt := self.t - t_0, t_max) and lower_than(self.MCsteps - MCsteps_0, MCsteps_max):
+            if self.params.UPDATE_SCHEME == Scheme.NEEL:
+                self.update(t_max=(t_max - dt), **kwargs)
+            else:
+                self.update(**kwargs)
+            yield
+        return self.t - t_0, self.MCsteps - MCsteps_0
+
+    # ---- observables (host, on the downloaded state) ----------------------------------------------
+    @property
+    def m_avg_x(self) -> float:
+        return float(np.mean(np.multiply(self.m, self.orientation[:, :, 0]))) * self.nx * self.ny / self.n if self.in_plane else 0
+
+    @property
+    def m_avg_y(self) -> float:
+        return float(np.mean(np.multiply(self.m, self.orientation[:, :, 1]))) * self.nx * self.ny / self.n if self.in_plane else 0
+
+    @property
+    def m_avg(self) -> float:
+        return (self.m_avg_x ** 2 + self.m_avg_y ** 2) ** (1 / 2) if self.in_plane else float(np.mean(self.m))
+
+    @property
+    def m_avg_angle(self) -> float:
+        return math.atan2(self.m_avg_y, self.m_avg_x) if self.in_plane else 0. + (self.m_avg < 0) * math.pi
+
+    @property
+    def T_avg(self) -> float:
+        return float(np.mean(self.T))
+
+    @property
+    def E_B_avg(self) -> float:
+        return float(np.mean(self.E_B))
+
+    @property
+    def moment_avg(self) -> float:
+        return float(np.mean(self.moment))
+
+    def history_save(self, *, E_tot=None, t=None, T_avg=None, m_avg=None):
+        self.history.entry(self)
+        if E_tot is not None:
+            self.history.E[-1] = float(E_tot)
+        if t is not None:
+            self.history.t[-1] = float(t)
+        if T_avg is not None:
+            self.history.T[-1] = float(T_avg)
+        if m_avg is not None:
+            self.history.m[-1] = float(m_avg)
+
+    # ---- lattice hooks ---------------------------------------------------------------------------
+    @abstractmethod
+    def _get_angles(self):
+        return np.zeros_like(self.ixx)
+
+    @abstractmethod
+    def _get_occupation(self):
+        return np.ones_like(self.ixx)
+
+    @abstractmethod
+    def _get_groundstate(self):
+        return 'random'
+
+    @abstractmethod
+    def _get_nearest_neighbors(self):
+        return np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]])

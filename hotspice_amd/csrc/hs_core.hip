@@ -1,0 +1,471 @@
+// libhotspice_hip.so: error state, Philox streams, dipolar kernel construction, FP64 full
+// recompute of the combined local fields, per-energy fields, totals and the Zeeman field step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hs_device.h"
+#include "hs_internal.h"
+
+static thread_local std::string g_err;
+
+int hs_set_error(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void hs_clear_error() { g_err.clear(); }
+
+extern "C" int hs_version(void) { return 100; }
+
+extern "C" const char *hs_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hs_device_info(int32_t *count, char *name, int32_t name_len) {
+  int n = 0;
+  HS_HIP_TRY(hipGetDeviceCount(&n));
+  if (count) *count = n;
+  if (name && name_len > 0) {
+    name[0] = 0;
+    if (n > 0) {
+      hipDeviceProp_t p;
+      HS_HIP_TRY(hipGetDeviceProperties(&p, 0));
+      snprintf(name, name_len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    }
+  }
+  return HS_OK;
+}
+
+#include <cstddef>
+
+// sizeof / offsetof of the ABI structs, so bindings can verify their mirrors without a device.
+extern "C" int hs_abi_layout(int64_t *out, int32_t n) {
+  const int64_t v[] = {
+      (int64_t)sizeof(hs_instance_t), (int64_t)offsetof(hs_instance_t, seed), (int64_t)offsetof(hs_instance_t, nx),
+      (int64_t)offsetof(hs_instance_t, sub_of), (int64_t)offsetof(hs_instance_t, nn_mask),
+      (int64_t)sizeof(hs_kernel_geom_t), (int64_t)offsetof(hs_kernel_geom_t, nx), (int64_t)offsetof(hs_kernel_geom_t, uc_o),
+      (int64_t)sizeof(hs_run_t), (int64_t)offsetof(hs_run_t, max_steps), (int64_t)offsetof(hs_run_t, begin),
+      (int64_t)sizeof(hs_state_t), (int64_t)offsetof(hs_state_t, done),
+      (int64_t)sizeof(hs_select_t), (int64_t)offsetof(hs_select_t, metropolis_time)};
+  const int cnt = (int)(sizeof(v) / sizeof(v[0]));
+  if (!out || n < cnt) return cnt;
+  for (int i = 0; i < cnt; ++i) out[i] = v[i];
+  return cnt;
+}
+
+int hs_check_instance(const hs_instance_t *I) {
+  if (!I) return hs_set_error(HS_ERR_ARG, "instance is NULL");
+  if (I->nx <= 0 || I->ny <= 0) return hs_set_error(HS_ERR_ARG, "bad grid %dx%d", I->nx, I->ny);
+  if ((int64_t)I->nx * I->ny > (1ll << 30)) return hs_set_error(HS_ERR_ARG, "grid too large");
+  if (I->ux <= 0 || I->uy <= 0 || I->ux * I->uy > HS_MAX_UC)
+    return hs_set_error(HS_ERR_ARG, "unit cell %dx%d exceeds %d sites", I->ux, I->uy, HS_MAX_UC);
+  if (I->nn_w < 0 || I->nn_h < 0 || I->nn_w * I->nn_h > HS_MAX_NN || (I->nn_w && !(I->nn_w & 1)) ||
+      (I->nn_h && !(I->nn_h & 1)))
+    return hs_set_error(HS_ERR_ARG, "exchange mask %dx%d must be odd and <= %d entries", I->nn_w, I->nn_h, HS_MAX_NN);
+  if (!I->m || !I->flags || !I->E || !I->Ec || !I->EB || !I->beta || !I->mu || !I->occ_list)
+    return hs_set_error(HS_ERR_ARG, "missing state array");
+  if (I->use_perp && (!I->Ep || !I->Epc)) return hs_set_error(HS_ERR_ARG, "use_perp without E_perp arrays");
+  if (I->dip_pf != 0.0) {
+    if (!I->K32 || !I->K64) return hs_set_error(HS_ERR_ARG, "dipolar energy without kernels");
+    if (I->use_perp && (!I->Kpo32 || !I->Kps64)) return hs_set_error(HS_ERR_ARG, "perp fields without perp kernels");
+    if (I->kwx < 0 || I->kwx > I->nx - 1 || I->kwy < 0 || I->kwy > I->ny - 1)
+      return hs_set_error(HS_ERR_ARG, "kernel window %d,%d outside grid", I->kwx, I->kwy);
+    if (I->n_sub <= 0) return hs_set_error(HS_ERR_ARG, "no dipolar kernels");
+    for (int s = 0; s < I->ux * I->uy; ++s)
+      if (I->sub_of[s] >= I->n_sub) return hs_set_error(HS_ERR_ARG, "sub_of[%d] out of range", s);
+  }
+  if (I->n_occ <= 0 || I->n_occ > I->nx * I->ny) return hs_set_error(HS_ERR_ARG, "bad n_occ %d", I->n_occ);
+  return HS_OK;
+}
+
+// ---- Philox streams --------------------------------------------------------------------------
+__global__ void k_philox_u53(uint64_t seed, uint64_t step, uint32_t tag, int64_t index0, int64_t n,
+                             int open_zero, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = hs_u53(hs_bits53(seed, step, tag, (uint64_t)(index0 + i)), open_zero);
+}
+
+extern "C" int hs_philox_uniform53(uint64_t seed, uint64_t step, uint32_t tag, int64_t index0, int64_t n,
+                                   int32_t open_zero, double *out, void *stream) {
+  if (n < 0 || index0 < 0 || (n > 0 && !out)) return hs_set_error(HS_ERR_ARG, "bad philox arguments");
+  if (n == 0) return HS_OK;
+  const int bs = 256;
+  k_philox_u53<<<(unsigned)((n + bs - 1) / bs), bs, 0, (hipStream_t)stream>>>(seed, step, tag, index0, n,
+                                                                                open_zero, out);
+  HS_LAUNCH_CHECK("k_philox_u53");
+  return HS_OK;
+}
+
+// ---- dipolar kernels (energies.py:176-288) -----------------------------------------------------
+struct KernVals {
+  double k, ps, po;
+};
+
+// Unfolded kernel entry at full-matrix position (R, C) of kernel n, before the 1e-7 factor.
+HS_INLINE KernVals kern_entry(const hs_kernel_geom_t &g, int n, int R, int C) {
+  KernVals v{0.0, 0.0, 0.0};
+  if (R == g.ny - 1 && C == g.nx - 1) return v;  // centre: rr_sq = inf (energies.py:218)
+  const double rx = g.rrx[(size_t)n * (2 * g.nx - 1) + C];
+  const double ry = g.rry[(size_t)n * (2 * g.ny - 1) + R];
+  const double rr_sq = rx * rx + ry * ry;
+  if (rr_sq > g.cutoff * g.cutoff) return v;  // energies.py:219
+  const double rr_inv = pow(rr_sq, -0.5);
+  const double rinv3 = pow(rr_inv, -g.decay);
+  const double rinv5 = pow(rr_inv, -g.decay + 2.0);
+  const double ux = rx * rr_inv, uy = ry * rr_inv;
+  const int site1 = g.site_y[n] * g.ux + g.site_x[n];
+  const int site2 = ((R + g.shift_y[n]) % g.uy) * g.ux + ((C + g.shift_x[n]) % g.ux);
+  if (g.in_plane) {
+    const double ox1 = g.uc_ox[site1], oy1 = g.uc_oy[site1];
+    const double ox2 = g.uc_ox[site2], oy2 = g.uc_oy[site2];
+    const double a3x = 3 * (ux * ux) - 1, a3y = 3 * (uy * uy) - 1, a5x = 5 * (ux * ux) - 1, a5y = 5 * (uy * uy) - 1;
+    const double uxy = ux * uy, c32 = 3 * g.I / 2;
+    auto pair = [&](double p1, double q1, double p2, double q2) {
+      const double k = p1 * p2 * a3x + q1 * q2 * a3y + 3 * uxy * (p1 * q2 + q1 * p2);
+      const double kc = p1 * p2 * a5x + q1 * q2 * a5y + 5 * uxy * (p1 * q2 + q1 * p2);
+      return -rinv3 * k - c32 * rinv5 * kc;
+    };
+    v.k = pair(ox1, oy1, ox2, oy2);
+    v.ps = pair(-oy1, ox1, ox2, oy2);
+    v.po = pair(ox1, oy1, -oy2, ox2);
+  } else {
+    const double o2 = g.uc_o[site2];
+    v.k = o2 * (rinv3 + 9 * g.I / 2 * rinv5);
+  }
+  return v;
+}
+
+HS_INLINE void kv_add(KernVals &a, const KernVals &b) {
+  a.k += b.k;
+  a.ps += b.ps;
+  a.po += b.po;
+}
+
+__global__ void k_build_kernels(hs_kernel_geom_t g, double *K64, double *Kps64, double *Kpo64, float *K32,
+                                float *Kpo32) {
+  const int W = 2 * g.kwx + 1, H = 2 * g.kwy + 1;
+  const int64_t per = (int64_t)W * H;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * g.n_sub) return;
+  const int n = (int)(i / per);
+  const int rem = (int)(i % per);
+  const int r = rem / W, c = rem % W;
+  const int R = g.ny - 1 - g.kwy + r, C = g.nx - 1 - g.kwx + c;
+  const int nx = g.nx, ny = g.ny;
+  KernVals v = kern_entry(g, n, R, C);
+  if (g.pbc) {  // energies.py:264-273, statement order
+    if (C >= nx) kv_add(v, kern_entry(g, n, R, C - nx));
+    if (R >= ny && C >= nx) kv_add(v, kern_entry(g, n, R - ny, C - nx));
+    if (R >= ny) kv_add(v, kern_entry(g, n, R - ny, C));
+    if (R >= ny && C <= nx - 2) kv_add(v, kern_entry(g, n, R - ny, C + nx));
+    if (C <= nx - 2) kv_add(v, kern_entry(g, n, R, C + nx));
+    if (R <= ny - 2 && C <= nx - 2) kv_add(v, kern_entry(g, n, R + ny, C + nx));
+    if (R <= ny - 2) kv_add(v, kern_entry(g, n, R + ny, C));
+    if (R <= ny - 2 && C >= nx) kv_add(v, kern_entry(g, n, R + ny, C - nx));
+  }
+  v.k *= 1e-7;
+  v.ps *= 1e-7;
+  v.po *= 1e-7;
+  K64[i] = v.k;
+  K32[i] = (float)v.k;
+  if (Kps64) Kps64[i] = v.ps;
+  if (Kpo64) Kpo64[i] = v.po;
+  if (Kpo32) Kpo32[i] = (float)v.po;
+}
+
+extern "C" int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, double *Kps64, double *Kpo64,
+                                        float *K32, float *Kpo32, void *stream) {
+  hs_clear_error();
+  if (!g || !g->rrx || !g->rry || !K64 || !K32) return hs_set_error(HS_ERR_ARG, "missing kernel arrays");
+  if (g->n_sub <= 0 || g->n_sub > HS_MAX_UC || g->ux <= 0 || g->uy <= 0 || g->ux * g->uy > HS_MAX_UC)
+    return hs_set_error(HS_ERR_ARG, "bad unit cell / n_sub");
+  if (g->kwx < 0 || g->kwx > g->nx - 1 || g->kwy < 0 || g->kwy > g->ny - 1)
+    return hs_set_error(HS_ERR_ARG, "kernel window outside grid");
+  if (g->pbc && (g->kwx != g->nx - 1 || g->kwy != g->ny - 1))
+    return hs_set_error(HS_ERR_ARG, "PBC needs the full kernel window");
+  for (int n = 0; n < g->n_sub; ++n)
+    if (g->site_x[n] < 0 || g->site_x[n] >= g->ux || g->site_y[n] < 0 || g->site_y[n] >= g->uy ||
+        g->shift_x[n] < 0 || g->shift_y[n] < 0)
+      return hs_set_error(HS_ERR_ARG, "bad site %d", n);
+  const int64_t total = (int64_t)(2 * g->kwx + 1) * (2 * g->kwy + 1) * g->n_sub;
+  const int bs = 256;
+  k_build_kernels<<<(unsigned)((total + bs - 1) / bs), bs, 0, (hipStream_t)stream>>>(*g, K64, Kps64, Kpo64, K32, Kpo32);
+  HS_LAUNCH_CHECK("k_build_kernels");
+  return HS_OK;
+}
+
+// ---- per-cell closed forms: Zeeman factors and exchange sums ------------------------------------
+HS_INLINE void zeeman_factors(const hs_instance_t &I, int c, int site, double Bx, double By, double &f,
+                              double &fp) {
+  const double mu = I.mu[c];
+  if (I.in_plane) {
+    const double ox = I.ox_uc[site], oy = I.oy_uc[site];
+    f = -mu * (Bx * ox + By * oy);   // energies.py:108
+    fp = -mu * (By * ox - Bx * oy);  // energies.py:109
+  } else {
+    f = -mu * Bx;  // energies.py:112
+    fp = 0.0;
+  }
+}
+
+HS_INLINE void field_at(const hs_instance_t &I, int c, double &Bx, double &By) {
+  Bx = I.zee_bx ? (double)I.zee_bx[c] : I.zee_Bx;
+  By = I.zee_by ? (double)I.zee_by[c] : I.zee_By;
+}
+
+// convolve2d(., NN, 'same', fill|wrap) sums at cell (y, x): out = sum_q NN[q] * v[i + h - q].
+HS_INLINE void exchange_sums(const hs_instance_t &I, int y, int x, double &sx, double &sy) {
+  sx = 0.0;
+  sy = 0.0;
+  const int hy = I.nn_h / 2, hx = I.nn_w / 2;
+  for (int qy = 0; qy < I.nn_h; ++qy)
+    for (int qx = 0; qx < I.nn_w; ++qx) {
+      const float wq = I.nn_mask[qy * I.nn_w + qx];
+      if (wq == 0.f) continue;
+      int yy = y + hy - qy, xx = x + hx - qx;
+      if (I.pbc) {
+        yy = hs_pmod(yy, I.ny);
+        xx = hs_pmod(xx, I.nx);
+      } else if (yy < 0 || yy >= I.ny || xx < 0 || xx >= I.nx) {
+        continue;
+      }
+      const int cc = yy * I.nx + xx;
+      const double mj = (double)I.m[cc];
+      if (I.in_plane) {
+        const int s = hs_site(I, yy, xx);
+        sx += wq * (I.ox_uc[s] * mj);
+        sy += wq * (I.oy_uc[s] * mj);
+      } else {
+        sx += wq * mj;
+      }
+    }
+}
+
+// Exchange energy and E_perp of cell c (energies.py:771-781).
+HS_INLINE void exchange_fields(const hs_instance_t &I, int y, int x, double mi, int site, double &e, double &ep) {
+  e = 0.0;
+  ep = 0.0;
+  if (I.J == 0.0 || mi == 0.0) return;
+  double sx, sy;
+  exchange_sums(I, y, x, sx, sy);
+  if (I.in_plane) {
+    const double mx = I.ox_uc[site] * mi, my = I.oy_uc[site] * mi;
+    e = -I.J * (sx * mx + sy * my);
+    ep = -I.J * (sy * mx - sx * my);
+  } else {
+    e = -I.J * (sx * mi);
+  }
+}
+
+// ---- FP64 full recompute (energies.py:290-308 + Zeeman/Exchange update) ------------------------
+constexpr int RC_TILE = 256;
+
+__global__ void __launch_bounds__(256) k_recompute(hs_instance_t I, int which) {
+  __shared__ int s_pos[RC_TILE];
+  __shared__ double s_mmu[RC_TILE];
+  const int ncell = I.nx * I.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = c < ncell;
+  int y = 0, x = 0, site = 0, n = -1;
+  double mi = 0.0;
+  if (active) {
+    y = c / I.nx;
+    x = c % I.nx;
+    site = hs_site(I, y, x);
+    n = I.dip_pf != 0.0 ? I.sub_of[site] : -1;
+    mi = (double)I.m[c];
+  }
+  const bool do_e = which & 1, do_p = (which & 2) && I.use_perp;
+  double acc = 0.0, accp = 0.0;
+  const int KW = 2 * I.kwx + 1;
+  const size_t KS = (size_t)KW * (2 * I.kwy + 1);
+  // dipolar: sum over occupied sources j of K_{n_i}[centre + (j - i)] * (m mu)_j
+  if (I.dip_pf != 0.0) {
+    for (int base = 0; base < I.n_occ; base += RC_TILE) {
+      __syncthreads();
+      const int j = base + threadIdx.x;
+      if (j < I.n_occ) {
+        const int cj = I.occ_list[j];
+        s_pos[threadIdx.x] = cj;
+        s_mmu[threadIdx.x] = (double)I.m[cj] * (double)I.mu[cj];
+      }
+      __syncthreads();
+      const int cnt = min(RC_TILE, I.n_occ - base);
+      if (active && n >= 0 && mi != 0.0) {
+        const double *Kn = I.K64 + (size_t)n * KS;
+        const double *Kp = do_p ? I.Kps64 + (size_t)n * KS : nullptr;
+        for (int t = 0; t < cnt; ++t) {
+          const int cj = s_pos[t];
+          const int dy = cj / I.nx - y, dx = cj % I.nx - x;
+          if (dy < -I.kwy || dy > I.kwy || dx < -I.kwx || dx > I.kwx) continue;
+          const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
+          const double sm = s_mmu[t];
+          if (do_e) acc += Kn[o] * sm;
+          if (do_p) accp += Kp[o] * sm;
+        }
+      }
+    }
+  }
+  if (!active) return;
+  const double mu = I.mu[c];
+  double e = 0.0, ep = 0.0;
+  if (mi != 0.0) {
+    // dipolar: prefactor*moment*(partial_m*conv)  (energies.py:303, 307)
+    if (n >= 0) {
+      e += (I.dip_pf * mu) * (mi * acc);
+      ep += (I.dip_pf * mu) * (mi * accp);
+    }
+    if (I.has_zeeman) {
+      double Bx, By, f, fp;
+      field_at(I, c, Bx, By);
+      zeeman_factors(I, c, site, Bx, By, f, fp);
+      e += mi * f;
+      ep += mi * fp;
+    }
+    double xe, xp;
+    exchange_fields(I, y, x, mi, site, xe, xp);
+    e += xe;
+    ep += xp;
+  }
+  if (do_e) {
+    const float hi = (float)e;
+    I.E[c] = hi;
+    I.Ec[c] = (float)((double)hi - e);
+  }
+  if (do_p) {
+    const float hi = (float)ep;
+    I.Ep[c] = hi;
+    I.Epc[c] = (float)((double)hi - ep);
+  }
+}
+
+extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  const int ncell = inst->nx * inst->ny;
+  k_recompute<<<(ncell + 255) / 256, 256, 0, (hipStream_t)stream>>>(*inst, which);
+  HS_LAUNCH_CHECK("k_recompute");
+  return HS_OK;
+}
+
+// ---- per-energy fields -------------------------------------------------------------------------
+__global__ void k_field_components(hs_instance_t I, double *out) {
+  const int ncell = I.nx * I.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int y = c / I.nx, x = c % I.nx, site = hs_site(I, y, x);
+  const double mi = (double)I.m[c];
+  double zf = 0.0, zp = 0.0, xe = 0.0, xp = 0.0;
+  if (I.has_zeeman && mi != 0.0) {
+    double Bx, By, f, fp;
+    field_at(I, c, Bx, By);
+    zeeman_factors(I, c, site, Bx, By, f, fp);
+    zf = mi * f;
+    zp = mi * fp;
+  }
+  exchange_fields(I, y, x, mi, site, xe, xp);
+  out[0 * (size_t)ncell + c] = zf;
+  out[1 * (size_t)ncell + c] = I.use_perp ? zp : 0.0;
+  out[2 * (size_t)ncell + c] = xe;
+  out[3 * (size_t)ncell + c] = I.use_perp ? xp : 0.0;
+  out[4 * (size_t)ncell + c] = (double)I.E[c] - (double)I.Ec[c];
+  out[5 * (size_t)ncell + c] = I.use_perp ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
+}
+
+extern "C" int hs_field_components(const hs_instance_t *inst, double *out, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!out) return hs_set_error(HS_ERR_ARG, "out is NULL");
+  const int ncell = inst->nx * inst->ny;
+  k_field_components<<<(ncell + 255) / 256, 256, 0, (hipStream_t)stream>>>(*inst, out);
+  HS_LAUNCH_CHECK("k_field_components");
+  return HS_OK;
+}
+
+// ---- total energy: 0.5*sum(E) over pair energies + sum(E_zeeman) = 0.5*sum(E) + 0.5*sum(E_z) ----
+__global__ void __launch_bounds__(1024) k_energy_total(hs_instance_t I, double *out) {
+  __shared__ double s[16];
+  const int ncell = I.nx * I.ny;
+  double acc = 0.0;
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const double mi = (double)I.m[c];
+    double v = (double)I.E[c] - (double)I.Ec[c];
+    if (I.has_zeeman && mi != 0.0) {
+      double Bx, By, f, fp;
+      field_at(I, c, Bx, By);
+      zeeman_factors(I, c, hs_site(I, c / I.nx, c % I.nx), Bx, By, f, fp);
+      v += mi * f;
+    }
+    acc += 0.5 * v;
+  }
+  acc = hs_wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s[w];
+    out[0] = t;
+  }
+}
+
+extern "C" int hs_energy_total(const hs_instance_t *inst, double *out_host, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!out_host) return hs_set_error(HS_ERR_ARG, "out is NULL");
+  double *d = nullptr;
+  HS_HIP_TRY(hipMallocAsync((void **)&d, sizeof(double), (hipStream_t)stream));
+  k_energy_total<<<1, 1024, 0, (hipStream_t)stream>>>(*inst, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out_host, d, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream);
+  hipError_t e2 = hipFreeAsync(d, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return hs_set_error(HS_ERR_HIP, "hs_energy_total: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return hs_set_error(HS_ERR_HIP, "hs_energy_total free: %s", hipGetErrorString(e2));
+  return HS_OK;
+}
+
+// ---- Zeeman field step (energies.py:101-118): E += m*(f_new - f_old) ----------------------------
+__global__ void k_zeeman_swap(hs_instance_t I, double Bx0, double By0, const float *bx0, const float *by0) {
+  const int ncell = I.nx * I.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const double mi = (double)I.m[c];
+  if (mi == 0.0) return;
+  const int site = hs_site(I, c / I.nx, c % I.nx);
+  double Bx, By, f1, fp1, f0, fp0;
+  field_at(I, c, Bx, By);
+  zeeman_factors(I, c, site, Bx, By, f1, fp1);
+  zeeman_factors(I, c, site, bx0 ? (double)bx0[c] : Bx0, by0 ? (double)by0[c] : By0, f0, fp0);
+  float s = I.E[c], cc = I.Ec[c];
+  hs_kahan_add(s, cc, (float)(mi * (f1 - f0)));
+  I.E[c] = s;
+  I.Ec[c] = cc;
+  if (I.use_perp) {
+    float sp = I.Ep[c], cp = I.Epc[c];
+    hs_kahan_add(sp, cp, (float)(mi * (fp1 - fp0)));
+    I.Ep[c] = sp;
+    I.Epc[c] = cp;
+  }
+}
+
+extern "C" int hs_zeeman_swap(const hs_instance_t *inst, double Bx0, double By0, const float *bx_old,
+                              const float *by_old, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  const int ncell = inst->nx * inst->ny;
+  k_zeeman_swap<<<(ncell + 255) / 256, 256, 0, (hipStream_t)stream>>>(*inst, Bx0, By0, bx_old, by_old);
+  HS_LAUNCH_CHECK("k_zeeman_swap");
+  return HS_OK;
+}

@@ -1,0 +1,27 @@
+// Host-side helpers shared by the translation units of libhotspice_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/hotspice_hip.h"
+
+int hs_set_error(int code, const char *fmt, ...);
+void hs_clear_error();
+
+#define HS_HIP_TRY(expr)                                                                      \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return hs_set_error(HS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define HS_LAUNCH_CHECK(name)                                                                 \
+  do {                                                                                        \
+    hipError_t e_ = hipGetLastError();                                                        \
+    if (e_ != hipSuccess) return hs_set_error(HS_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e_)); \
+  } while (0)
+
+// Validates the fields every stepping kernel relies on (sizes, pointers, table bounds).
+int hs_check_instance(const hs_instance_t *I);
+
+// Run-state helpers (hs_neel.hip): reset per-call fields of state[0]; copy state[from] to state[0].
+__global__ void k_state_begin(hs_state_t *st, int begin);
+__global__ void k_state_copy(hs_state_t *st, int from);

@@ -1,0 +1,480 @@
+// Metropolis stepping with grid multisampling (Magnets._update_Metropolis, core.py:659-677).
+//
+// Two launches per step:
+//   k_metro_select (one workgroup): the stop predicate of progress(), the supergrid selection
+//     of _select_grid (core.py:496-558) with its retries (core.py:484-487) and the
+//     _select_single fallback (core.py:489-494, 555-558), acceptance min(1, e^{-beta dE})
+//     clipped to [1e-10, 1e10] (core.py:666, 671), optional METROPOLIS_TIME (core.py:667-669),
+//     the flips with E[w] *= -1 (energies.py:124-126, 332-334), and the order-preserving list of
+//     accepted switches with per-sublattice counts (energies.py:335-343);
+//   k_metro_update (one workgroup per 32x8 output tile): the sparse-source correlation of the
+//     accepted switches with the dipolar kernel -- full window, or the (2k+1)^2 REDUCED_KERNEL_SIZE
+//     window with fill/wrap boundary when a sublattice has more than
+//     SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF switches (energies.py:343-361) -- plus the
+//     exchange stencil, added once per cell with Kahan compensation (energies.py:362-366).
+//     Sources are staged in LDS and filtered per tile; tiles no source reaches return at once.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "hs_device.h"
+#include "hs_internal.h"
+
+namespace {
+
+constexpr int MS_THREADS = 1024;
+constexpr int MU_TX = 32, MU_TY = 8, MU_THREADS = MU_TX * MU_TY;
+constexpr int MAX_ATTEMPTS = 4096;
+
+struct MetroWS {
+  int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
+  int8_t *accf;      // [nsc] accepted flag
+  int32_t *acc_cell; // [nsc] accepted cells, selection order
+  float *acc_mmu;    // [nsc] (m mu) after the flip
+  int32_t *acc_n;    // [1]  number accepted this step
+  int32_t *trunc;    // [HS_MAX_UC] per unit-cell site: 1 = truncated window (convolution branch)
+};
+
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+MetroWS carve(void *base, int nsc) {
+  char *p = (char *)base;
+  MetroWS w;
+  const size_t n = (size_t)(nsc > 0 ? nsc : 1);
+  w.cand = (int32_t *)p;
+  p += align256(n * 4);
+  w.accf = (int8_t *)p;
+  p += align256(n);
+  w.acc_cell = (int32_t *)p;
+  p += align256(n * 4);
+  w.acc_mmu = (float *)p;
+  p += align256(n * 4);
+  w.acc_n = (int32_t *)p;
+  p += 256;
+  w.trunc = (int32_t *)p;
+  p += align256(HS_MAX_UC * 4);
+  return w;
+}
+
+size_t ws_bytes(int nsc) {
+  const size_t n = (size_t)(nsc > 0 ? nsc : 1);
+  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4);
+}
+
+// Block-wide sum of an int (1024 threads).
+HS_INLINE int block_sum_int(int v, int *s_tmp) {
+  v = hs_wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_tmp[w];
+  return t;
+}
+
+HS_INLINE double block_sum_double(double v, double *s_tmp) {
+  v = hs_wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_tmp[w];
+  return t;
+}
+
+// Exclusive prefix sum over the block (in thread order).
+HS_INLINE int block_exclusive_scan(int v, int *s_tmp, int &total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  __syncthreads();
+  if (lane == 63) s_tmp[wid] = incl;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wid) base += s_tmp[w];
+    total += s_tmp[w];
+  }
+  return base + incl - v;
+}
+
+// min-barrier of a sample for METROPOLIS_TIME (E_barrier(idx, min_only=True), core.py:625-656), FP64.
+HS_INLINE double barrier_min_f64(const hs_instance_t &I, int c) {
+  const double e = (double)I.E[c] - (double)I.Ec[c];
+  const double dE = -2.0 * e, El = e, eb = (double)I.EB[c];
+  if (I.use_perp) {
+    const double ep = I.m_perp_factor * ((double)I.Ep[c] - (double)I.Epc[c]);
+    return fmin(fmax(dE, (eb + ep) - El), fmax(dE, (eb - ep) - El));
+  }
+  if (I.barrier == 0) return fmax(dE, eb - El);
+  const double q = dE * (0.25 / eb) + 1.0;
+  return dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
+}
+
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select(hs_instance_t I, hs_select_t SEL, hs_run_t R,
+                                                             hs_state_t *st, MetroWS ws, int32_t *log_cells,
+                                                             int32_t *log_count) {
+  __shared__ int s_i[MS_THREADS / 64];
+  __shared__ double s_d[MS_THREADS / 64];
+  __shared__ int s_site[HS_MAX_UC];
+  __shared__ int s_single;
+  const int tid = threadIdx.x;
+  hs_state_t S = st[0];
+  const double n = (double)I.n_occ;
+  if (tid < HS_MAX_UC) s_site[tid] = 0;
+  bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
+            hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  if (!go) {
+    __syncthreads();
+    if (tid == 0) {
+      *ws.acc_n = 0;
+      if (!S.done) {
+        S.done = 1;
+        st[0] = S;
+      }
+    }
+    return;
+  }
+  const uint64_t step = (uint64_t)S.step;
+  const int nsc = SEL.nsc;
+  const int q = (nsc + MS_THREADS - 1) / MS_THREADS;
+  const int j0 = min(nsc, tid * q), j1 = min(nsc, j0 + q);
+  const int nx = I.nx, ny = I.ny;
+  int nvalid = 0;
+  bool single = false;
+  int attempt = 0;
+  for (; attempt < MAX_ATTEMPTS; ++attempt) {
+    const uint64_t b0 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0));
+    const uint64_t b1 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1));
+    const uint64_t b2 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 2));
+    if (SEL.scheme == 0) {
+      int ox, oy;
+      if (I.pbc) {  // np.random.randint(nx), randint(ny) (core.py:527)
+        ox = (int)floor(hs_u53(b0, 0) * nx);
+        oy = (int)floor(hs_u53(b1, 0) * ny);
+      } else {  // randint(-Rx, Rx), randint(-Ry, Ry) (core.py:529)
+        ox = -SEL.Rx + (int)floor(hs_u53(b0, 0) * (2.0 * SEL.Rx));
+        oy = -SEL.Ry + (int)floor(hs_u53(b1, 0) * (2.0 * SEL.Ry));
+      }
+      if (SEL.Rx == nx) ox = 0;  // core.py:530-531
+      if (SEL.Ry == ny) oy = 0;
+      const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
+      const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
+      int inb = 0, val = 0;
+      for (int j = j0; j < j1; ++j) {
+        const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u | ((uint32_t)attempt << 8), (uint64_t)j), 0) * cnt);
+        const int pc = SEL.pat_cells[p0 + pick];
+        const int sy = 2 * (j / SEL.ncols), sx = 2 * (j % SEL.ncols);
+        int x = sx * SEL.Rx + (pc & 0xffff), y = sy * SEL.Ry + (pc >> 16);
+        bool ok;
+        if (I.pbc) {
+          ok = (x < max(SEL.Rx, nx - SEL.Rx)) && (y < max(SEL.Ry, ny - SEL.Ry));  // core.py:545
+          x = (x + ox) % nx;
+          y = (y + oy) % ny;
+        } else {
+          x += ox;
+          y += oy;
+          ok = x >= 0 && x < nx && y >= 0 && y < ny;  // core.py:552
+        }
+        const bool v = ok && (I.flags[y * nx + x] & HS_FLAG_OCC);
+        ws.cand[j] = v ? y * nx + x : -1;
+        inb += ok;
+        val += v;
+      }
+      const int tot_inb = block_sum_int(inb, s_i);
+      const int tot_val = block_sum_int(val, s_i);
+      if (tot_inb > 0) {
+        if (tot_val > 0) {
+          nvalid = tot_val;
+          break;
+        }
+        continue;  // nothing occupied: select() again (core.py:487)
+      }
+    }
+    // _select_single: np.random.randint(self.n) over the initial nonzero list (core.py:493)
+    const int i = (int)floor(hs_u53(b2, 0) * n);
+    const int cell = I.occ_list[min(i, I.n_occ - 1)];
+    if (I.flags[cell] & HS_FLAG_OCC) {
+      if (tid == 0) s_single = cell;
+      single = true;
+      nvalid = 1;
+      break;
+    }
+  }
+  __syncthreads();
+  if (attempt >= MAX_ATTEMPTS) {
+    if (tid == 0) {
+      S.err = 2;
+      S.done = 1;
+      st[0] = S;
+      *ws.acc_n = 0;
+    }
+    return;
+  }
+  // acceptance (core.py:662-671); with a single sample thread 0 plays candidate 0
+  int my_acc = 0;
+  double tsum = 0.0;
+  const int a0 = single ? (tid == 0 ? 0 : 1) : j0, a1 = single ? 1 : j1;
+  for (int j = a0; j < a1; ++j) {
+    const int c = single ? s_single : ws.cand[j];
+    bool acc = false;
+    if (c >= 0) {
+      const double e = (double)I.E[c] - (double)I.Ec[c];
+      const double beta = (double)I.beta[c];
+      double p = exp(2.0 * e * beta);  // exp(-switch_energy*beta), switch_energy = -2E
+      p = fmin(fmax(p, 1e-10), 1e10);
+      if (SEL.metropolis_time) {
+        const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
+        tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * beta);
+      }
+      const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
+      acc = ua < p;
+    }
+    if (single) {
+      if (tid == 0) s_single = acc ? c : -1;
+    } else {
+      ws.accf[j] = acc ? 1 : 0;
+    }
+    my_acc += acc;
+  }
+  if (SEL.metropolis_time) tsum = block_sum_double(tsum, s_d);
+  int total = 0;
+  int pos = block_exclusive_scan(my_acc, s_i, total);
+  __syncthreads();
+  for (int j = a0; j < a1; ++j) {
+    int c;
+    if (single) {
+      c = s_single;
+    } else {
+      c = ws.accf[j] ? ws.cand[j] : -1;
+    }
+    if (c < 0) continue;
+    const int mnew = -(int)I.m[c];
+    I.m[c] = (int8_t)mnew;
+    I.E[c] = -I.E[c];
+    I.Ec[c] = -I.Ec[c];
+    if (I.use_perp) {
+      I.Ep[c] = -I.Ep[c];
+      I.Epc[c] = -I.Epc[c];
+    }
+    ws.acc_cell[pos] = c;
+    ws.acc_mmu[pos] = (float)mnew * I.mu[c];
+    atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
+    if (R.log) log_cells[(size_t)S.steps_done * (size_t)(nsc > 0 ? nsc : 1) + pos] = c;
+    ++pos;
+  }
+  __syncthreads();
+  if (tid < I.ux * I.uy) {
+    // update_single for one switch (core.py:250-251) uses the full kernel; update_multiple
+    // truncates per sublattice above the cutoff when REDUCED_KERNEL_SIZE != 0 (energies.py:343-349)
+    ws.trunc[tid] = (total > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
+  }
+  if (tid == 0) {
+    *ws.acc_n = total;
+    if (R.log) log_count[S.steps_done] = total;
+    S.attempts += nvalid;
+    S.switches += total;
+    S.step += 1;
+    S.steps_done += 1;
+    if (SEL.metropolis_time) S.t += tsum;
+    st[0] = S;
+  }
+}
+
+// Displacement candidates d = i - s (+k*n) with |d| <= w: wrap enumerates images, fill only k = 0.
+HS_INLINE int disp_candidates(int i, int s, int n, int w, bool wrap, int out[3]) {
+  int cnt = 0;
+  const int d = i - s;
+  if (!wrap) {
+    if (d >= -w && d <= w) out[cnt++] = d;
+    return cnt;
+  }
+  for (int k = -1; k <= 1; ++k) {
+    const int dd = d + k * n;
+    if (dd >= -w && dd <= w) out[cnt++] = dd;
+  }
+  return cnt;
+}
+
+// Can a source at s with half-width w (optionally wrapping over n) reach [lo, hi]?
+HS_INLINE bool reaches(int s, int w, int n, bool wrap, int lo, int hi) {
+  if (w < 0) return false;
+  if (!wrap) return s + w >= lo && s - w <= hi;
+  for (int k = -1; k <= 1; ++k)
+    if (s + k * n + w >= lo && s + k * n - w <= hi) return true;
+  return false;
+}
+
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update(hs_instance_t I, hs_select_t SEL, MetroWS ws) {
+  __shared__ int s_cell[MU_THREADS];
+  __shared__ float s_mmu[MU_THREADS];
+  __shared__ int s_site_of[MU_THREADS];
+  __shared__ int s_cnt[MU_THREADS / 64 + 1];
+  const int a = *ws.acc_n;
+  if (a == 0) return;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * MU_TX, y0 = blockIdx.y * MU_TY;
+  const int x = x0 + (tid % MU_TX), y = y0 + (tid / MU_TX);
+  const int nx = I.nx, ny = I.ny;
+  const bool inside = x < nx && y < ny;
+  const int c = inside ? y * nx + x : 0;
+  const int mi = inside ? (int)I.m[c] : 0;
+  const bool has_dip = I.dip_pf != 0.0;
+  const int hy = I.nn_h / 2, hx = I.nn_w / 2;
+  const bool has_ex = I.J != 0.0 && I.nn_w > 0;
+  const int KW = 2 * I.kwx + 1;
+  const size_t KS = (size_t)KW * (2 * I.kwy + 1);
+  const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
+  const int site_i = hs_site(I, y % ny, x % nx);
+  const float oxi = I.ox_uc[site_i], oyi = I.oy_uc[site_i];
+  float acc = 0.f, accp = 0.f, ex = 0.f, exp_ = 0.f;
+  bool touched = false;
+  for (int base = 0; base < a; base += MU_THREADS) {
+    // stage this chunk's sources that reach the tile
+    const int sidx = base + tid;
+    bool rel = false;
+    int sc = 0, ssite = 0;
+    float smmu = 0.f;
+    if (sidx < a) {
+      sc = ws.acc_cell[sidx];
+      smmu = ws.acc_mmu[sidx];
+      const int sy = sc / nx, sx = sc % nx;
+      ssite = hs_site(I, sy, sx);
+      if (has_dip && I.sub_of[ssite] >= 0) {
+        const bool tr = ws.trunc[ssite] != 0;
+        const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
+        const bool wrap = tr && I.pbc;
+        rel = reaches(sx, wx, nx, wrap, x0, x0 + MU_TX - 1) && reaches(sy, wy, ny, wrap, y0, y0 + MU_TY - 1);
+      }
+      if (!rel && has_ex)
+        rel = reaches(sx, hx, nx, I.pbc, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, I.pbc, y0, y0 + MU_TY - 1);
+    }
+    const unsigned long long ball = __ballot(rel);
+    const int lane = tid & 63, wid = tid >> 6;
+    __syncthreads();
+    if (lane == 0) s_cnt[wid] = __popcll(ball);
+    __syncthreads();
+    int off = 0, nrel = 0;
+    for (int w = 0; w < MU_THREADS / 64; ++w) {
+      if (w < wid) off += s_cnt[w];
+      nrel += s_cnt[w];
+    }
+    if (rel) {
+      const int p = off + __popcll(ball & ((1ull << lane) - 1ull));
+      s_cell[p] = sc;
+      s_mmu[p] = smmu;
+      s_site_of[p] = ssite;
+    }
+    __syncthreads();
+    if (nrel == 0 || !inside || mi == 0) continue;
+    for (int r = 0; r < nrel; ++r) {
+      const int scell = s_cell[r];
+      const int sy = scell / nx, sx = scell % nx;
+      const int ss = s_site_of[r];
+      const float mm = s_mmu[r];
+      if (has_dip) {
+        const int n = I.sub_of[ss];
+        if (n >= 0) {
+          const bool tr = ws.trunc[ss] != 0;
+          const bool wrap = tr && I.pbc;
+          const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
+          int dys[3], dxs[3];
+          const int ndy = disp_candidates(y, sy, ny, wy, wrap, dys);
+          const int ndx = disp_candidates(x, sx, nx, wx, wrap, dxs);
+          const float *Kn = I.K32 + (size_t)n * KS;
+          const float *Kp = I.use_perp ? I.Kpo32 + (size_t)n * KS : nullptr;
+          for (int a1 = 0; a1 < ndy; ++a1)
+            for (int a2 = 0; a2 < ndx; ++a2) {
+              const size_t o = (size_t)(I.kwy + dys[a1]) * KW + (I.kwx + dxs[a2]);
+              acc += mm * Kn[o];
+              if (Kp) accp += mm * Kp[o];
+              touched = true;
+            }
+        }
+      }
+      if (has_ex) {
+        int dy = y - sy, dx = x - sx;
+        if (I.pbc) {
+          dy = hs_pmod(dy, ny);
+          dy = dy > ny / 2 ? dy - ny : dy;
+          dx = hs_pmod(dx, nx);
+          dx = dx > nx / 2 ? dx - nx : dx;
+        }
+        if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
+          const float wq = I.nn_mask[(dy + hy) * I.nn_w + (dx + hx)];
+          if (wq != 0.f) {
+            const float msn = mm > 0.f ? 1.f : -1.f;  // m of the source after its flip
+            if (I.in_plane) {
+              const float oxs = I.ox_uc[ss], oys = I.oy_uc[ss];
+              ex += msn * wq * (oxi * oxs + oyi * oys);
+              exp_ += msn * wq * (oys * oxi - oxs * oyi);
+            } else {
+              ex += msn * wq;
+            }
+            touched = true;
+          }
+        }
+      }
+    }
+  }
+  if (!inside || mi == 0 || !touched) return;
+  const float mmui = (float)mi * I.mu[c];
+  // E += 2*prefactor*(m mu)_i*conv (energies.py:362-363); exchange: -J m_i * 2 m_s * NN
+  const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
+  float e = I.E[c], ec = I.Ec[c];
+  hs_kahan_add(e, ec, de);
+  I.E[c] = e;
+  I.Ec[c] = ec;
+  if (I.use_perp) {
+    const float dep = (float)(2.0 * I.dip_pf) * (mmui * accp) + (float)(-2.0 * I.J) * ((float)mi * exp_);
+    float p = I.Ep[c], pc = I.Epc[c];
+    hs_kahan_add(p, pc, dep);
+    I.Ep[c] = p;
+    I.Epc[c] = pc;
+  }
+}
+
+}  // namespace
+
+extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
+  (void)inst;
+  if (!sel) return -1;
+  return (int64_t)ws_bytes(sel->nsc);
+}
+
+extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
+                                 hs_state_t *state, void *ws, int32_t *log_cells, int32_t *log_count, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!sel || !run || !state || !ws) return hs_set_error(HS_ERR_ARG, "missing sel/run/state/ws");
+  if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
+  if (run->log && (!log_cells || !log_count)) return hs_set_error(HS_ERR_ARG, "log requested without arrays");
+  if (sel->scheme == 0) {
+    if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
+      return hs_set_error(HS_ERR_ARG, "bad selection geometry");
+    if (sel->Rx > 0xffff || sel->Ry > 0x7fff) return hs_set_error(HS_ERR_ARG, "supercell too large");
+  } else if (sel->scheme != 1) {
+    return hs_set_error(HS_ERR_ARG, "unsupported MULTISAMPLING_SCHEME");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int nsc = sel->scheme == 0 ? sel->nsc : 1;
+  hs_select_t S = *sel;
+  S.nsc = nsc;
+  MetroWS w = carve(ws, nsc);
+  k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
+  HS_LAUNCH_CHECK("k_state_begin");
+  const dim3 grid((inst->nx + MU_TX - 1) / MU_TX, (inst->ny + MU_TY - 1) / MU_TY);
+  for (int64_t j = 0; j < run->max_steps; ++j) {
+    k_metro_select<<<1, MS_THREADS, 0, s>>>(*inst, S, *run, state, w, log_cells, log_count);
+    HS_LAUNCH_CHECK("k_metro_select");
+    k_metro_update<<<grid, MU_THREADS, 0, s>>>(*inst, S, w);
+    HS_LAUNCH_CHECK("k_metro_update");
+  }
+  return HS_OK;
+}

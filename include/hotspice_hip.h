@@ -1,0 +1,242 @@
+/*
+ * hotspice_hip.h -- C-ABI of libhotspice_hip.so, the gfx950 (MI355X) implementation of the
+ * Hotspice Monte Carlo stepping path.
+ *
+ * The library replaces the array expressions that Hotspice's stepping path evaluates with
+ * NumPy/CuPy (the reference has no native code). Every entry point below names the
+ * reference function(s) it stands in for. The host-side Python package hotspice_amd binds
+ * these with ctypes; its classes keep the reference's Magnets / Energy API.
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers (HBM), row-major (ny, nx) grids indexed [y][x],
+ *    flat index c = y*nx + x. Memory is owned by the caller (PyTorch tensors in hotspice_amd).
+ *  - Every function returns 0 on success and a negative code on failure; hs_last_error()
+ *    returns a message for the calling thread. Launches are asynchronous on `stream`
+ *    (a hipStream_t passed as void*); only functions documented as blocking synchronise.
+ *  - Energies are held as ONE combined local field per magnet (the sum over the Energy
+ *    objects the reference keeps separately, energies.py:22-83) in FP32 with a Kahan
+ *    compensation array: value = E - Ec. Likewise for the perpendicular field E_perp.
+ *  - Random numbers are Philox4x32-10 streams: key = seed, counter = (index, step, tag);
+ *    see hs_philox_uniform53.
+ */
+#ifndef HOTSPICE_HIP_H
+#define HOTSPICE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_MAX_UC 64  /* max unit-cell sites ux*uy */
+#define HS_MAX_NN 49  /* max entries of the exchange nearest-neighbour mask (7x7) */
+
+/* Cell flag bits (uint8 per cell). */
+#define HS_FLAG_OCC 1u   /* occupied now (Magnets.occupation, used by selection core.py:485) */
+#define HS_FLAG_NEEL 2u  /* finite Magnets._occupation_inf (core.py:95-96): Neel candidate */
+
+/* Error codes. */
+#define HS_OK 0
+#define HS_ERR_ARG -1
+#define HS_ERR_HIP -2
+#define HS_ERR_STATE -3
+
+/*
+ * One spin-ice instance: geometry, parameters and device pointers. Field order is part of
+ * the ABI (mirrored by hotspice_amd/_lib.py). Layout: 8-byte members first.
+ */
+typedef struct hs_instance {
+  /* device pointers */
+  int8_t *m;            /* (ny,nx) spins -1/0/+1 */
+  uint8_t *flags;       /* (ny,nx) HS_FLAG_* */
+  float *E, *Ec;        /* combined local field and its Kahan compensation */
+  float *Ep, *Epc;      /* combined perpendicular field (without m_perp_factor) */
+  const float *EB;      /* (ny,nx) E_B */
+  const float *beta;    /* (ny,nx) 1/(kB T) */
+  const float *mu;      /* (ny,nx) moment */
+  const float *K32;     /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP32 (stepping) */
+  const float *Kpo32;   /* same shape, K_perpother (stepping E_perp) */
+  const double *K64;    /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP64 (recompute) */
+  const double *Kps64;  /* same shape, K_perpself (recompute E_perp) */
+  const float *zee_bx;  /* optional (ny,nx) field x-component (OOP: magnitude); NULL = uniform */
+  const float *zee_by;  /* optional (ny,nx) field y-component; NULL = uniform */
+  const int32_t *occ_list; /* [n_occ] flat indices of occupied cells, row-major (core.py:97-98) */
+  uint64_t seed;        /* Philox key */
+  double zee_Bx, zee_By; /* uniform field components when zee_bx/zee_by are NULL */
+  double dip_pf;        /* DipolarEnergy.prefactor (0 = no dipolar energy) */
+  double J;             /* ExchangeEnergy.J (0 = no exchange energy) */
+  double m_perp_factor; /* Magnets.m_perp_factor */
+  /* 4-byte members */
+  int32_t nx, ny, ux, uy;
+  int32_t n_sub;        /* number of dipolar kernels (occupied unit-cell sites) */
+  int32_t kwx, kwy;     /* stored kernel half widths (nx-1, ny-1 for the full kernel) */
+  int32_t pbc;          /* periodic boundaries */
+  int32_t in_plane;
+  int32_t use_perp;     /* Magnets.USE_PERP_ENERGY (core.py:273-275) */
+  int32_t barrier;      /* 0 = 'simple', 1 = 'parabolic' (core.py:641-648) */
+  int32_t nn_w, nn_h;   /* exchange NN mask width/height (odd) */
+  int32_t n_occ;        /* Magnets.n */
+  int32_t has_zeeman;
+  int32_t uniform_beta; /* 1 when T is uniform over the grid */
+  int32_t sub_of[HS_MAX_UC];    /* unit-cell site (y%uy)*ux + x%ux -> kernel index or -1 */
+  float ox_uc[HS_MAX_UC];       /* in-plane orientation per unit-cell site */
+  float oy_uc[HS_MAX_UC];
+  float nn_mask[HS_MAX_NN];     /* exchange NN mask, [nn_h][nn_w] */
+} hs_instance_t;
+
+/* Geometry needed to build dipolar kernels (DipolarEnergy._initialize, energies.py:176-288). */
+typedef struct hs_kernel_geom {
+  const double *rrx;  /* [n_sub][2nx-1] x-offsets (m) from the centre magnet, host-computed cumsums */
+  const double *rry;  /* [n_sub][2ny-1] y-offsets (m) */
+  double I;           /* (major^2 + minor^2)/16 finite-size term */
+  double cutoff;      /* cutoff_range (m), +inf for none */
+  double decay;       /* decay_exponent (-3 by default) */
+  int32_t nx, ny, ux, uy;
+  int32_t n_sub;
+  int32_t kwx, kwy;   /* half widths of the stored window */
+  int32_t pbc, in_plane;
+  int32_t site_x[HS_MAX_UC], site_y[HS_MAX_UC]; /* unit-cell site of kernel n */
+  int32_t shift_x[HS_MAX_UC], shift_y[HS_MAX_UC]; /* tiled-matrix slice starts (energies.py:205-206) */
+  float uc_ox[HS_MAX_UC], uc_oy[HS_MAX_UC]; /* orientation per site (IP) */
+  float uc_o[HS_MAX_UC];                    /* occupation per site (OOP) */
+} hs_kernel_geom_t;
+
+/* Run control for a stepping call (Magnets.progress, core.py:736-760). */
+typedef struct hs_run {
+  double t_max;         /* progress t_max (may be +inf) */
+  double mc_max;        /* progress MCsteps_max (may be +inf) */
+  double attempt_freq;  /* f0 (core.py:659, 679) */
+  int64_t max_steps;    /* step budget of this call */
+  int32_t log;          /* 1: write per-step log arrays */
+  int32_t nblocks;      /* grid size for grid-wide kernels (0 = automatic) */
+  int32_t begin;        /* 1: this call starts a progress() call (t0, attempts0 := t, attempts) */
+  int32_t pad;
+} hs_run_t;
+
+/* Device-resident run state; the library keeps two copies (double buffer). */
+typedef struct hs_state {
+  double t;             /* Magnets.t */
+  double t0;            /* t at the start of the progress call */
+  int64_t attempts;     /* Magnets.attempted_switches */
+  int64_t attempts0;
+  int64_t switches;     /* Magnets.switches */
+  int64_t step;         /* update() calls so far: the Philox step counter */
+  int64_t steps_done;   /* steps done by this call */
+  int32_t done;         /* 1 once the stop predicate fired */
+  int32_t err;          /* nonzero on a device-side error */
+} hs_state_t;
+
+/* Metropolis selection geometry for one r (Magnets._select_grid, core.py:496-558). */
+typedef struct hs_select {
+  const int32_t *pat_off;   /* [uy*ux+1] offsets into pat_cells for supercell pattern (dy,dx) */
+  const int32_t *pat_cells; /* occupied (oy*Rx+ox) cells of occupation[dy:dy+Ry, dx:dx+Rx], row-major */
+  int32_t Rx, Ry;           /* supercell size in cells */
+  int32_t snx, sny;         /* supercells along x/y: n//R + 1 */
+  int32_t ncols, nsc;       /* chosen (every other) supercells per row and in total */
+  int32_t scheme;           /* 0 = 'grid', 1 = 'single' */
+  int32_t cutoff;           /* SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF */
+  int32_t reduced_k;        /* REDUCED_KERNEL_SIZE (0 = full kernel) */
+  int32_t metropolis_time;  /* SimParams.METROPOLIS_TIME */
+} hs_select_t;
+
+/* ---- entry points ----------------------------------------------------------------------- */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int hs_version(void);
+
+/* Message of the last error on this thread ("" if none). */
+const char *hs_last_error(void);
+
+/*
+ * Layout of the ABI structs for binding self-checks: writes {sizeof(hs_instance_t),
+ * offsetof seed, nx, sub_of, nn_mask; sizeof(hs_kernel_geom_t), offsetof nx, uc_o;
+ * sizeof(hs_run_t), offsetof max_steps, begin; sizeof(hs_state_t), offsetof done;
+ * sizeof(hs_select_t), offsetof metropolis_time} to out[0..14] when n >= 15. Returns the
+ * count (15). Touches no device. No counterpart in the reference (binding infrastructure).
+ */
+int hs_abi_layout(int64_t *out, int32_t n);
+
+/* Number of HIP devices and the name/arch of device 0 (blocking). */
+int hs_device_info(int32_t *count, char *name, int32_t name_len);
+
+/*
+ * Philox4x32-10 53-bit uniforms out[i] for index = index0 + i under (seed, step, tag);
+ * open_zero=0 gives [0,1), 1 gives (0,1]. Exposes the exact streams the stepping kernels
+ * draw (replacing mm.rng / numpy.random at core.py:83, 493, 527-538, 671, 698) for replay.
+ */
+int hs_philox_uniform53(uint64_t seed, uint64_t step, uint32_t tag, int64_t index0, int64_t n,
+                        int32_t open_zero, double *out, void *stream);
+
+/*
+ * Dipolar kernels K, K_perpself, K_perpother of every occupied unit-cell site, FP64, plus FP32
+ * copies of K and K_perpother. Replaces DipolarEnergy._initialize (energies.py:176-288),
+ * including the cutoff (219), PBC folding (263-277) and the mu0/4pi factor (279-281).
+ * Outputs have shape [n_sub][2kwy+1][2kwx+1]; Kps64/Kpo64 may be NULL for OOP lattices.
+ */
+int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, double *Kps64, double *Kpo64,
+                             float *K32, float *Kpo32, void *stream);
+
+/*
+ * FP64 full recompute of the combined field (and E_perp when which&2), written as FP32 +
+ * compensation. Replaces Energy.update() of DipolarEnergy (energies.py:290-308), ZeemanEnergy
+ * (energies.py:116-118) and ExchangeEnergy (energies.py:771-781) as called by
+ * Magnets.update_energy() with index=None (core.py:247-249). which: 1 = E, 2 = E_perp, 3 = both.
+ */
+int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *stream);
+
+/*
+ * Per-energy FP64 fields for the host API (Energy.E / Energy.E_perp): out[0..3] planes =
+ * Zeeman E, Zeeman E_perp, exchange E, exchange E_perp (closed forms, energies.py:106-118,
+ * 771-781) and out[4], out[5] = combined E and E_perp (value - compensation). Dipolar fields
+ * follow as combined minus Zeeman minus exchange. out: [6][ny][nx] doubles.
+ */
+int hs_field_components(const hs_instance_t *inst, double *out, void *stream);
+
+/*
+ * Total energy sum over energies of Energy.E_tot (energies.py:128-130, 368-370, 789-791;
+ * core.py:426-428) from the incremental fields, FP64: out[0] = E_tot. Blocking.
+ */
+int hs_energy_total(const hs_instance_t *inst, double *out_host, void *stream);
+
+/*
+ * Swap the Zeeman contribution of the combined fields from the old uniform field (Bx0,By0)
+ * to the instance's current field (inst->zee_Bx/By or zee_bx/zee_by arrays):
+ * E += m*(E_factor_new - E_factor_old). Replaces ZeemanEnergy.set_field (energies.py:101-114).
+ * bx_old/by_old arrays (NULL = uniform old values).
+ */
+int hs_zeeman_swap(const hs_instance_t *inst, double Bx0, double By0, const float *bx_old,
+                   const float *by_old, void *stream);
+
+/*
+ * Neel-Arrhenius steps (Magnets._update_Neel core.py:679-709 with E_barrier core.py:625-656
+ * and DipolarEnergy/ZeemanEnergy/ExchangeEnergy.update_single energies.py:120-122, 310-329,
+ * 783-784), looped as Magnets.progress (core.py:736-760): up to run->max_steps steps while
+ * lower_than(t-t0, t_max) and lower_than(MCsteps-MCsteps0, MCsteps_max).
+ * state: device hs_state_t[2] (copy 0 holds the input state; the result is in copy
+ * (steps_launched+1)&1, reported through *final_copy). partials: device scratch of
+ * 32*nblocks*2 bytes. log_idx/log_dt: optional device arrays [max_steps] (flat winner index or
+ * -1 when no switch; dt added to t). Non-blocking.
+ */
+int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_state_t *state, void *partials,
+                int32_t *log_idx, double *log_dt, int32_t *final_copy, void *stream);
+
+/*
+ * Metropolis steps with multisampling (Magnets._update_Metropolis core.py:659-677, select /
+ * _select_grid / _select_single core.py:456-558, update_single / update_multiple
+ * energies.py:310-366 with its convolution-or-sum branch rule, ZeemanEnergy / ExchangeEnergy
+ * updates), looped as Magnets.progress. ws: device scratch of hs_metropolis_ws_bytes(inst, sel)
+ * bytes. state: device hs_state_t[2] (copy 0 is the state). log_cells/log_count: optional
+ * per-step accepted cells (capacity sel->nsc per step, at least 1) and counts. Non-blocking.
+ */
+int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
+                      hs_state_t *state, void *ws, int32_t *log_cells, int32_t *log_count,
+                      void *stream);
+
+/* Scratch bytes hs_metropolis_run needs for this instance and selection geometry. */
+int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HOTSPICE_HIP_H */

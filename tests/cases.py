@@ -62,3 +62,22 @@ def oracle_magnets(name, fx=None):
     return model.make_magnets(spec["lattice"], spec["a"], n=spec["n"], T=spec["T"], E_B=fx["E_B"],
                               moment=spec["moment"], pattern=fx["pattern"], energies=ens, params=params,
                               seed=spec["seed"], PBC=spec.get("kw", {}).get("PBC", False))
+
+
+def product_magnets(name, fx=None, **extra):
+    """hotspice_amd Magnets for a fixture case (same pattern, E_B and Philox seed)."""
+    import hotspice_amd as hs
+    spec = CASES[name]
+    fx = load(name) if fx is None else fx
+    kT = kB * spec["T"]
+    ens = []
+    for kind, kw in spec["energies"]:
+        if kind == "dipolar":
+            ens.append(hs.DipolarEnergy(**kw))
+        elif kind == "zeeman":
+            ens.append(hs.ZeemanEnergy(**kw))
+        else:
+            ens.append(hs.ExchangeEnergy(J=kw["J_kT"] * kT))
+    cls = getattr(hs.ASI, spec["lattice"])
+    return cls(spec["a"], spec["n"], T=spec["T"], E_B=fx["E_B"], moment=spec["moment"], pattern=fx["pattern"],
+               energies=ens, params=hs.SimParams(**spec["params"]), seed=spec["seed"], **spec.get("kw", {}), **extra)
