@@ -40,6 +40,7 @@ class DeviceState:
         self.EB = torch.zeros(n, dtype=torch.float32, device=dev)
         self.beta = torch.zeros(n, dtype=torch.float32, device=dev)
         self.mu = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.mu64 = torch.zeros(n, dtype=torch.float64, device=dev)
         self.occ_list = torch.zeros(max(1, n), dtype=torch.int32, device=dev)
         self.state = torch.zeros(2 * _lib.STATE_BYTES, dtype=torch.uint8, device=dev)
         self.partials = torch.zeros(2 * 1024 * 32, dtype=torch.uint8, device=dev)
@@ -47,6 +48,7 @@ class DeviceState:
         self.zee_by = None
         self.kern = None  # dict of kernel tensors + window
         self.metro_ws = None
+        self.rec_scratch = None
         self.sel_cache = {}
         self.log_cap = 0
         self.log_idx = None
@@ -116,7 +118,13 @@ class DeviceState:
 
     # ---- calls -----------------------------------------------------------------------------
     def recompute(self, which=3):
-        _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.stream()))
+        need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
+        if need < 0:
+            raise _lib.HotspiceError("bad instance for recompute")
+        if self.rec_scratch is None or self.rec_scratch.numel() < need:
+            self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
+        _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.ptr(self.rec_scratch),
+                                                self.rec_scratch.numel(), self.stream()))
 
     def components(self):
         out = self.torch.empty((6, self.ny, self.nx), dtype=self.torch.float64, device=self.device)

@@ -18,7 +18,7 @@ HS_FLAG_NEEL = 2
 TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME = 1, 2, 3, 4, 5
 
 EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs_philox_uniform53", "hs_build_dipolar_kernels",
-           "hs_recompute_fields", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
+           "hs_recompute_fields", "hs_recompute_scratch_bytes", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
            "hs_metropolis_run", "hs_metropolis_ws_bytes")
 
 
@@ -27,7 +27,7 @@ class Instance(C.Structure):
     _fields_ = [
         ("m", C.c_void_p), ("flags", C.c_void_p), ("E", C.c_void_p), ("Ec", C.c_void_p),
         ("Ep", C.c_void_p), ("Epc", C.c_void_p), ("EB", C.c_void_p), ("beta", C.c_void_p), ("mu", C.c_void_p),
-        ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
+        ("mu64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
         ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p),
         ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
         ("J", C.c_double), ("m_perp_factor", C.c_double),
@@ -35,7 +35,7 @@ class Instance(C.Structure):
         ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
         ("use_perp", C.c_int32), ("barrier", C.c_int32), ("nn_w", C.c_int32), ("nn_h", C.c_int32),
         ("n_occ", C.c_int32), ("has_zeeman", C.c_int32), ("uniform_beta", C.c_int32),
-        ("sub_of", C.c_int32 * HS_MAX_UC), ("ox_uc", C.c_float * HS_MAX_UC), ("oy_uc", C.c_float * HS_MAX_UC),
+        ("sub_of", C.c_int32 * HS_MAX_UC), ("ox_uc", C.c_double * HS_MAX_UC), ("oy_uc", C.c_double * HS_MAX_UC),
         ("nn_mask", C.c_float * HS_MAX_NN),
     ]
 
@@ -48,7 +48,7 @@ class KernelGeom(C.Structure):
         ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
         ("site_x", C.c_int32 * HS_MAX_UC), ("site_y", C.c_int32 * HS_MAX_UC),
         ("shift_x", C.c_int32 * HS_MAX_UC), ("shift_y", C.c_int32 * HS_MAX_UC),
-        ("uc_ox", C.c_float * HS_MAX_UC), ("uc_oy", C.c_float * HS_MAX_UC), ("uc_o", C.c_float * HS_MAX_UC),
+        ("uc_ox", C.c_double * HS_MAX_UC), ("uc_oy", C.c_double * HS_MAX_UC), ("uc_o", C.c_double * HS_MAX_UC),
     ]
 
 
@@ -99,7 +99,8 @@ def load():
         "hs_device_info": ([C.POINTER(i32), C.c_char_p, i32], C.c_int),
         "hs_philox_uniform53": ([u64, u64, C.c_uint32, i64, i64, i32, vp, vp], C.c_int),
         "hs_build_dipolar_kernels": ([C.POINTER(KernelGeom), vp, vp, vp, vp, vp, vp], C.c_int),
-        "hs_recompute_fields": ([C.POINTER(Instance), i32, vp], C.c_int),
+        "hs_recompute_fields": ([C.POINTER(Instance), i32, vp, i64, vp], C.c_int),
+        "hs_recompute_scratch_bytes": ([C.POINTER(Instance)], C.c_int64),
         "hs_field_components": ([C.POINTER(Instance), vp, vp], C.c_int),
         "hs_energy_total": ([C.POINTER(Instance), C.POINTER(dbl), vp], C.c_int),
         "hs_zeeman_swap": ([C.POINTER(Instance), dbl, dbl, vp, vp, vp], C.c_int),

@@ -371,6 +371,7 @@ class Magnets(ABC):
         self._moment = np.array(as_2D_array(value, self.shape), dtype=float)
         self._momentSq = self._moment * self._moment
         self._dev.upload(self._dev.mu, self._moment.astype(np.float32))
+        self._dev.upload(self._dev.mu64, self._moment)
 
     @property
     def USE_PERP_ENERGY(self):
@@ -592,7 +593,7 @@ class Magnets(ABC):
         I = d.inst
         P = d.ptr
         I.m, I.flags, I.E, I.Ec = P(d.m), P(d.flags), P(d.E), P(d.Ec)
-        I.Ep, I.Epc, I.EB, I.beta, I.mu = P(d.Ep), P(d.Epc), P(d.EB), P(d.beta), P(d.mu)
+        I.Ep, I.Epc, I.EB, I.beta, I.mu, I.mu64 = P(d.Ep), P(d.Epc), P(d.EB), P(d.beta), P(d.mu), P(d.mu64)
         I.occ_list = P(d.occ_list)
         I.seed = self.seed
         I.nx, I.ny, I.ux, I.uy = self.nx, self.ny, self.unitcell.x, self.unitcell.y

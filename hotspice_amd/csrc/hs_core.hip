@@ -70,7 +70,7 @@ int hs_check_instance(const hs_instance_t *I) {
   if (I->nn_w < 0 || I->nn_h < 0 || I->nn_w * I->nn_h > HS_MAX_NN || (I->nn_w && !(I->nn_w & 1)) ||
       (I->nn_h && !(I->nn_h & 1)))
     return hs_set_error(HS_ERR_ARG, "exchange mask %dx%d must be odd and <= %d entries", I->nn_w, I->nn_h, HS_MAX_NN);
-  if (!I->m || !I->flags || !I->E || !I->Ec || !I->EB || !I->beta || !I->mu || !I->occ_list)
+  if (!I->m || !I->flags || !I->E || !I->Ec || !I->EB || !I->beta || !I->mu || !I->mu64 || !I->occ_list)
     return hs_set_error(HS_ERR_ARG, "missing state array");
   if (I->use_perp && (!I->Ep || !I->Epc)) return hs_set_error(HS_ERR_ARG, "use_perp without E_perp arrays");
   if (I->dip_pf != 0.0) {
@@ -205,7 +205,7 @@ extern "C" int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, 
 // ---- per-cell closed forms: Zeeman factors and exchange sums ------------------------------------
 HS_INLINE void zeeman_factors(const hs_instance_t &I, int c, int site, double Bx, double By, double &f,
                               double &fp) {
-  const double mu = I.mu[c];
+  const double mu = I.mu64[c];
   if (I.in_plane) {
     const double ox = I.ox_uc[site], oy = I.oy_uc[site];
     f = -mu * (Bx * ox + By * oy);   // energies.py:108
@@ -266,60 +266,144 @@ HS_INLINE void exchange_fields(const hs_instance_t &I, int y, int x, double mi, 
 }
 
 // ---- FP64 full recompute (energies.py:290-308 + Zeeman/Exchange update) ------------------------
-constexpr int RC_TILE = 256;
+//
+// DipolarEnergy.update computes, per target sublattice p (the unit-cell site of kernel n_p),
+// E_i = prefactor * mu_i * m_i * sum_j K_{n_p}[centre + (j - i)] * (m mu)_j. Restricted to one
+// source sublattice q, targets and sources sit on coarse grids a, a' (y = site_y + uy*a,
+// x = site_x + ux*b), so each (p, q) pair is a dense 2D correlation with the decimated kernel
+// D_pq[a'-a][b'-b]. One workgroup owns a 32x64 coarse target tile of one p and a group of
+// source tiles (split for parallelism); it stages the source tile (m mu) and the 63x127 kernel
+// block in LDS and each thread accumulates 8 consecutive outputs with a register window that
+// slides along the kernel row (15 LDS reads per 64 FMAs). Per-group partial sums are reduced
+// in a fixed order by k_recompute_finish, which also adds the Zeeman and exchange closed forms.
+constexpr int RA = 32, RB = 64, RR = 8;  // coarse tile rows, cols, outputs per thread (256 threads)
+constexpr int KBR = 2 * RA - 1, KBC = 2 * RB - 1;
 
-__global__ void __launch_bounds__(256) k_recompute(hs_instance_t I, int which) {
-  __shared__ int s_pos[RC_TILE];
-  __shared__ double s_mmu[RC_TILE];
-  const int ncell = I.nx * I.ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = c < ncell;
-  int y = 0, x = 0, site = 0, n = -1;
-  double mi = 0.0;
-  if (active) {
-    y = c / I.nx;
-    x = c % I.nx;
-    site = hs_site(I, y, x);
-    n = I.dip_pf != 0.0 ? I.sub_of[site] : -1;
-    mi = (double)I.m[c];
-  }
-  const bool do_e = which & 1, do_p = (which & 2) && I.use_perp;
-  double acc = 0.0, accp = 0.0;
+struct RecPlan {
+  int Ay, Bx;      // coarse grid extents ceil(ny/uy), ceil(nx/ux)
+  int tA, tB;      // target tiles along a, b
+  int F;           // source-tile groups per target tile
+  int sites_y[HS_MAX_UC], sites_x[HS_MAX_UC];  // unit-cell site of kernel n
+};
+
+HS_INLINE int div_floor(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+HS_INLINE int div_ceil(int a, int b) { return -div_floor(-a, b); }
+
+// Coarse offset range da with |dy0 + u*da| <= kw.
+HS_INLINE void coarse_range(int dy0, int u, int kw, int &lo, int &hi) {
+  lo = div_ceil(-kw - dy0, u);
+  hi = div_floor(kw - dy0, u);
+}
+
+__global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan plan, const double *Kbase,
+                                                       double *part) {
+  __shared__ double sK[KBR * KBC];
+  __shared__ double sS[RA * RB];
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x, p = blockIdx.y, g = blockIdx.z;
+  const int a0 = (tile / plan.tB) * RA, b0 = (tile % plan.tB) * RB;
+  const int py = plan.sites_y[p], px = plan.sites_x[p];
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
-  // dipolar: sum over occupied sources j of K_{n_i}[centre + (j - i)] * (m mu)_j
-  if (I.dip_pf != 0.0) {
-    for (int base = 0; base < I.n_occ; base += RC_TILE) {
-      __syncthreads();
-      const int j = base + threadIdx.x;
-      if (j < I.n_occ) {
-        const int cj = I.occ_list[j];
-        s_pos[threadIdx.x] = cj;
-        s_mmu[threadIdx.x] = (double)I.m[cj] * (double)I.mu[cj];
-      }
-      __syncthreads();
-      const int cnt = min(RC_TILE, I.n_occ - base);
-      if (active && n >= 0 && mi != 0.0) {
-        const double *Kn = I.K64 + (size_t)n * KS;
-        const double *Kp = do_p ? I.Kps64 + (size_t)n * KS : nullptr;
-        for (int t = 0; t < cnt; ++t) {
-          const int cj = s_pos[t];
-          const int dy = cj / I.nx - y, dx = cj % I.nx - x;
-          if (dy < -I.kwy || dy > I.kwy || dx < -I.kwx || dx > I.kwx) continue;
-          const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
-          const double sm = s_mmu[t];
-          if (do_e) acc += Kn[o] * sm;
-          if (do_p) accp += Kp[o] * sm;
+  const double *Kn = Kbase + (size_t)p * KS;
+  const int ta = tid / (RB / RR), tbg = tid % (RB / RR);
+  double acc[RR];
+#pragma unroll
+  for (int j = 0; j < RR; ++j) acc[j] = 0.0;
+  // count the (q, source tile) chunks of this target tile; group g takes an even slice
+  int total = 0;
+  for (int q = 0; q < I.n_sub; ++q) {
+    int dlo, dhi, elo, ehi;
+    coarse_range(plan.sites_y[q] - py, I.uy, I.kwy, dlo, dhi);
+    coarse_range(plan.sites_x[q] - px, I.ux, I.kwx, elo, ehi);
+    const int r0 = max(0, a0 + dlo), r1 = min(plan.Ay - 1, a0 + RA - 1 + dhi);
+    const int c0 = max(0, b0 + elo), c1 = min(plan.Bx - 1, b0 + RB - 1 + ehi);
+    if (r0 <= r1 && c0 <= c1) total += ((r1 - r0) / RA + 1) * ((c1 - c0) / RB + 1);
+  }
+  const int lo = (int)((int64_t)total * g / plan.F), hi = (int)((int64_t)total * (g + 1) / plan.F);
+  int idx = 0;
+  for (int q = 0; q < I.n_sub && idx < hi; ++q) {
+    const int qy = plan.sites_y[q], qx = plan.sites_x[q];
+    const int dy0 = qy - py, dx0 = qx - px;
+    int dlo, dhi, elo, ehi;
+    coarse_range(dy0, I.uy, I.kwy, dlo, dhi);
+    coarse_range(dx0, I.ux, I.kwx, elo, ehi);
+    const int r0 = max(0, a0 + dlo), r1 = min(plan.Ay - 1, a0 + RA - 1 + dhi);
+    const int c0 = max(0, b0 + elo), c1 = min(plan.Bx - 1, b0 + RB - 1 + ehi);
+    if (r0 > r1 || c0 > c1) continue;
+    for (int as0 = r0; as0 <= r1 && idx < hi; as0 += RA)
+      for (int bs0 = c0; bs0 <= c1 && idx < hi; bs0 += RB, ++idx) {
+        if (idx < lo) continue;
+        __syncthreads();
+        // source tile (m mu) on sublattice q
+        for (int e = tid; e < RA * RB; e += 256) {
+          const int r = e / RB, c = e % RB;
+          const int a = as0 + r, b = bs0 + c;
+          const int y = qy + I.uy * a, x = qx + I.ux * b;
+          double v = 0.0;
+          if (a <= r1 && b <= c1 && y < I.ny && x < I.nx) {
+            const int cc = y * I.nx + x;
+            v = (double)I.m[cc] * I.mu64[cc];
+          }
+          sS[e] = v;
+        }
+        // kernel block: row ra <-> da = as0 - a0 - (RA-1) + ra, col cb <-> db = bs0 - b0 - (RB-1) + cb
+        for (int e = tid; e < KBR * KBC; e += 256) {
+          const int ra = e / KBC, cb = e % KBC;
+          const int dy = dy0 + I.uy * (as0 - a0 - (RA - 1) + ra);
+          const int dx = dx0 + I.ux * (bs0 - b0 - (RB - 1) + cb);
+          double v = 0.0;
+          if (dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx)
+            v = Kn[(size_t)(I.kwy + dy) * KW + (I.kwx + dx)];
+          sK[e] = v;
+        }
+        __syncthreads();
+        // acc[j] += sum_{r,c} K[r - ta + RA-1][c - (8 tbg + j) + RB-1] * S[r][c]
+        for (int r = 0; r < RA; ++r) {
+          const double *krow = sK + (r - ta + RA - 1) * KBC + (RB - 1) - RR * tbg;
+          const double *srow = sS + r * RB;
+          for (int c0b = 0; c0b < RB; c0b += RR) {
+            double w[2 * RR - 1];  // w[t] = krow[c0b - (RR-1) + t]
+#pragma unroll
+            for (int t = 0; t < 2 * RR - 1; ++t) w[t] = krow[c0b - (RR - 1) + t];
+#pragma unroll
+            for (int cc = 0; cc < RR; ++cc) {
+              const double s = srow[c0b + cc];
+#pragma unroll
+              for (int j = 0; j < RR; ++j) acc[j] = fma(w[cc - j + RR - 1], s, acc[j]);
+            }
+          }
         }
       }
-    }
   }
-  if (!active) return;
-  const double mu = I.mu[c];
+  // partial sums of this group for the tile's targets
+  const int a = a0 + ta;
+  const int y = py + I.uy * a;
+#pragma unroll
+  for (int j = 0; j < RR; ++j) {
+    const int b = b0 + RR * tbg + j;
+    const int x = px + I.ux * b;
+    if (a < plan.Ay && b < plan.Bx && y < I.ny && x < I.nx) part[(size_t)g * I.nx * I.ny + (size_t)y * I.nx + x] = acc[j];
+  }
+}
+
+__global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, const double *partP, int which) {
+  const int ncell = I.nx * I.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int y = c / I.nx, x = c % I.nx, site = hs_site(I, y, x);
+  const int n = I.dip_pf != 0.0 ? I.sub_of[site] : -1;
+  const double mi = (double)I.m[c];
+  const bool do_e = which & 1, do_p = (which & 2) && I.use_perp;
   double e = 0.0, ep = 0.0;
   if (mi != 0.0) {
-    // dipolar: prefactor*moment*(partial_m*conv)  (energies.py:303, 307)
-    if (n >= 0) {
+    const double mu = I.mu64[c];
+    if (n >= 0) {  // prefactor*moment*(partial_m*conv) (energies.py:303, 307)
+      double acc = 0.0, accp = 0.0;
+      for (int g = 0; g < F; ++g) {
+        if (do_e) acc += partE[(size_t)g * ncell + c];
+        if (do_p) accp += partP[(size_t)g * ncell + c];
+      }
       e += (I.dip_pf * mu) * (mi * acc);
       ep += (I.dip_pf * mu) * (mi * accp);
     }
@@ -336,24 +420,77 @@ __global__ void __launch_bounds__(256) k_recompute(hs_instance_t I, int which) {
     ep += xp;
   }
   if (do_e) {
-    const float hi = (float)e;
-    I.E[c] = hi;
-    I.Ec[c] = (float)((double)hi - e);
+    const float h = (float)e;
+    I.E[c] = h;
+    I.Ec[c] = (float)((double)h - e);
   }
   if (do_p) {
-    const float hi = (float)ep;
-    I.Ep[c] = hi;
-    I.Epc[c] = (float)((double)hi - ep);
+    const float h = (float)ep;
+    I.Ep[c] = h;
+    I.Epc[c] = (float)((double)h - ep);
   }
 }
 
-extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *stream) {
+static RecPlan make_plan(const hs_instance_t *I) {
+  RecPlan pl{};
+  pl.Ay = (I->ny + I->uy - 1) / I->uy;
+  pl.Bx = (I->nx + I->ux - 1) / I->ux;
+  pl.tA = (pl.Ay + RA - 1) / RA;
+  pl.tB = (pl.Bx + RB - 1) / RB;
+  const int tiles = pl.tA * pl.tB * (I->n_sub > 0 ? I->n_sub : 1);
+  int F = (768 + tiles - 1) / tiles;
+  pl.F = F < 1 ? 1 : (F > 64 ? 64 : F);
+  for (int s = 0; s < I->ux * I->uy; ++s) {
+    const int n = I->sub_of[s];
+    if (n >= 0 && n < HS_MAX_UC) {
+      pl.sites_y[n] = s / I->ux;
+      pl.sites_x[n] = s % I->ux;
+    }
+  }
+  return pl;
+}
+
+extern "C" int64_t hs_recompute_scratch_bytes(const hs_instance_t *inst) {
+  if (!inst || inst->nx <= 0 || inst->ny <= 0 || inst->ux <= 0 || inst->uy <= 0) return -1;
+  const RecPlan pl = make_plan(inst);
+  return (int64_t)2 * pl.F * inst->nx * inst->ny * (int64_t)sizeof(double);
+}
+
+extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *scratch, int64_t scratch_bytes,
+                                   void *stream) {
   hs_clear_error();
   int rc = hs_check_instance(inst);
   if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const RecPlan pl = make_plan(inst);
   const int ncell = inst->nx * inst->ny;
-  k_recompute<<<(ncell + 255) / 256, 256, 0, (hipStream_t)stream>>>(*inst, which);
-  HS_LAUNCH_CHECK("k_recompute");
+  const bool dip = inst->dip_pf != 0.0;
+  const bool do_p = (which & 2) && inst->use_perp;
+  double *partE = nullptr, *partP = nullptr;
+  if (dip) {
+    const int64_t need = hs_recompute_scratch_bytes(inst);
+    if (!scratch || scratch_bytes < need)
+      return hs_set_error(HS_ERR_ARG, "recompute scratch of %lld bytes is below the %lld needed", (long long)scratch_bytes,
+                          (long long)need);
+    for (int n = 0; n < inst->n_sub; ++n) {
+      bool found = false;
+      for (int q = 0; q < inst->ux * inst->uy; ++q) found |= inst->sub_of[q] == n;
+      if (!found) return hs_set_error(HS_ERR_ARG, "kernel %d has no unit-cell site", n);
+    }
+    partE = (double *)scratch;
+    partP = partE + (size_t)pl.F * ncell;
+    const dim3 grid(pl.tA * pl.tB, inst->n_sub, pl.F);
+    if (which & 1) {
+      k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->K64, partE);
+      HS_LAUNCH_CHECK("k_recompute_dip");
+    }
+    if (do_p) {
+      k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->Kps64, partP);
+      HS_LAUNCH_CHECK("k_recompute_dip");
+    }
+  }
+  k_recompute_finish<<<(ncell + 255) / 256, 256, 0, s>>>(*inst, dip ? pl.F : 0, partE, partP, which);
+  HS_LAUNCH_CHECK("k_recompute_finish");
   return HS_OK;
 }
 

@@ -93,6 +93,21 @@ HS_INLINE void hs_wave_argmin(float &k, int &i) {
   }
 }
 
+// Same, carrying a payload (the slot the winning pair came from).
+HS_INLINE void hs_wave_argmin_slot(float &k, int &i, int &slot) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float k2 = __shfl_xor(k, off, 64);
+    const int i2 = __shfl_xor(i, off, 64);
+    const int s2 = __shfl_xor(slot, off, 64);
+    if (k2 < k || (k2 == k && i2 < i)) {
+      k = k2;
+      i = i2;
+      slot = s2;
+    }
+  }
+}
+
 template <typename T>
 HS_INLINE T hs_wave_sum(T v) {
 #pragma unroll

@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
                                                         int j, int last) {
   __shared__ float s_key[NB_THREADS / 64];
   __shared__ int s_idx[NB_THREADS / 64];
+  __shared__ int s_slot[NB_THREADS / 64];
   __shared__ NeelPartial s_win;
   const int P = gridDim.x;
   const int tid = threadIdx.x;
@@ -92,31 +93,39 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
   bool flip = false;
   int w = -1, mw = 0;
 
-  // 1. the previous launch's winner
+  // 1. the previous launch's winner: (key, index) argmin over the partials, carrying the slot
   if (!done && j > 0) {
     const NeelPartial *pp = parts + (size_t)((j - 1) & 1) * P;
     float k = INFINITY;
-    int idx = 0x7fffffff;
-    for (int b = tid; b < P; b += NB_THREADS) hs_argmin_pair(k, idx, pp[b].key, pp[b].idx < 0 ? 0x7fffffff : pp[b].idx);
-    hs_wave_argmin(k, idx);
+    int idx = 0x7fffffff, slot = -1;
+    for (int b = tid; b < P; b += NB_THREADS) {
+      const float kb = pp[b].key;
+      const int ib = pp[b].idx < 0 ? 0x7fffffff : pp[b].idx;
+      if (kb < k || (kb == k && ib < idx)) {
+        k = kb;
+        idx = ib;
+        slot = b;
+      }
+    }
+    hs_wave_argmin_slot(k, idx, slot);
     if ((tid & 63) == 0) {
       s_key[tid >> 6] = k;
       s_idx[tid >> 6] = idx;
+      s_slot[tid >> 6] = slot;
     }
     __syncthreads();
     if (tid == 0) {
       float bk = s_key[0];
-      int bi = s_idx[0];
-      for (int q = 1; q < NB_THREADS / 64; ++q) hs_argmin_pair(bk, bi, s_key[q], s_idx[q]);
+      int bi = s_idx[0], bs = s_slot[0];
+      for (int q = 1; q < NB_THREADS / 64; ++q)
+        if (s_key[q] < bk || (s_key[q] == bk && s_idx[q] < bi)) {
+          bk = s_key[q];
+          bi = s_idx[q];
+          bs = s_slot[q];
+        }
       NeelPartial win{};
       win.idx = -1;
-      if (bi != 0x7fffffff) {
-        for (int b = 0; b < P; ++b)
-          if (pp[b].idx == bi) {
-            win = pp[b];
-            break;
-          }
-      }
+      if (bi != 0x7fffffff && bs >= 0) win = pp[bs];
       s_win = win;
     }
     __syncthreads();

@@ -53,7 +53,8 @@ typedef struct hs_instance {
   float *Ep, *Epc;      /* combined perpendicular field (without m_perp_factor) */
   const float *EB;      /* (ny,nx) E_B */
   const float *beta;    /* (ny,nx) 1/(kB T) */
-  const float *mu;      /* (ny,nx) moment */
+  const float *mu;      /* (ny,nx) moment, FP32 (stepping) */
+  const double *mu64;   /* (ny,nx) moment, FP64 (recompute, totals, Zeeman) */
   const float *K32;     /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP32 (stepping) */
   const float *Kpo32;   /* same shape, K_perpother (stepping E_perp) */
   const double *K64;    /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP64 (recompute) */
@@ -79,8 +80,8 @@ typedef struct hs_instance {
   int32_t has_zeeman;
   int32_t uniform_beta; /* 1 when T is uniform over the grid */
   int32_t sub_of[HS_MAX_UC];    /* unit-cell site (y%uy)*ux + x%ux -> kernel index or -1 */
-  float ox_uc[HS_MAX_UC];       /* in-plane orientation per unit-cell site */
-  float oy_uc[HS_MAX_UC];
+  double ox_uc[HS_MAX_UC];      /* in-plane orientation per unit-cell site */
+  double oy_uc[HS_MAX_UC];
   float nn_mask[HS_MAX_NN];     /* exchange NN mask, [nn_h][nn_w] */
 } hs_instance_t;
 
@@ -97,8 +98,8 @@ typedef struct hs_kernel_geom {
   int32_t pbc, in_plane;
   int32_t site_x[HS_MAX_UC], site_y[HS_MAX_UC]; /* unit-cell site of kernel n */
   int32_t shift_x[HS_MAX_UC], shift_y[HS_MAX_UC]; /* tiled-matrix slice starts (energies.py:205-206) */
-  float uc_ox[HS_MAX_UC], uc_oy[HS_MAX_UC]; /* orientation per site (IP) */
-  float uc_o[HS_MAX_UC];                    /* occupation per site (OOP) */
+  double uc_ox[HS_MAX_UC], uc_oy[HS_MAX_UC]; /* orientation per site (IP) */
+  double uc_o[HS_MAX_UC];                    /* occupation per site (OOP) */
 } hs_kernel_geom_t;
 
 /* Run control for a stepping call (Magnets.progress, core.py:736-760). */
@@ -181,8 +182,15 @@ int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, double *Kps
  * compensation. Replaces Energy.update() of DipolarEnergy (energies.py:290-308), ZeemanEnergy
  * (energies.py:116-118) and ExchangeEnergy (energies.py:771-781) as called by
  * Magnets.update_energy() with index=None (core.py:247-249). which: 1 = E, 2 = E_perp, 3 = both.
+ * The dipolar sum runs per (target, source) sublattice pair as a direct correlation on the
+ * unit-cell grids, split over groups of source tiles; scratch holds the per-group partial
+ * sums (hs_recompute_scratch_bytes bytes).
  */
-int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *stream);
+int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *scratch, int64_t scratch_bytes,
+                        void *stream);
+
+/* Scratch bytes hs_recompute_fields needs for this instance. */
+int64_t hs_recompute_scratch_bytes(const hs_instance_t *inst);
 
 /*
  * Per-energy FP64 fields for the host API (Energy.E / Energy.E_perp): out[0..3] planes =

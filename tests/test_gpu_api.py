@@ -1,0 +1,138 @@
+"""The drop-in API on the device: the reference's own unit test, field steps, progress(),
+host edits of m, counters. Bars as in test_gpu_parity.py (FP32 + Kahan fields: 1e-5 of the
+field scale; FP64 recomputes: 1e-9)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nx,ny", [(5, 3), (7, 4), (10, 10), (6, 9)])
+def test_kagome_hexagon_known_answer(nx, ny):
+    """tests/test_hexagon.py:11-25 of the reference, run through hotspice_amd."""
+    import hotspice_amd as hotspice
+    l = 470e-9
+    m = 1.1278401e-15
+    a = l / np.tan(30 * np.pi / 180)
+    mm = hotspice.ASI.IP_Kagome(a, nx=nx, ny=ny, moment=m, energies=[dd := hotspice.energies.DipolarEnergy()], T=300,
+                                m_perp_factor=0, PBC=False, seed=1)
+    mm.occupation[3:, :] = mm.m[3:, :] = 0
+    mm.occupation[:, 5:] = mm.m[:, 5:] = 0
+    mm.m[(0, 0, 1), (1, 3, 4)] *= -1
+    mm.update_energy()
+    E = -1e-7 * m * m / a ** 3 * (29 + 20 * np.sqrt(3) / 9)
+    DD_m = dd.E[mm.m.astype(bool)]
+    assert np.allclose(DD_m, E, rtol=1e-9, atol=0)
+
+
+def _pair(lat="IP_Kagome", n=20, seed=3, **kw):
+    import hotspice_amd as hs
+    from oracle import model
+    rng = np.random.default_rng(seed)
+    kT = hs.kB * 300
+    probe_geom = __import__("oracle.lattices", fromlist=["make"]).make(lat, 1e-6, n=n)
+    shape = (probe_geom.ny, probe_geom.nx)
+    pat = np.where(rng.random(shape) < .5, -1., 1.)
+    EB = 40 * kT * rng.normal(1, 0.05, size=shape)
+    mm = getattr(hs.ASI, lat)(1e-6, n, T=300, E_B=EB, moment=2.37e-16, pattern=pat, seed=seed,
+                              energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(1e-3, 0.3), hs.ExchangeEnergy(J=0.2 * kT)], **kw)
+    om = model.make_magnets(lat, 1e-6, n=n, T=300, E_B=EB, moment=2.37e-16, pattern=pat, seed=seed,
+                            energies=[model.Dipolar(), model.Zeeman(1e-3, 0.3), model.Exchange(J=0.2 * kT)],
+                            params=dict(UPDATE_SCHEME="Neel"))
+    return mm, om
+
+
+def test_field_steps_match_oracle():
+    mm, om = _pair()
+    scale = np.abs(om.E).max()
+    for mag, ang in [(2e-3, 0.0), (0.5e-3, math.pi / 4), (3e-3, 2.0), (0.0, 0.0)]:
+        mm.get_energy('zeeman').set_field(mag, ang)
+        om.get_energy('zeeman').set_field(mag, ang)
+        np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(mm.E_perp, om.E_perp, rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(mm.get_energy('zeeman').E, om.get_energy('zeeman').E, rtol=1e-12, atol=1e-30)
+        assert mm.E_tot == pytest.approx(om.E_tot, rel=1e-6, abs=1e-6 * scale)
+
+
+def test_per_energy_fields_match_oracle():
+    mm, om = _pair()
+    scale = np.abs(om.E).max()
+    for name in ("dipolar", "zeeman", "exchange"):
+        np.testing.assert_allclose(mm.get_energy(name).E, om.get_energy(name).E, rtol=0, atol=1e-9 * scale)
+        np.testing.assert_allclose(mm.get_energy(name).E_perp, om.get_energy(name).E_perp, rtol=0, atol=1e-9 * scale)
+
+
+def test_neel_progress_matches_oracle():
+    mm, om = _pair(lat="IP_Pinwheel", n=16)
+    dt1, mc1 = mm.progress(t_max=math.inf, MCsteps_max=1.0)
+    dt2, mc2 = om.progress(t_max=math.inf, MCsteps_max=1.0)
+    assert mc1 == mc2 and mm.attempted_switches == om.attempted_switches
+    assert dt1 == pytest.approx(dt2, rel=1e-5)
+    np.testing.assert_array_equal(mm.m, om.m)
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * scale)
+
+
+def test_neel_progress_t_max():
+    """Finite t_max: stops once the elapsed time reaches t_max (core.py:701-704, 756-757)."""
+    mm, om = _pair(lat="IP_Pinwheel", n=16)
+    dt1, mc1 = mm.progress(t_max=1e-6, MCsteps_max=math.inf)
+    dt2, mc2 = om.progress(t_max=1e-6, MCsteps_max=math.inf)
+    assert mm.attempted_switches == om.attempted_switches
+    assert dt1 == pytest.approx(dt2, rel=1e-5)
+    np.testing.assert_array_equal(mm.m, om.m)
+
+
+def test_metropolis_progress_matches_oracle():
+    import hotspice_amd as hs
+    from oracle import model
+    rng = np.random.default_rng(8)
+    n = 40
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    kT = hs.kB * 300
+    mm = hs.ASI.OOP_Square(1e-6, n, T=300, moment=1.1e-15, pattern=pat, seed=8,
+                           energies=[hs.DipolarEnergy(cutoff_range=6e-6), hs.ExchangeEnergy(J=0.3 * kT)],
+                           params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=6,
+                                               SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=3))
+    om = model.make_magnets("OOP_Square", 1e-6, n=n, T=300, moment=1.1e-15, pattern=pat, seed=8,
+                            energies=[model.Dipolar(cutoff_range=6e-6), model.Exchange(J=0.3 * kT)],
+                            params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,
+                                        SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=3))
+    r1 = mm.progress(MCsteps_max=3.0, r=3e-6)
+    r2 = om.progress(MCsteps_max=3.0, r=3e-6)
+    assert r1 == pytest.approx(r2)
+    assert mm.attempted_switches == om.attempted_switches and mm.switches == om.switches
+    np.testing.assert_array_equal(mm.m, om.m)
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * scale)
+
+
+def test_host_edit_then_update_energy():
+    mm, om = _pair()
+    m = mm.m
+    occ = np.argwhere(mm.occupation)[::7]
+    m[occ[:, 0], occ[:, 1]] *= -1
+    om.m[occ[:, 0], occ[:, 1]] *= -1
+    mm.update_energy()
+    om.update_energy()
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-9 * scale)
+    np.testing.assert_array_equal(mm.m, om.m)
+
+
+def test_counters_and_reset():
+    mm, _ = _pair(lat="IP_Pinwheel", n=16)
+    mm.progress(t_max=math.inf, MCsteps_max=0.25)
+    assert mm.attempted_switches == math.ceil(0.25 * mm.n)
+    assert 0 < mm.switches <= mm.attempted_switches and mm.t > 0
+    mm.reset_stats()
+    assert mm.t == 0 and mm.switches == 0 and mm.attempted_switches == 0 and mm.MCsteps == 0
+
+
+def test_zero_temperature_returns_none():
+    import hotspice_amd as hs
+    mm = hs.ASI.OOP_Square(1e-6, 8, T=0, seed=1, params=hs.SimParams(UPDATE_SCHEME='Néel'))
+    with pytest.warns(UserWarning):
+        assert mm.update() is None

@@ -46,7 +46,7 @@ def test_argument_errors_need_no_device():
     rc = lib.hs_philox_uniform53(1, 0, 1, -1, 10, 0, None, None)
     assert rc == -1 and b"philox" in lib.hs_last_error()
     inst = _lib.Instance()
-    rc = lib.hs_recompute_fields(C.byref(inst), 1, None)
+    rc = lib.hs_recompute_fields(C.byref(inst), 1, None, 0, None)
     assert rc == -1 and b"grid" in lib.hs_last_error()
     g = _lib.KernelGeom()
     rc = lib.hs_build_dipolar_kernels(C.byref(g), None, None, None, None, None, None)
