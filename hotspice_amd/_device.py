@@ -49,6 +49,7 @@ class DeviceState:
         self.kern = None  # dict of kernel tensors + window
         self.metro_ws = None
         self.rec_scratch = None
+        self.team_ws = None
         self.sel_cache = {}
         self.log_cap = 0
         self.log_idx = None
@@ -147,14 +148,34 @@ class DeviceState:
             self.log_dt = self.torch.zeros(cap, dtype=self.torch.float64, device=self.device)
             self.log_cap = cap
 
-    def neel_run(self, run: _lib.Run):
+    def neel_run(self, run: _lib.Run, persistent=True, nblocks=0):
+        """Neel steps: one persistent team launch (default) or one launch per step."""
         if run.log:
             self.ensure_log(run.max_steps)
+        if persistent:
+            team = _lib.Team(inst=C.addressof(self.inst), sel=None, run=run, state=self.state.data_ptr(),
+                             log_idx=self.log_idx.data_ptr() if run.log else None,
+                             log_dt=self.log_dt.data_ptr() if run.log else None, log_cells=None, nblocks=int(nblocks))
+            self.run_teams([team], "neel")
+            return
         fc = C.c_int32(0)
         _lib.check(self.lib.hs_neel_run(C.byref(self.inst), C.byref(run), self.ptr(self.state), self.ptr(self.partials),
                                         self.ptr(self.log_idx) if run.log else C.c_void_p(0),
                                         self.ptr(self.log_dt) if run.log else C.c_void_p(0), C.byref(fc),
                                         self.stream()))
+
+    def run_teams(self, teams, kind):
+        """One cooperative launch stepping every team (instance) in `teams`."""
+        arr = (_lib.Team * len(teams))(*teams)
+        lib = self.lib
+        size_fn, run_fn = (lib.hs_neel_teams_ws_bytes, lib.hs_neel_run_teams) if kind == "neel" else \
+            (lib.hs_metro_teams_ws_bytes, lib.hs_metro_run_teams)
+        need = int(size_fn(arr, len(teams)))
+        if need < 0:
+            raise _lib.HotspiceError("bad team descriptors")
+        if self.team_ws is None or self.team_ws.numel() < need:
+            self.team_ws = self.torch.zeros(max(need, 256), dtype=self.torch.uint8, device=self.device)
+        _lib.check(run_fn(arr, len(teams), self.ptr(self.team_ws), self.team_ws.numel(), self.stream()))
 
     def metropolis_run(self, sel: _lib.Select, run: _lib.Run, log_cells=None, log_count=None):
         need = int(self.lib.hs_metropolis_ws_bytes(C.byref(self.inst), C.byref(sel)))

@@ -19,7 +19,7 @@ TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME = 1, 2, 3, 4, 5
 
 EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs_philox_uniform53", "hs_build_dipolar_kernels",
            "hs_recompute_fields", "hs_recompute_scratch_bytes", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
-           "hs_metropolis_run", "hs_metropolis_ws_bytes")
+           "hs_metropolis_run", "hs_metropolis_ws_bytes", "hs_neel_run_teams", "hs_neel_teams_ws_bytes")
 
 
 class Instance(C.Structure):
@@ -74,6 +74,13 @@ class Select(C.Structure):
                 ("metropolis_time", C.c_int32)]
 
 
+class Team(C.Structure):
+    """hs_team_t"""
+    _fields_ = [("inst", C.c_void_p), ("sel", C.c_void_p), ("run", Run), ("state", C.c_void_p),
+                ("log_idx", C.c_void_p), ("log_dt", C.c_void_p), ("log_cells", C.c_void_p),
+                ("nblocks", C.c_int32), ("pad", C.c_int32)]
+
+
 STATE_BYTES = C.sizeof(State)
 
 _lib = None
@@ -107,6 +114,8 @@ def load():
         "hs_neel_run": ([C.POINTER(Instance), C.POINTER(Run), vp, vp, vp, vp, C.POINTER(i32), vp], C.c_int),
         "hs_metropolis_run": ([C.POINTER(Instance), C.POINTER(Select), C.POINTER(Run), vp, vp, vp, vp, vp], C.c_int),
         "hs_metropolis_ws_bytes": ([C.POINTER(Instance), C.POINTER(Select)], C.c_int64),
+        "hs_neel_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),
+        "hs_neel_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -130,15 +139,15 @@ def exported_symbols():
 
 
 def abi_layout_python():
-    """The same 15 numbers as hs_abi_layout, from the ctypes mirrors."""
+    """The same 18 numbers as hs_abi_layout, from the ctypes mirrors."""
     return [C.sizeof(Instance), Instance.seed.offset, Instance.nx.offset, Instance.sub_of.offset,
             Instance.nn_mask.offset, C.sizeof(KernelGeom), KernelGeom.nx.offset, KernelGeom.uc_o.offset,
             C.sizeof(Run), Run.max_steps.offset, Run.begin.offset, C.sizeof(State), State.done.offset,
-            C.sizeof(Select), Select.metropolis_time.offset]
+            C.sizeof(Select), Select.metropolis_time.offset, C.sizeof(Team), Team.run.offset, Team.nblocks.offset]
 
 
 def abi_layout_library():
     lib = load()
-    out = (C.c_int64 * 15)()
-    n = lib.hs_abi_layout(out, 15)
+    out = (C.c_int64 * 18)()
+    n = lib.hs_abi_layout(out, 18)
     return list(out)[:n]

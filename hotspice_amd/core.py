@@ -100,6 +100,8 @@ class Magnets(ABC):
                  params: SimParams = None, in_plane: bool = False, seed: int = None, device: int = None,
                  recompute_every: int | None = 1024):
         self.seed = int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.neel_persistent = True  # one persistent launch per progress() chunk (False: one launch per step)
+        self.neel_blocks = 0         # workgroups in the instance's team (0: automatic)
         self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
         self.params = SimParams() if params is None else params
         self.recompute_every = recompute_every
@@ -704,7 +706,7 @@ class Magnets(ABC):
                        nblocks=0, begin=1)
         if not self._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-        self._dev.neel_run(run)
+        self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self.neel_blocks)
         self._after_device_step()
         idx = int(self._dev.log_idx[0].item())
         st = self._state()
@@ -846,7 +848,7 @@ class Magnets(ABC):
                     raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
                 if begin:
                     self._set_progress_origin(t0, a0)
-                self._dev.neel_run(run)
+                self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self.neel_blocks)
                 begin = False
                 st = self._state()
                 if st.err:

@@ -54,7 +54,8 @@ extern "C" int hs_abi_layout(int64_t *out, int32_t n) {
       (int64_t)sizeof(hs_kernel_geom_t), (int64_t)offsetof(hs_kernel_geom_t, nx), (int64_t)offsetof(hs_kernel_geom_t, uc_o),
       (int64_t)sizeof(hs_run_t), (int64_t)offsetof(hs_run_t, max_steps), (int64_t)offsetof(hs_run_t, begin),
       (int64_t)sizeof(hs_state_t), (int64_t)offsetof(hs_state_t, done),
-      (int64_t)sizeof(hs_select_t), (int64_t)offsetof(hs_select_t, metropolis_time)};
+      (int64_t)sizeof(hs_select_t), (int64_t)offsetof(hs_select_t, metropolis_time),
+      (int64_t)sizeof(hs_team_t), (int64_t)offsetof(hs_team_t, run), (int64_t)offsetof(hs_team_t, nblocks)};
   const int cnt = (int)(sizeof(v) / sizeof(v[0]));
   if (!out || n < cnt) return cnt;
   for (int i = 0; i < cnt; ++i) out[i] = v[i];

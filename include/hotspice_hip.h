@@ -140,6 +140,24 @@ typedef struct hs_select {
   int32_t metropolis_time;  /* SimParams.METROPOLIS_TIME */
 } hs_select_t;
 
+/*
+ * One spin-ice instance of a batched (persistent, multi-instance) launch. Every instance gets
+ * a team of workgroups that steps it independently of the other teams; a team of one
+ * workgroup synchronises with barriers only, larger teams through a bounded-spin atomic
+ * barrier of their own.
+ */
+typedef struct hs_team {
+  const hs_instance_t *inst;  /* HOST pointer to the instance descriptor (copied by the call) */
+  const hs_select_t *sel;     /* HOST pointer, Metropolis teams only */
+  hs_run_t run;
+  hs_state_t *state;          /* device state (copy 0 is read and written) */
+  int32_t *log_idx;           /* optional device logs: Neel winner (-1 = no switch) / Metropolis counts */
+  double *log_dt;             /* Neel: time added per step */
+  int32_t *log_cells;         /* Metropolis: accepted cells, capacity sel->nsc per step */
+  int32_t nblocks;            /* workgroups of this team (0 = automatic) */
+  int32_t pad;
+} hs_team_t;
+
 /* ---- entry points ----------------------------------------------------------------------- */
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -152,8 +170,9 @@ const char *hs_last_error(void);
  * Layout of the ABI structs for binding self-checks: writes {sizeof(hs_instance_t),
  * offsetof seed, nx, sub_of, nn_mask; sizeof(hs_kernel_geom_t), offsetof nx, uc_o;
  * sizeof(hs_run_t), offsetof max_steps, begin; sizeof(hs_state_t), offsetof done;
- * sizeof(hs_select_t), offsetof metropolis_time} to out[0..14] when n >= 15. Returns the
- * count (15). Touches no device. No counterpart in the reference (binding infrastructure).
+ * sizeof(hs_select_t), offsetof metropolis_time; sizeof(hs_team_t), offsetof run, nblocks}
+ * to out[0..17] when n >= 18. Returns the count (18). Touches no device. No counterpart in
+ * the reference (binding infrastructure).
  */
 int hs_abi_layout(int64_t *out, int32_t n);
 
@@ -242,6 +261,17 @@ int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const h
 
 /* Scratch bytes hs_metropolis_run needs for this instance and selection geometry. */
 int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel);
+
+/*
+ * Neel-Arrhenius steps for many instances in ONE persistent cooperative launch (same
+ * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760). Each
+ * team keeps its magnets' fields in registers, draws the same Philox streams and reduces its
+ * winner with an atomic (key, index) minimum plus a team barrier per step; teams never wait
+ * for each other. ws: device scratch of hs_neel_teams_ws_bytes bytes. Blocking for the small
+ * descriptor upload, otherwise asynchronous.
+ */
+int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
+int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,78 @@
+"""Persistent team stepping: identical to the per-step launches, and an instance gives the same
+result alone as inside a batch (bit-exact: same Philox streams, same arithmetic)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pinwheel(n, seed, **kw):
+    import hotspice_amd as hs
+    rng = np.random.default_rng(seed)
+    kT = hs.kB * 300
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    EB = 60 * kT * np.exp(0.05 * rng.normal(size=(n, n)))
+    return hs.ASI.IP_Pinwheel(1e-6, n, T=300, E_B=EB, moment=2.37e-16, pattern=pat, seed=seed,
+                              energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(1e-3, math.pi / 8)],
+                              params=hs.SimParams(UPDATE_SCHEME='Néel'), **kw)
+
+
+def _oop(n, seed, **kw):
+    import hotspice_amd as hs
+    rng = np.random.default_rng(seed)
+    kT = hs.kB * 300
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    return hs.ASI.OOP_Square(1e-6, n, T=300, E_B=20 * kT, moment=1.1e-15, pattern=pat, seed=seed,
+                             energies=[hs.DipolarEnergy(), hs.ExchangeEnergy(J=0.3 * kT)],
+                             params=hs.SimParams(UPDATE_SCHEME='Néel', ENERGY_BARRIER_METHOD='parabolic'), **kw)
+
+
+def _snapshot(mm):
+    d = mm._dev
+    return (d.m.cpu().numpy().copy(), d.E.cpu().numpy().copy(), d.Ec.cpu().numpy().copy(),
+            d.Ep.cpu().numpy().copy(), mm.t, mm.attempted_switches, mm.switches)
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        if isinstance(x, np.ndarray):
+            np.testing.assert_array_equal(x, y)
+        else:
+            assert x == y
+
+
+@pytest.mark.parametrize("make,n", [(_pinwheel, 32), (_oop, 24)])
+@pytest.mark.parametrize("nblocks", [1, 2, 5, 0])
+def test_team_equals_per_step_launches(make, n, nblocks):
+    a = make(n, 3, recompute_every=None)
+    b = make(n, 3, recompute_every=None)
+    a.neel_persistent = False
+    b.neel_blocks = nblocks
+    a.progress(t_max=math.inf, MCsteps_max=1.5)
+    b.progress(t_max=math.inf, MCsteps_max=1.5)
+    _same(_snapshot(a), _snapshot(b))
+
+
+def test_team_update_returns_index():
+    a = _pinwheel(16, 4)
+    b = _pinwheel(16, 4)
+    b.neel_persistent = False
+    for _ in range(30):
+        ra, rb = a.update(t_max=math.inf), b.update(t_max=math.inf)
+        assert tuple(map(int, ra)) == tuple(map(int, rb))
+    _same(_snapshot(a), _snapshot(b))
+
+
+def test_batch_equals_alone():
+    from hotspice_amd import batch
+    sizes = [(16, 11), (40, 12), (24, 13), (64, 14)]
+    alone = [_pinwheel(n, s, recompute_every=64) for n, s in sizes]
+    for m in alone:
+        m.progress(t_max=math.inf, MCsteps_max=0.75)
+    mixed = [_pinwheel(n, s, recompute_every=64) for n, s in sizes]
+    res = batch.progress_many(mixed, t_max=math.inf, MCsteps_max=0.75, nblocks=[1, 3, 2, 0])
+    for m1, m2, r in zip(alone, mixed, res):
+        _same(_snapshot(m1), _snapshot(m2))
+        assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
