@@ -135,7 +135,7 @@ def main():
                    "switches": mm.switches, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_neel_step", "avg_launch_us": neel_us, "bytes_per_launch": bytes_per},
+                     "kernel": "k_neel_team", "avg_launch_us": neel_us, "bytes_per_launch": bytes_per},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -49,6 +49,28 @@ HS_INLINE double hs_exp1_f64(uint64_t bits) {
   return -log1p(-(double)((1ull << 53) - 1 - bits) * 0x1p-53);
 }
 
+// Neel waiting time of a cell in FP64 from its compensated fields (value - compensation):
+// tau*exp(bmin*beta)/f0 == -ln(u)*exp(b*beta)/f0 (core.py:698-700), or with two channels
+// -ln(u)/(exp(-b1 beta)+exp(-b2 beta))/(f0/2) (core.py:686-693).
+HS_INLINE double hs_neel_dt(double e, double ep, double eb, double beta, double m_perp, int parabolic, bool perp,
+                            uint64_t bits, double f0) {
+  const double dE = -2.0 * e, El = e, x = hs_exp1_f64(bits);
+  if (perp) {
+    const double epf = m_perp * ep;
+    const double b1 = fmax(dE, (eb + epf) - El), b2 = fmax(dE, (eb - epf) - El);
+    const double lo = fmin(b1, b2), hi = fmax(b1, b2);
+    return x * exp(lo * beta) / (1.0 + exp(-(hi - lo) * beta)) / (f0 / 2.0);
+  }
+  double b;
+  if (!parabolic) {
+    b = fmax(dE, eb - El);
+  } else {
+    const double q = dE * (0.25 / eb) + 1.0;
+    b = dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
+  }
+  return x * exp(b * beta) / f0;
+}
+
 // ---- Kahan-compensated FP32 accumulation (value = s - c) ------------------------------------
 HS_INLINE void hs_kahan_add(float &s, float &c, float x) {
   const float y = x - c;

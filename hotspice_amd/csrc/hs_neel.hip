@@ -52,30 +52,6 @@ HS_INLINE float log_rate_f32(const hs_instance_t &I, float e, float ep, float eb
   return -b * beta;
 }
 
-// FP64 waiting time of cell c (the winner): tau*exp(bmin*beta)/f0 == -ln(u)*exp(b*beta)/f0.
-template <bool PERP>
-HS_INLINE double dt_f64(const hs_instance_t &I, int c, uint64_t bits, double f0) {
-  const double e = (double)I.E[c] - (double)I.Ec[c];
-  const double dE = -2.0 * e, El = e;
-  const double eb = (double)I.EB[c], beta = (double)I.beta[c];
-  const double x = hs_exp1_f64(bits);
-  if (PERP) {
-    const double ep = I.m_perp_factor * ((double)I.Ep[c] - (double)I.Epc[c]);
-    const double b1 = fmax(dE, (eb + ep) - El), b2 = fmax(dE, (eb - ep) - El);
-    const double lo = fmin(b1, b2), hi = fmax(b1, b2);
-    // attempt_freq /= 2 with two channels (core.py:693)
-    return x * exp(lo * beta) / (1.0 + exp(-(hi - lo) * beta)) / (f0 / 2.0);
-  }
-  double b;
-  if (I.barrier == 0) {
-    b = fmax(dE, eb - El);
-  } else {
-    const double q = dE * (0.25 / eb) + 1.0;
-    b = dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
-  }
-  return x * exp(b * beta) / f0;
-}
-
 template <bool PERP>
 __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_run_t R, hs_state_t *st,
                                                         NeelPartial *parts, int32_t *log_idx, double *log_dt,
@@ -295,9 +271,12 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
     p.dt = NAN;
     p.mw = 0;
     if (p.idx >= 0) {
-      const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)p.idx);
-      p.dt = dt_f64<PERP>(I, p.idx, bits, R.attempt_freq);
-      p.mw = I.m[p.idx];
+      const int c = p.idx;
+      const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)c);
+      const double ep64 = PERP ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
+      p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], (double)I.beta[c], I.m_perp_factor,
+                        I.barrier, PERP, bits, R.attempt_freq);
+      p.mw = I.m[c];
     }
     parts[(size_t)(j & 1) * P + blockIdx.x] = p;
   }

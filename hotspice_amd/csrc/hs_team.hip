@@ -24,7 +24,7 @@
 
 namespace {
 
-constexpr int TT = 256;            // threads per workgroup
+constexpr int TT = 1024;           // threads per workgroup (16 waves: latency hiding)
 constexpr uint64_t SPIN_LIMIT = 1ull << 26;
 
 struct TeamDev {
@@ -35,8 +35,8 @@ struct TeamDev {
   double *log_dt;
   uint32_t *bar;
   unsigned long long *minslot;  // [3]
-  double *dtslot;               // [3][G]
-  int32_t *mwslot;              // [3][G]
+  double *dtslot;               // [3][G] waiting time of each workgroup's best cell
+  uint32_t *dtflag;             // [3][G] round tag: the slot holds this round's value
   int32_t G, first;
 };
 
@@ -44,6 +44,8 @@ HS_INLINE uint32_t key_bits(float k) {
   uint32_t u = __float_as_uint(k);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+
+
 
 template <bool PERP>
 HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float eb, float beta) {
@@ -64,37 +66,24 @@ HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float e
   return -b * beta;
 }
 
-template <bool PERP>
-HS_INLINE double team_dt(const hs_instance_t &I, double e, double ep, double eb, double beta, uint64_t bits, double f0) {
-  const double dE = -2.0 * e, El = e, x = hs_exp1_f64(bits);
-  if (PERP) {
-    const double epf = I.m_perp_factor * ep;
-    const double b1 = fmax(dE, (eb + epf) - El), b2 = fmax(dE, (eb - epf) - El);
-    const double lo = fmin(b1, b2), hi = fmax(b1, b2);
-    return x * exp(lo * beta) / (1.0 + exp(-(hi - lo) * beta)) / (f0 / 2.0);
+HS_INLINE unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    v = o < v ? o : v;
   }
-  double b;
-  if (I.barrier == 0) {
-    b = fmax(dE, eb - El);
-  } else {
-    const double q = dE * (0.25 / eb) + 1.0;
-    b = dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
-  }
-  return x * exp(b * beta) / f0;
+  return v;
 }
-
-HS_INLINE bool team_better(float k2, int i2, float k, int i) { return k2 < k || (k2 == k && i2 < i); }
 
 template <int MAXC, bool PERP>
 __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const int32_t *block_team, int32_t *err_out) {
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
-  __shared__ float s_k[TT / 64];
-  __shared__ int s_i[TT / 64];
-  __shared__ double s_d[TT / 64];
-  __shared__ int s_m[TT / 64];
-  __shared__ int s_w, s_mw, s_bad;
-  __shared__ double s_dt;
+  __shared__ unsigned long long s_part[TT / 64];
+  __shared__ unsigned long long s_block, s_win;
+  __shared__ double s_dtw;
+  __shared__ int s_bad;
   const int tid = threadIdx.x;
   const int team = block_team[blockIdx.x];
   if (tid == 0) sT = teams[team];
@@ -145,117 +134,118 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   int64_t sw = S0.switches, step = S0.step, sd = 0;
   int err = 0;
   uint32_t rounds = 0;
+  // lower_than(att/n - att0/n, MCsteps_max) is monotone in att: find the first att where it
+  // fails once, so the per-step test is an integer compare (same FP64 expression as core.py:756)
+  int64_t att_stop = INT64_MAX;
+  if (isfinite(R.mc_max) && R.mc_max * nocc < 9.0e18) {
+    auto more = [&](int64_t a) { return hs_lower_than((double)a / nocc - (double)att0 / nocc, R.mc_max); };
+    int64_t a = att0 + (int64_t)ceil(R.mc_max * nocc);
+    if (a < att) a = att;
+    while (more(a)) ++a;
+    while (a > att && !more(a - 1)) --a;
+    att_stop = a;
+  }
+  const int lane = tid & 63, wv = tid >> 6;
 
   for (;;) {
-    if (!(sd < R.max_steps && hs_lower_than(t - t0, R.t_max) && hs_lower_than((double)att / nocc - (double)att0 / nocc, R.mc_max)))
-      break;
-    // 1. keys of the owned cells
-    float bk = INFINITY;
-    int bi = 0x7fffffff, bkk = -1;
+    if (!(sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max))) break;
+    // 1. keys of the owned cells, packed (key bits | cell | m+1) so one 64-bit minimum picks the
+    //    lowest key and, on ties, the lowest flat index (nanargmin's first index)
+    unsigned long long best = ~0ull;
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
       if (cell[k] >= 0 && cand[k]) {
         const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)cell[k]);
         float key = logf(hs_exp1_f32(bits)) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], bt[k]);
         if (isnan(key)) key = INFINITY;
-        if (team_better(key, cell[k], bk, bi)) {
-          bk = key;
-          bi = cell[k];
-          bkk = k;
-        }
+        const unsigned long long p = ((unsigned long long)key_bits(key) << 32) | ((uint32_t)cell[k] << 2) | (uint32_t)(mm[k] + 1);
+        best = p < best ? p : best;
       }
     }
-    double bdt = NAN;
-    int bmw = 0;
-#pragma unroll
-    for (int k = 0; k < MAXC; ++k) {
-      if (k == bkk) {
-        const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)cell[k]);
-        bdt = team_dt<PERP>(I, (double)e[k] - (double)ec[k], (double)ep[k] - (double)epc[k], (double)eb[k], (double)bt[k],
-                            bits, R.attempt_freq);
-        bmw = mm[k];
-      }
-    }
-    // workgroup argmin with payload (dt, m)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float k2 = __shfl_xor(bk, off, 64);
-      const int i2 = __shfl_xor(bi, off, 64);
-      const double d2 = __shfl_xor(bdt, off, 64);
-      const int m2 = __shfl_xor(bmw, off, 64);
-      if (team_better(k2, i2, bk, bi)) {
-        bk = k2;
-        bi = i2;
-        bdt = d2;
-        bmw = m2;
-      }
-    }
-    if ((tid & 63) == 0) {
-      s_k[tid >> 6] = bk;
-      s_i[tid >> 6] = bi;
-      s_d[tid >> 6] = bdt;
-      s_m[tid >> 6] = bmw;
+    best = wave_min_u64(best);
+    if (lane == 0) s_part[wv] = best;
+    __syncthreads();
+    if (wv == 0) {
+      unsigned long long v = lane < TT / 64 ? s_part[lane] : ~0ull;
+      v = wave_min_u64(v);
+      if (lane == 0) s_block = v;
     }
     __syncthreads();
-    if (tid == 0) {
-      for (int q = 1; q < TT / 64; ++q)
-        if (team_better(s_k[q], s_i[q], bk, bi)) {
-          bk = s_k[q];
-          bi = s_i[q];
-          bdt = s_d[q];
-          bmw = s_m[q];
+    const unsigned long long bb = s_block;
+    const int bcell = bb == ~0ull ? -1 : (int)((bb & 0xffffffffull) >> 2);
+    const int sl = (int)(sd % 3);
+    // 2a. the owner of the workgroup's best cell evaluates its FP64 waiting time from the
+    //     compensated fields and publishes it (program order before the exchange: a wave never
+    //     waits on its own lanes)
+#pragma unroll
+    for (int k = 0; k < MAXC; ++k) {
+      if (cell[k] == bcell) {
+        const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)cell[k]);
+        const double dtk = hs_neel_dt((double)e[k] - (double)ec[k], PERP ? (double)ep[k] - (double)epc[k] : 0.0,
+                                      (double)eb[k], (double)bt[k], I.m_perp_factor, I.barrier, PERP, bits,
+                                      R.attempt_freq);
+        if (G > 1) {
+          __hip_atomic_store(&sT.dtslot[sl * G + rank], dtk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __builtin_amdgcn_s_waitcnt(0);
+          __hip_atomic_store(&sT.dtflag[sl * G + rank], rounds + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          s_dtw = dtk;
         }
-      // 2. team winner
-      int w = bi == 0x7fffffff ? -1 : bi;
-      double dt = bdt;
-      int mw = bmw;
+      }
+    }
+    // 2. team winner: one 64-bit atomic minimum + counter barrier (bounded spins)
+    if (tid == 0) {
+      unsigned long long win = bb;
       int bad = 0;
       if (G > 1) {
-        const int sl = (int)(sd % 3);
-        __hip_atomic_store(&sT.dtslot[sl * G + rank], bdt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&sT.mwslot[sl * G + rank], bmw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long packed = ((unsigned long long)key_bits(bk) << 32) | (uint32_t)bi;
-        __hip_atomic_fetch_min(&sT.minslot[sl], packed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_min(&sT.minslot[sl], bb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (rank == 0) __hip_atomic_store(&sT.minslot[(sl + 1) % 3], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(sT.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // all team data travels through device-scope atomics performed at the coherence point, so
+        // the arrival only waits for them to complete: no agent-scope release/acquire, which would
+        // write back and invalidate the L2 holding the kernels every step
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_fetch_add(sT.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t target = (rounds + 1) * (uint32_t)G;
         uint64_t spins = 0;
-        while (__hip_atomic_load(sT.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(sT.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
           if (++spins > SPIN_LIMIT) {
             bad = 1;
             break;
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        const unsigned long long win = __hip_atomic_load(&sT.minslot[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t wi = (uint32_t)(win & 0xffffffffull);
-        if (bad || wi == 0x7fffffffu || wi == 0xffffffffu) {
-          w = -1;
-        } else {
-          w = (int)wi;
-          const int owner = (w / TT) % G;
-          dt = __hip_atomic_load(&sT.dtslot[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          mw = __hip_atomic_load(&sT.mwslot[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        win = __hip_atomic_load(&sT.minslot[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!bad && win != ~0ull) {  // the winner's time, once its owner's slot shows this round
+          const int owner = ((int)((win & 0xffffffffull) >> 2) / TT) % G;
+          while (__hip_atomic_load(&sT.dtflag[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != rounds + 1u) {
+            if (++spins > SPIN_LIMIT) {
+              bad = 1;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          s_dtw = __hip_atomic_load(&sT.dtslot[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      s_w = w;
-      s_dt = dt;
-      s_mw = mw;
+      s_win = win;
       s_bad = bad;
     }
     __syncthreads();
-    const int w = s_w;
-    const double dt = s_dt;
-    const int mw = s_mw;
+    const unsigned long long win = s_win;
     ++rounds;
     if (s_bad) {
       err = 3;  // team barrier timed out
       break;
     }
-    if (w < 0) {
+    if (win == ~0ull) {
       err = 1;  // nanargmin over an all-NaN grid
       break;
     }
+    const int w = (int)((win & 0xffffffffull) >> 2);
+    const int mw = (int)(win & 3ull) - 1;
+    const double dt = s_dtw;
     // 3. time, counters (core.py:700-707) and the field update of the winner
     const double tmax_step = R.t_max - (t - t0);
     att += 1;
@@ -277,6 +267,21 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       const float oxw = (float)I.ox_uc[sitew], oyw = (float)I.oy_uc[sitew];
       const float *Kw = nw >= 0 ? I.K32 + (size_t)nw * KS : nullptr;
       const float *Kpw = (PERP && nw >= 0) ? I.Kpo32 + (size_t)nw * KS : nullptr;
+      float kv[MAXC], kpv[MAXC];  // the winner's kernel entries for every owned cell, loaded together
+#pragma unroll
+      for (int k = 0; k < MAXC; ++k) {
+        kv[k] = 0.f;
+        kpv[k] = 0.f;
+        const int c = cell[k];
+        if (c >= 0 && Kw) {
+          const int dy = c / nx - yw, dx = c % nx - xw;
+          if (dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx) {
+            const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
+            kv[k] = Kw[o];
+            if (PERP) kpv[k] = Kpw[o];
+          }
+        }
+      }
 #pragma unroll
       for (int k = 0; k < MAXC; ++k) {
         const int c = cell[k];
@@ -296,11 +301,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         const int dy = y - yw, dx = x - xw;
         float de = 0.f, dep = 0.f;
         const float mmu = (float)mm[k] * mu[k];
-        if (Kw && dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx) {
-          const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
-          de = A * (mmu * Kw[o]);
-          if (PERP) dep = A * (mmu * Kpw[o]);
-        }
+        de = A * (mmu * kv[k]);
+        if (PERP) dep = A * (mmu * kpv[k]);
         if (Ax != 0.f) {
           int qy = dy, qx = dx;
           if (I.pbc) {
@@ -366,6 +368,7 @@ __global__ void k_team_reset(TeamDev *teams, int nteams, int32_t *err) {
   TeamDev d = teams[t];
   *d.bar = 0u;
   d.minslot[0] = d.minslot[1] = d.minslot[2] = ~0ull;
+  for (int i = 0; i < 3 * d.G; ++i) d.dtflag[i] = 0u;
 }
 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -380,9 +383,9 @@ struct Layout {
 int team_blocks(const hs_team_t &t) {
   const int ncell = t.inst->nx * t.inst->ny;
   if (t.nblocks > 0) return t.nblocks;
-  // default: up to 4 cells per thread, at most 256 workgroups
-  int g = (ncell + TT * 4 - 1) / (TT * 4);
-  return g < 1 ? 1 : (g > 256 ? 256 : g);
+  // default: one cell per thread, at most 128 workgroups
+  int g = (ncell + TT - 1) / TT;
+  return g < 1 ? 1 : (g > 128 ? 128 : g);
 }
 
 int make_layout(const hs_team_t *teams, int n, Layout &L) {
@@ -492,7 +495,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     d.bar = (uint32_t *)sy;
     d.minslot = (unsigned long long *)(sy + 256);
     d.dtslot = (double *)(sy + 256 + al256(24));
-    d.mwslot = (int32_t *)(sy + 256 + al256(24) + al256(24 * (size_t)L.G[i]));
+    d.dtflag = (uint32_t *)(sy + 256 + al256(24) + al256(24 * (size_t)L.G[i]));
     d.G = L.G[i];
     d.first = L.first[i];
     td[i] = d;
