@@ -22,12 +22,19 @@ HS_INLINE uint4 hs_philox(uint4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-// 53-bit integer of stream (seed, step, tag) at `index`: two 32-bit words of Philox call index>>1.
-HS_INLINE uint64_t hs_bits53(uint64_t seed, uint64_t step, uint32_t tag, uint64_t index) {
+// The two 32-bit words (a, b) of stream (seed, step, tag) at `index`: words 0,1 (even index) or
+// 2,3 (odd index) of Philox call index>>1.
+HS_INLINE void hs_words(uint64_t seed, uint64_t step, uint32_t tag, uint64_t index, uint32_t &a, uint32_t &b) {
   const uint4 w = hs_philox(make_uint4((uint32_t)(index >> 1), (uint32_t)step, (uint32_t)(step >> 32), tag),
                             (uint32_t)seed, (uint32_t)(seed >> 32));
-  const uint32_t a = (index & 1) ? w.z : w.x;
-  const uint32_t b = (index & 1) ? w.w : w.y;
+  a = (index & 1) ? w.z : w.x;
+  b = (index & 1) ? w.w : w.y;
+}
+
+// 53-bit integer (a>>5)*2^26 + (b>>6) of stream (seed, step, tag) at `index`.
+HS_INLINE uint64_t hs_bits53(uint64_t seed, uint64_t step, uint32_t tag, uint64_t index) {
+  uint32_t a, b;
+  hs_words(seed, step, tag, index, a, b);
   return ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
 }
 
@@ -36,30 +43,64 @@ HS_INLINE double hs_u53(uint64_t bits, int open_zero) {
   return (double)(bits + (uint64_t)open_zero) * 0x1p-53;
 }
 
-// -ln(u) for u = (bits+1)*2^-53 in (0,1], FP32 with full relative precision at both ends.
-HS_INLINE float hs_exp1_f32(uint64_t bits) {
-  if (bits < (1ull << 52)) return -logf((float)(bits + 1) * 0x1p-53f);
-  const float w = (float)((1ull << 53) - 1 - bits) * 0x1p-53f;  // 1-u, exact integer then rounded
-  return -log1pf(-w);
+// -ln(u) in FP32 for u = (bits53 + 1)*2^-53 in (0,1], given the two words behind bits53, without
+// branches: u and w = 1-u are both formed from exact 27- and 26-bit integer parts (one rounding
+// each); for w < 1/4 the series sum_{n<=11} w^n/n (truncation < 3e-8 relative), otherwise
+// -log(u) from the hardware log2. Relative error < ~2e-7 over the whole range.
+HS_INLINE float hs_exp1_fast(uint32_t a, uint32_t b) {
+  const float u = __builtin_fmaf((float)(a >> 5), 0x1p-27f, (float)((b >> 6) + 1u) * 0x1p-53f);
+  const float w = __builtin_fmaf((float)(~a >> 5), 0x1p-27f, (float)(~b >> 6) * 0x1p-53f);
+  const float L = -__builtin_amdgcn_logf(u) * 0.693147180559945f;
+  float p = 1.f / 11.f;
+  p = __builtin_fmaf(p, w, 1.f / 10.f);
+  p = __builtin_fmaf(p, w, 1.f / 9.f);
+  p = __builtin_fmaf(p, w, 1.f / 8.f);
+  p = __builtin_fmaf(p, w, 1.f / 7.f);
+  p = __builtin_fmaf(p, w, 1.f / 6.f);
+  p = __builtin_fmaf(p, w, 1.f / 5.f);
+  p = __builtin_fmaf(p, w, 1.f / 4.f);
+  p = __builtin_fmaf(p, w, 1.f / 3.f);
+  p = __builtin_fmaf(p, w, 1.f / 2.f);
+  p = __builtin_fmaf(p, w, 1.f);
+  return w < 0.25f ? w * p : L;
 }
 
-// Same in FP64.
-HS_INLINE double hs_exp1_f64(uint64_t bits) {
-  if (bits < (1ull << 52)) return -log((double)(bits + 1) * 0x1p-53);
-  return -log1p(-(double)((1ull << 53) - 1 - bits) * 0x1p-53);
+// Natural log with the hardware log2 (keys only: their rounding matters for near-ties alone).
+HS_INLINE float hs_log_fast(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+
+// FP64 value with its low 6 mantissa bits cleared (toward zero, relative change < 2^-46), so a
+// waiting time and a 6-bit round tag fit one 64-bit word; every NaN becomes the canonical NaN.
+HS_INLINE double hs_dt_q(double v) {
+  if (v != v) return __longlong_as_double(0x7FF8000000000000ll);
+  return __longlong_as_double(__double_as_longlong(v) & ~63ll);
 }
 
-// Neel waiting time of a cell in FP64 from its compensated fields (value - compensation):
-// tau*exp(bmin*beta)/f0 == -ln(u)*exp(b*beta)/f0 (core.py:698-700), or with two channels
-// -ln(u)/(exp(-b1 beta)+exp(-b2 beta))/(f0/2) (core.py:686-693).
+// x * exp(y) * g * s: the exponent is split as y*log2(e) = n + f in FP64, the mantissa
+// x * 2^f * g is formed in FP32 (v_exp_f32, ~1 ulp) and scaled by 2^n and s in FP64. Relative
+// error ~3e-7; +inf on overflow (NaN for x = 0, like x*exp(y) in FP64), NaN stays NaN.
+HS_INLINE double hs_scaled_exp(float x, double y, float g, double s) {
+  if (!(y < 709.78)) return y != y ? y : (double)x * (double)INFINITY * s;
+  if (y < -745.2) return 0.0;
+  const double z = y * 1.4426950408889634074;
+  const double n = floor(z);
+  const float m = x * __builtin_amdgcn_exp2f((float)(z - n)) * g;
+  return ldexp((double)m, (int)n) * s;
+}
+
+// Neel waiting time of a cell in FP64 from its compensated fields (value - compensation) and its
+// x = -ln(u): tau*exp(bmin*beta)/f0 == x*exp(b*beta)/f0 (core.py:698-700), or with two channels
+// x/(exp(-b1 beta)+exp(-b2 beta))/(f0/2) (core.py:686-693). The barriers are formed in FP64
+// (b*beta ~ 10-100 needs ~1e-9 relative), the rest as in hs_scaled_exp: relative error ~3e-7.
+// Returned through hs_dt_q so every stepping path accumulates the same value.
 HS_INLINE double hs_neel_dt(double e, double ep, double eb, double beta, double m_perp, int parabolic, bool perp,
-                            uint64_t bits, double f0) {
-  const double dE = -2.0 * e, El = e, x = hs_exp1_f64(bits);
+                            float x, double f0) {
+  const double dE = -2.0 * e, El = e;
   if (perp) {
     const double epf = m_perp * ep;
     const double b1 = fmax(dE, (eb + epf) - El), b2 = fmax(dE, (eb - epf) - El);
     const double lo = fmin(b1, b2), hi = fmax(b1, b2);
-    return x * exp(lo * beta) / (1.0 + exp(-(hi - lo) * beta)) / (f0 / 2.0);
+    const float g = 1.f / (1.f + __builtin_amdgcn_exp2f((float)(-(hi - lo) * beta * 1.4426950408889634074)));
+    return hs_dt_q(hs_scaled_exp(x, lo * beta, g, 2.0 / f0));
   }
   double b;
   if (!parabolic) {
@@ -68,7 +109,7 @@ HS_INLINE double hs_neel_dt(double e, double ep, double eb, double beta, double 
     const double q = dE * (0.25 / eb) + 1.0;
     b = dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
   }
-  return x * exp(b * beta) / f0;
+  return hs_dt_q(hs_scaled_exp(x, b * beta, 1.f, 1.0 / f0));
 }
 
 // ---- Kahan-compensated FP32 accumulation (value = s - c) ------------------------------------
@@ -135,4 +176,39 @@ HS_INLINE T hs_wave_sum(T v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// ---- DPP wave reductions (wave64, all lanes active) ------------------------------------------
+// The minimum moves through quad, half-row, row and row-broadcast steps into lane 63 and is read
+// back wave-uniform: six VALU-DPP steps instead of six LDS-crossbar shuffles.
+template <int CTRL, int ROW_MASK = 0xf>
+HS_INLINE uint32_t hs_dpp_u32(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+HS_INLINE uint32_t hs_wave_min_u32(uint32_t v) {
+  v = min(v, hs_dpp_u32<0xB1>(~0u, v));        // quad_perm [1,0,3,2]
+  v = min(v, hs_dpp_u32<0x4E>(~0u, v));        // quad_perm [2,3,0,1]
+  v = min(v, hs_dpp_u32<0x141>(~0u, v));       // row_half_mirror
+  v = min(v, hs_dpp_u32<0x140>(~0u, v));       // row_mirror
+  v = min(v, hs_dpp_u32<0x142, 0xa>(~0u, v));  // row_bcast:15 into rows 1 and 3
+  v = min(v, hs_dpp_u32<0x143, 0xc>(~0u, v));  // row_bcast:31 into rows 2 and 3
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// 64-bit minimum as (min of the high words, then min of the low words among those lanes).
+HS_INLINE unsigned long long hs_wave_min_u64(unsigned long long v) {
+  const uint32_t hi = hs_wave_min_u32((uint32_t)(v >> 32));
+  const uint32_t lo = hs_wave_min_u32((uint32_t)(v >> 32) == hi ? (uint32_t)v : ~0u);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Wave minimum of v; `pay` becomes the payload of the lowest lane holding it.
+HS_INLINE unsigned long long hs_wave_min_pay(unsigned long long v, unsigned long long &pay) {
+  const unsigned long long m = hs_wave_min_u64(v);
+  const int src = __ffsll((long long)__ballot(v == m)) - 1;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pay, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pay >> 32), src);
+  pay = ((unsigned long long)hi << 32) | lo;
+  return m;
 }

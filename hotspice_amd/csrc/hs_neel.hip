@@ -40,7 +40,7 @@ HS_INLINE float log_rate_f32(const hs_instance_t &I, float e, float ep, float eb
     const float epf = (float)I.m_perp_factor * ep;
     const float b1 = fmaxf(dE, (eb + epf) - El), b2 = fmaxf(dE, (eb - epf) - El);  // core.py:651-653
     const float lo = fminf(b1, b2), hi = fmaxf(b1, b2);
-    return -lo * beta + log1pf(expf(-(hi - lo) * beta));
+    return -lo * beta + hs_log_fast(1.f + __builtin_amdgcn_exp2f(-(hi - lo) * beta * 1.44269504088896f));
   }
   float b;
   if (I.barrier == 0) {
@@ -246,9 +246,10 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
       }
     }
     if (compute && (I.flags[c] & HS_FLAG_NEEL)) {
-      const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)c);
-      const float x = hs_exp1_f32(bits);
-      float key = logf(x) - log_rate_f32<PERP>(I, e, ep, I.EB[c], I.beta[c]);
+      uint32_t wa, wb;
+      hs_words(I.seed, (uint64_t)step, 1u, (uint64_t)c, wa, wb);
+      const float x = hs_exp1_fast(wa, wb);
+      float key = hs_log_fast(x) - log_rate_f32<PERP>(I, e, ep, I.EB[c], I.beta[c]);
       if (isnan(key)) key = INFINITY;
       hs_argmin_pair(bk, bi, key, c);
     }
@@ -272,10 +273,11 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
     p.mw = 0;
     if (p.idx >= 0) {
       const int c = p.idx;
-      const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)c);
+      uint32_t wa, wb;
+      hs_words(I.seed, (uint64_t)step, 1u, (uint64_t)c, wa, wb);
       const double ep64 = PERP ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
       p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], (double)I.beta[c], I.m_perp_factor,
-                        I.barrier, PERP, bits, R.attempt_freq);
+                        I.barrier, PERP, hs_exp1_fast(wa, wb), R.attempt_freq);
       p.mw = I.m[c];
     }
     parts[(size_t)(j & 1) * P + blockIdx.x] = p;

@@ -1,15 +1,19 @@
 // Persistent multi-instance ("team") stepping: the Neel-Arrhenius step of many spin ices in
 // one cooperative launch.
 //
-// Every instance gets a team of G co-resident workgroups (256 threads). Each thread owns up to
-// MAXC grid cells (c = rank*256 + tid + k*G*256) and keeps their spin, combined fields (+Kahan
+// Every instance gets a team of G co-resident workgroups (1024 threads). Each thread owns up to
+// MAXC grid cells (c = rank*1024 + tid + k*G*1024) and keeps their spin, combined fields (+Kahan
 // compensations), E_B and beta in registers for the whole launch. Per step:
 //   1. keys log(tau) = log(-ln u) - log(rate) for the owned cells (Philox counter = cell, same
-//      streams as the single-instance path), best (key, cell) per thread, then per workgroup;
-//   2. G == 1: the workgroup's best is the winner. G > 1: thread 0 publishes the workgroup's
-//      winner time/spin in a slot and folds its (key, cell) into one 64-bit atomic minimum,
-//      then arrives at the team's counter barrier (bounded spin, error on timeout) and reads
-//      the team winner back; slots rotate over three steps so no reset races a reader;
+//      streams as the single-instance path; -ln u of the next step is drawn while the winner's
+//      kernel row is being loaded), packed as (key bits | cell | spin) so one 64-bit minimum per
+//      wave, then per workgroup, gives the lowest key with nanargmin's first-index tie break;
+//   2. the owner of the workgroup's best cell evaluates its FP64 waiting time. G == 1: that is
+//      the winner. G > 1: the owner writes the record {candidate, time} into the round's slot
+//      and arrives at the team's counter barrier (bounded spin, error on timeout); one wave
+//      then reads the G records and reduces them. Records rotate over three slots, and every
+//      workgroup reads round k's slot before it arrives at round k+1, so no write of a later
+//      round reaches a slot that is still being read;
 //   3. every thread applies the identical time/counter update (core.py:700-707) and the
 //      rank-1 field update of the winner (energies.py:310-329 + exchange stencil) to its
 //      registers, reading the winner's kernel row from L2.
@@ -24,8 +28,18 @@
 
 namespace {
 
-constexpr int TT = 1024;           // threads per workgroup (16 waves: latency hiding)
-constexpr uint64_t SPIN_LIMIT = 1ull << 26;
+constexpr int TT = 1024;  // threads per workgroup (16 waves: latency hiding)
+constexpr uint64_t SPIN_LIMIT = 1ull << 22;
+// a packed candidate is key bits (32) | round tag (6) | cell (24) | m+1 (2)
+constexpr unsigned long long TAG_MASK = 63ull << 26;
+constexpr unsigned long long NONE = ~0ull & ~TAG_MASK;  // "no candidate in this workgroup"
+
+// One workgroup's contribution to a round: its best packed candidate and that cell's waiting
+// time (low 6 mantissa bits cleared), each word carrying the round tag, so a reader that sees the
+// tag in both words has the whole record even if the two words landed separately.
+struct alignas(16) TeamRec {
+  unsigned long long c, d;
+};
 
 struct TeamDev {
   const hs_instance_t *inst;  // device copy
@@ -33,10 +47,7 @@ struct TeamDev {
   hs_state_t *state;
   int32_t *log_idx;
   double *log_dt;
-  uint32_t *bar;
-  unsigned long long *minslot;  // [3]
-  double *dtslot;               // [3][G] waiting time of each workgroup's best cell
-  uint32_t *dtflag;             // [3][G] round tag: the slot holds this round's value
+  TeamRec *rec;  // [3][G]: round k uses slot k % 3
   int32_t G, first;
 };
 
@@ -45,8 +56,8 @@ HS_INLINE uint32_t key_bits(float k) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-
-
+// FP32 log-rate of a cell: -b*beta (one channel) or log(exp(-b1 beta) + exp(-b2 beta)); the
+// same expression as the per-step launches use, so both paths rank cells identically.
 template <bool PERP>
 HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float eb, float beta) {
   const float dE = -2.f * e, El = e;
@@ -54,7 +65,7 @@ HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float e
     const float epf = (float)I.m_perp_factor * ep;
     const float b1 = fmaxf(dE, (eb + epf) - El), b2 = fmaxf(dE, (eb - epf) - El);
     const float lo = fminf(b1, b2), hi = fmaxf(b1, b2);
-    return -lo * beta + log1pf(expf(-(hi - lo) * beta));
+    return -lo * beta + hs_log_fast(1.f + __builtin_amdgcn_exp2f(-(hi - lo) * beta * 1.44269504088896f));
   }
   float b;
   if (I.barrier == 0) {
@@ -66,24 +77,14 @@ HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float e
   return -b * beta;
 }
 
-HS_INLINE unsigned long long wave_min_u64(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
-    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
-    v = o < v ? o : v;
-  }
-  return v;
-}
-
 template <int MAXC, bool PERP>
 __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const int32_t *block_team, int32_t *err_out) {
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
-  __shared__ unsigned long long s_part[TT / 64];
-  __shared__ unsigned long long s_block, s_win;
-  __shared__ double s_dtw;
-  __shared__ int s_bad;
+  __shared__ unsigned long long s_pc[TT / 64], s_pd[TT / 64];
+  __shared__ unsigned long long s_win;
+  __shared__ uint32_t s_ctl;
+  __shared__ float s_x[2][MAXC][TT];  // -ln(u) of the owned cells for the even / odd step
   const int tid = threadIdx.x;
   const int team = block_team[blockIdx.x];
   if (tid == 0) sT = teams[team];
@@ -97,25 +98,34 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   const hs_instance_t &I = sI;
   const hs_run_t R = sT.run;
   const int G = sT.G, rank = blockIdx.x - sT.first;
+  const int lane = tid & 63, wv = tid >> 6;
   const int nx = I.nx, ncell = I.nx * I.ny;
+  const int XB = nx > 1 ? 32 - __clz(nx - 1) : 0;  // positions y << XB | x order like y*nx + x
+  const uint32_t XM = (1u << XB) - 1u;
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
   const int hy = I.nn_h / 2, hx = I.nn_w / 2;
   const double nocc = (double)I.n_occ;
+  const float beta = I.beta[0];  // uniform over the grid (the host checks)
 
-  // register-resident state of the owned cells
-  int cell[MAXC], mm[MAXC];
-  float e[MAXC], ec[MAXC], ep[MAXC], epc[MAXC], eb[MAXC], bt[MAXC], mu[MAXC];
-  bool cand[MAXC];
+  // register-resident state of the owned cells c_k = base + k*stride (valid for k < nval)
+  const int base = rank * TT + tid, stride = G * TT;
+  int nval = 0;
+  uint32_t pos[MAXC];
+  float e[MAXC], ec[MAXC], ep[MAXC], epc[MAXC], eb[MAXC], mu[MAXC];
+  uint32_t occ = 0, neg = 0, cand = 0;  // bit k: occupied, m = -1, Neel candidate
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
-    const int c = rank * TT + tid + k * G * TT;
-    cell[k] = c < ncell ? c : -1;
-    mm[k] = 0;
-    e[k] = ec[k] = ep[k] = epc[k] = eb[k] = bt[k] = mu[k] = 0.f;
-    cand[k] = false;
+    const int c = base + k * stride;
+    pos[k] = 0;
+    e[k] = ec[k] = ep[k] = epc[k] = eb[k] = mu[k] = 0.f;
     if (c < ncell) {
-      mm[k] = I.m[c];
+      nval = k + 1;
+      pos[k] = ((uint32_t)(c / nx) << XB) | (uint32_t)(c % nx);
+      const int m = I.m[c];
+      occ |= (m != 0 ? 1u : 0u) << k;
+      neg |= (m < 0 ? 1u : 0u) << k;
+      cand |= ((I.flags[c] & HS_FLAG_NEEL) != 0 ? 1u : 0u) << k;
       e[k] = I.E[c];
       ec[k] = I.Ec[c];
       if (PERP) {
@@ -123,11 +133,12 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         epc[k] = I.Epc[c];
       }
       eb[k] = I.EB[c];
-      bt[k] = I.beta[c];
       mu[k] = I.mu[c];
-      cand[k] = (I.flags[c] & HS_FLAG_NEEL) != 0;
     }
   }
+  auto spin = [&](int k) { return ((occ >> k) & 1u) ? (((neg >> k) & 1u) ? -1 : 1) : 0; };
+
+  // time and counters: wave 0 advances them, rank 0 writes them back
   const hs_state_t S0 = sT.state[0];
   double t = S0.t, t0 = R.begin ? S0.t : S0.t0;
   int64_t att = S0.attempts, att0 = R.begin ? S0.attempts : S0.attempts0;
@@ -145,149 +156,191 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     while (a > att && !more(a - 1)) --a;
     att_stop = a;
   }
-  const int lane = tid & 63, wv = tid >> 6;
 
-  for (;;) {
-    if (!(sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max))) break;
-    // 1. keys of the owned cells, packed (key bits | cell | m+1) so one 64-bit minimum picks the
-    //    lowest key and, on ties, the lowest flat index (nanargmin's first index)
+  // -ln(u) draws. Philox call j serves cells 2j and 2j+1 (words 0,1 and 2,3), which sit in
+  // neighbouring lanes because base and stride are even. For an even step s the even lane
+  // evaluates call j at step s and the odd lane at step s+1, and each hands the other the half
+  // it needs (DPP swap of adjacent lanes): one Philox call per cell covers two steps. A single
+  // step (an odd first step) is drawn by both lanes evaluating the same call.
+  auto draw = [&](int64_t s, bool two) {
+    const uint32_t odd = (uint32_t)tid & 1u;
+#pragma unroll 1
+    for (int k = 0; k < MAXC; ++k) {
+      const uint32_t j = (uint32_t)(base + k * stride) >> 1;
+      const uint64_t sk = (uint64_t)s + (two ? odd : 0u);
+      const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
+                                (uint32_t)(I.seed >> 32));
+      const uint32_t ma = odd ? W.z : W.x, mb = odd ? W.w : W.y;  // this lane's cell in this call
+      if (!two) {
+        s_x[s & 1][k][tid] = hs_exp1_fast(ma, mb);
+      } else {
+        const uint32_t ra = hs_dpp_u32<0xB1>(0u, odd ? W.x : W.z);  // quad_perm [1,0,3,2]: lane ^ 1
+        const uint32_t rb = hs_dpp_u32<0xB1>(0u, odd ? W.y : W.w);
+        s_x[0][k][tid] = hs_exp1_fast(odd ? ra : ma, odd ? rb : mb);
+        s_x[1][k][tid] = hs_exp1_fast(odd ? ma : ra, odd ? mb : rb);
+      }
+    }
+  };
+
+  bool go = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
+  if (go) draw(step, (step & 1) == 0);
+  while (go) {
+    const int par = (int)(step & 1);
+    // 1. keys of the owned candidates, packed (key bits | position | m+1) so one 64-bit minimum
+    //    picks the lowest key and, on ties, the lowest flat index (nanargmin's first index)
     unsigned long long best = ~0ull;
+    int bk = -1;
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
-      if (cell[k] >= 0 && cand[k]) {
-        const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)cell[k]);
-        float key = logf(hs_exp1_f32(bits)) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], bt[k]);
+      if ((cand >> k) & 1u) {
+        float key = hs_log_fast(s_x[par][k][tid]) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], beta);
         if (isnan(key)) key = INFINITY;
-        const unsigned long long p = ((unsigned long long)key_bits(key) << 32) | ((uint32_t)cell[k] << 2) | (uint32_t)(mm[k] + 1);
-        best = p < best ? p : best;
+        const unsigned long long p =
+            ((unsigned long long)key_bits(key) << 32) | (pos[k] << 2) | (uint32_t)(spin(k) + 1);
+        if (p < best) {
+          best = p;
+          bk = k;
+        }
       }
     }
-    best = wave_min_u64(best);
-    if (lane == 0) s_part[wv] = best;
-    __syncthreads();
-    if (wv == 0) {
-      unsigned long long v = lane < TT / 64 ? s_part[lane] : ~0ull;
-      v = wave_min_u64(v);
-      if (lane == 0) s_block = v;
-    }
-    __syncthreads();
-    const unsigned long long bb = s_block;
-    const int bcell = bb == ~0ull ? -1 : (int)((bb & 0xffffffffull) >> 2);
-    const int sl = (int)(sd % 3);
-    // 2a. the owner of the workgroup's best cell evaluates its FP64 waiting time from the
-    //     compensated fields and publishes it (program order before the exchange: a wave never
-    //     waits on its own lanes)
+    // 2. the lane holding its wave's best evaluates that cell's FP64 waiting time from the
+    //    compensated fields (core.py:686-700), so the time travels with the candidate
+    const unsigned long long wbest = hs_wave_min_u64(best);
+    if (bk >= 0 && best == wbest) {
+      float se = 0.f, sec = 0.f, sep = 0.f, sepc = 0.f, seb = 0.f;
 #pragma unroll
-    for (int k = 0; k < MAXC; ++k) {
-      if (cell[k] == bcell) {
-        const uint64_t bits = hs_bits53(I.seed, (uint64_t)step, 1u, (uint64_t)cell[k]);
-        const double dtk = hs_neel_dt((double)e[k] - (double)ec[k], PERP ? (double)ep[k] - (double)epc[k] : 0.0,
-                                      (double)eb[k], (double)bt[k], I.m_perp_factor, I.barrier, PERP, bits,
-                                      R.attempt_freq);
-        if (G > 1) {
-          __hip_atomic_store(&sT.dtslot[sl * G + rank], dtk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __builtin_amdgcn_s_waitcnt(0);
-          __hip_atomic_store(&sT.dtflag[sl * G + rank], rounds + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-          s_dtw = dtk;
+      for (int k = 0; k < MAXC; ++k) {
+        if (k == bk) {
+          se = e[k];
+          sec = ec[k];
+          sep = ep[k];
+          sepc = epc[k];
+          seb = eb[k];
         }
       }
+      const double dtl = hs_neel_dt((double)se - (double)sec, PERP ? (double)sep - (double)sepc : 0.0, (double)seb,
+                                    (double)beta, I.m_perp_factor, I.barrier, PERP, s_x[par][bk][tid], R.attempt_freq);
+      s_pc[wv] = wbest;
+      s_pd[wv] = (unsigned long long)__double_as_longlong(dtl);
+    } else if (lane == 0 && wbest == ~0ull) {
+      s_pc[wv] = ~0ull;
+      s_pd[wv] = 0ull;
     }
-    // 2. team winner: one 64-bit atomic minimum + counter barrier (bounded spins)
-    if (tid == 0) {
-      unsigned long long win = bb;
-      int bad = 0;
+    __syncthreads();
+    // 3. wave 0: the workgroup's best; with G > 1 it writes the workgroup's record for this round
+    //    and waits (bounded) until all G records of the round carry its tag, then reduces them.
+    //    Wave 0 then advances time and counters (core.py:700-707) and tells the others whether
+    //    the winner flips and whether another step follows.
+    if (wv == 0) {
+      unsigned long long pd = lane < TT / 64 ? s_pd[lane] : 0ull;
+      unsigned long long pc = hs_wave_min_pay(lane < TT / 64 ? s_pc[lane] : ~0ull, pd);
+      if (pc == ~0ull) pc = NONE;
+      bool bad = false;
       if (G > 1) {
-        __hip_atomic_fetch_min(&sT.minslot[sl], bb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (rank == 0) __hip_atomic_store(&sT.minslot[(sl + 1) % 3], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // all team data travels through device-scope atomics performed at the coherence point, so
-        // the arrival only waits for them to complete: no agent-scope release/acquire, which would
-        // write back and invalidate the L2 holding the kernels every step
-        __builtin_amdgcn_s_waitcnt(0);
-        __hip_atomic_fetch_add(sT.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t target = (rounds + 1) * (uint32_t)G;
-        uint64_t spins = 0;
-        while (__hip_atomic_load(sT.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-          if (++spins > SPIN_LIMIT) {
-            bad = 1;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
+        const unsigned long long tg = rounds & 63u;
+        TeamRec *slot = sT.rec + (size_t)(rounds % 3u) * G;
+        if (lane == 0) {
+          __hip_atomic_store(&slot[rank].c, pc | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&slot[rank].d, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        win = __hip_atomic_load(&sT.minslot[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!bad && win != ~0ull) {  // the winner's time, once its owner's slot shows this round
-          const int owner = ((int)((win & 0xffffffffull) >> 2) / TT) % G;
-          while (__hip_atomic_load(&sT.dtflag[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != rounds + 1u) {
+        unsigned long long c2 = ~0ull, d2 = 0ull;
+        for (int r = lane; r < G; r += 64) {
+          unsigned long long cr, dr;
+          uint64_t spins = 0;
+          for (;;) {
+            cr = __hip_atomic_load(&slot[r].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            dr = __hip_atomic_load(&slot[r].d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (((cr & TAG_MASK) >> 26) == tg && (dr & 63ull) == tg) break;
             if (++spins > SPIN_LIMIT) {
-              bad = 1;
+              bad = true;
               break;
             }
             __builtin_amdgcn_s_sleep(1);
           }
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-          s_dtw = __hip_atomic_load(&sT.dtslot[sl * G + owner], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cr < c2) {
+            c2 = cr;
+            d2 = dr;
+          }
         }
+        pc = hs_wave_min_pay(c2, d2) & ~TAG_MASK;
+        pd = d2 & ~63ull;
+        bad = __ballot(bad) != 0ull;
       }
-      s_win = win;
-      s_bad = bad;
+      ++rounds;
+      uint32_t ctl = 4u;  // stop without a step
+      if (bad) {
+        err = 3;  // a record of the round never arrived
+      } else if (pc == NONE) {
+        err = 1;  // nanargmin over an all-NaN grid
+      } else {
+        const double dt = __longlong_as_double((long long)pd);
+        const double tmax_step = R.t_max - (t - t0);
+        att += 1;
+        sd += 1;
+        const bool flip = !(dt > tmax_step || isnan(dt));
+        t += flip ? dt : tmax_step;
+        sw += flip ? 1 : 0;
+        if (rank == 0 && lane == 0 && R.log) {
+          const uint32_t wp = (uint32_t)(pc >> 2) & 0xFFFFFFu;
+          sT.log_idx[sd - 1] = flip ? (int)(wp >> XB) * nx + (int)(wp & XM) : -1;
+          sT.log_dt[sd - 1] = flip ? dt : tmax_step;
+        }
+        const bool more = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
+        ctl = (flip ? 1u : 0u) | (more ? 2u : 0u);
+      }
+      if (lane == 0) {
+        s_win = pc;
+        s_ctl = ctl;
+      }
     }
     __syncthreads();
-    const unsigned long long win = s_win;
-    ++rounds;
-    if (s_bad) {
-      err = 3;  // team barrier timed out
-      break;
-    }
-    if (win == ~0ull) {
-      err = 1;  // nanargmin over an all-NaN grid
-      break;
-    }
-    const int w = (int)((win & 0xffffffffull) >> 2);
-    const int mw = (int)(win & 3ull) - 1;
-    const double dt = s_dtw;
-    // 3. time, counters (core.py:700-707) and the field update of the winner
-    const double tmax_step = R.t_max - (t - t0);
-    att += 1;
+    const uint32_t ctl = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ctl);
+    if (ctl & 4u) break;
     step += 1;
-    sd += 1;
-    const bool flip = !(dt > tmax_step || isnan(dt));
-    t += flip ? dt : tmax_step;
-    if (rank == 0 && tid == 0 && R.log) {
-      sT.log_idx[sd - 1] = flip ? w : -1;
-      sT.log_dt[sd - 1] = flip ? dt : tmax_step;
+    go = (ctl & 2u) != 0;
+    const bool two = (step & 1) == 0;  // an even step draws itself and the next
+    if (!(ctl & 1u)) {
+      if (two) draw(step, true);
+      continue;
     }
-    if (flip) {
-      sw += 1;
-      const int yw = w / nx, xw = w % nx, sitew = hs_site(I, yw, xw);
-      const int nw = I.dip_pf != 0.0 ? I.sub_of[sitew] : -1;
-      const double mnew = -(double)mw;
-      const float A = (float)(2.0 * I.dip_pf * (mnew * (double)I.mu[w]));
-      const float Ax = (float)(-2.0 * I.J * mnew);
-      const float oxw = (float)I.ox_uc[sitew], oyw = (float)I.oy_uc[sitew];
-      const float *Kw = nw >= 0 ? I.K32 + (size_t)nw * KS : nullptr;
-      const float *Kpw = (PERP && nw >= 0) ? I.Kpo32 + (size_t)nw * KS : nullptr;
-      float kv[MAXC], kpv[MAXC];  // the winner's kernel entries for every owned cell, loaded together
+    // 4. rank-1 field update of the winner (energies.py:310-329 + exchange stencil)
+    const unsigned long long win = s_win;
+    const uint32_t wpos = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)(win >> 2) & 0xFFFFFFu));
+    const int mw = __builtin_amdgcn_readfirstlane((int)(win & 3ull)) - 1;
+    const int yw = (int)(wpos >> XB), xw = (int)(wpos & XM);
+    const int w = yw * nx + xw, sitew = hs_site(I, yw, xw);
+    const int nw = I.dip_pf != 0.0 ? I.sub_of[sitew] : -1;
+    const double mnew = -(double)mw;
+    const float A = (float)(2.0 * I.dip_pf * (mnew * (double)I.mu[w]));
+    const float Ax = (float)(-2.0 * I.J * mnew);
+    const float oxw = (float)I.ox_uc[sitew], oyw = (float)I.oy_uc[sitew];
+    const float *Kw = nw >= 0 ? I.K32 + (size_t)nw * KS : nullptr;
+    const float *Kpw = (PERP && nw >= 0) ? I.Kpo32 + (size_t)nw * KS : nullptr;
+    constexpr int CH = MAXC < 4 ? MAXC : 4;  // cells whose kernel entries are loaded together
 #pragma unroll
-      for (int k = 0; k < MAXC; ++k) {
-        kv[k] = 0.f;
-        kpv[k] = 0.f;
-        const int c = cell[k];
-        if (c >= 0 && Kw) {
-          const int dy = c / nx - yw, dx = c % nx - xw;
+    for (int k0 = 0; k0 < MAXC; k0 += CH) {
+      float kv[CH], kpv[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int k = k0 + j;
+        kv[j] = 0.f;
+        kpv[j] = 0.f;
+        if (k < nval && Kw) {
+          const int dy = (int)(pos[k] >> XB) - yw, dx = (int)(pos[k] & XM) - xw;
           if (dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx) {
             const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
-            kv[k] = Kw[o];
-            if (PERP) kpv[k] = Kpw[o];
+            kv[j] = Kw[o];
+            if (PERP) kpv[j] = Kpw[o];
           }
         }
       }
+      if (k0 == 0 && two) draw(step, true);  // the next draws overlap the loads above
 #pragma unroll
-      for (int k = 0; k < MAXC; ++k) {
-        const int c = cell[k];
-        if (c < 0) continue;
-        if (c == w) {
-          mm[k] = -mm[k];
+      for (int j = 0; j < CH; ++j) {
+        const int k = k0 + j;
+        if (k >= nval) continue;
+        if (pos[k] == wpos) {
+          neg ^= 1u << k;
           e[k] = -e[k];
           ec[k] = -ec[k];
           if (PERP) {
@@ -296,13 +349,13 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
           continue;
         }
-        if (mm[k] == 0) continue;
-        const int y = c / nx, x = c % nx;
+        const int mk = spin(k);
+        if (mk == 0) continue;
+        const int y = (int)(pos[k] >> XB), x = (int)(pos[k] & XM);
         const int dy = y - yw, dx = x - xw;
-        float de = 0.f, dep = 0.f;
-        const float mmu = (float)mm[k] * mu[k];
-        de = A * (mmu * kv[k]);
-        if (PERP) dep = A * (mmu * kpv[k]);
+        const float mmu = (float)mk * mu[k];
+        float de = A * (mmu * kv[j]);
+        float dep = PERP ? A * (mmu * kpv[j]) : 0.f;
         if (Ax != 0.f) {
           int qy = dy, qx = dx;
           if (I.pbc) {
@@ -319,10 +372,10 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
               if (I.in_plane) {
                 const int s = hs_site(I, y, x);
                 const float oxi = (float)I.ox_uc[s], oyi = (float)I.oy_uc[s];
-                de += Ax * (float)mm[k] * wq * (oxi * oxw + oyi * oyw);
-                if (PERP) dep += Ax * (float)mm[k] * wq * (oyw * oxi - oxw * oyi);
+                de += Ax * (float)mk * wq * (oxi * oxw + oyi * oyw);
+                if (PERP) dep += Ax * (float)mk * wq * (oyw * oxi - oxw * oyi);
               } else {
-                de += Ax * (float)mm[k] * wq;
+                de += Ax * (float)mk * wq;
               }
             }
           }
@@ -335,9 +388,9 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   // write back
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
-    const int c = cell[k];
-    if (c < 0) continue;
-    I.m[c] = (int8_t)mm[k];
+    if (k >= nval) continue;
+    const int c = base + k * stride;
+    I.m[c] = (int8_t)spin(k);
     I.E[c] = e[k];
     I.Ec[c] = ec[k];
     if (PERP) {
@@ -361,14 +414,15 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   if (err && tid == 0) atomicMax(err_out, err);
 }
 
-__global__ void k_team_reset(TeamDev *teams, int nteams, int32_t *err) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) *err = 0;
-  if (t >= nteams) return;
-  TeamDev d = teams[t];
-  *d.bar = 0u;
-  d.minslot[0] = d.minslot[1] = d.minslot[2] = ~0ull;
-  for (int i = 0; i < 3 * d.G; ++i) d.dtflag[i] = 0u;
+// Clears the error word and marks every record slot stale (tag 63 in both words: rounds 0-2 use
+// tags 0-2 and every slot is rewritten before tag 63 comes round).
+__global__ void k_team_reset(const TeamDev *teams, int32_t *err) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0;
+  const TeamDev d = teams[blockIdx.x];
+  for (int i = threadIdx.x; i < 3 * d.G; i += blockDim.x) {
+    d.rec[i].c = ~0ull;
+    d.rec[i].d = ~0ull;
+  }
 }
 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -410,7 +464,7 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
   L.team_sync.resize(n);
   for (int i = 0; i < n; ++i) {
     L.team_sync[i] = off;
-    off += 256 + al256(3 * 8) + al256(3 * 8 * (size_t)L.G[i]) + al256(3 * 4 * (size_t)L.G[i]);
+    off += al256(sizeof(TeamRec) * 3 * (size_t)L.G[i]);
   }
   L.err_off = off;
   L.total = off + 256;
@@ -456,9 +510,13 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   if (ws_bytes < (int64_t)L.total) return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)L.total);
   if (L.maxc > 16) return hs_set_error(HS_ERR_ARG, "more than 16 cells per thread: give the team more workgroups");
   for (int i = 0; i < nteams; ++i) {
-    // the barrier counter counts rounds*G in 32 bits
-    if ((double)teams[i].run.max_steps * (double)L.G[i] >= 4.0e9)
-      return hs_set_error(HS_ERR_ARG, "team %d: max_steps*nblocks exceeds the 32-bit barrier counter", i);
+    // packed candidates hold the position y << ceil(log2 nx) | x in 24 bits
+    const hs_instance_t &in = *teams[i].inst;
+    int xb = 0;
+    while ((1 << xb) < in.nx) ++xb;
+    if (xb > 23 || ((int64_t)in.ny << xb) > (1 << 24))
+      return hs_set_error(HS_ERR_ARG, "team %d: grid %dx%d too large for the persistent path (ny * 2^ceil(log2 nx) > 2^24)",
+                          i, in.nx, in.ny);
   }
   hipStream_t s = (hipStream_t)stream;
   int dev = 0;
@@ -491,11 +549,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     d.state = teams[i].state;
     d.log_idx = teams[i].log_idx;
     d.log_dt = teams[i].log_dt;
-    char *sy = base + L.team_sync[i];
-    d.bar = (uint32_t *)sy;
-    d.minslot = (unsigned long long *)(sy + 256);
-    d.dtslot = (double *)(sy + 256 + al256(24));
-    d.dtflag = (uint32_t *)(sy + 256 + al256(24) + al256(24 * (size_t)L.G[i]));
+    d.rec = (TeamRec *)(base + L.team_sync[i]);
     d.G = L.G[i];
     d.first = L.first[i];
     td[i] = d;
@@ -504,7 +558,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
   int32_t *err = (int32_t *)(base + L.err_off);
-  k_team_reset<<<(nteams + 255) / 256, 256, 0, s>>>((TeamDev *)base, nteams, err);
+  k_team_reset<<<nteams, 256, 0, s>>>((const TeamDev *)base, err);
   HS_LAUNCH_CHECK("k_team_reset");
   hipError_t e;
   const TeamDev *dtd = (const TeamDev *)(base + L.teams_off);
