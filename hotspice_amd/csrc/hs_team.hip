@@ -81,7 +81,8 @@ template <int MAXC, bool PERP>
 __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const int32_t *block_team, int32_t *err_out) {
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
-  __shared__ unsigned long long s_pc[TT / 64], s_pd[TT / 64];
+  __shared__ unsigned long long s_pc[TT / 64];
+  __shared__ float s_in[TT / 64][8];  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each wave's best
   __shared__ unsigned long long s_win;
   __shared__ uint32_t s_ctl;
   __shared__ float s_x[2][MAXC][TT];  // -ln(u) of the owned cells for the even / odd step
@@ -203,8 +204,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         }
       }
     }
-    // 2. the lane holding its wave's best evaluates that cell's FP64 waiting time from the
-    //    compensated fields (core.py:686-700), so the time travels with the candidate
+    // 2. the lane holding its wave's best leaves that cell's fields and draw for wave 0
     const unsigned long long wbest = hs_wave_min_u64(best);
     if (bk >= 0 && best == wbest) {
       float se = 0.f, sec = 0.f, sep = 0.f, sepc = 0.f, seb = 0.f;
@@ -218,13 +218,15 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           seb = eb[k];
         }
       }
-      const double dtl = hs_neel_dt((double)se - (double)sec, PERP ? (double)sep - (double)sepc : 0.0, (double)seb,
-                                    (double)beta, I.m_perp_factor, I.barrier, PERP, s_x[par][bk][tid], R.attempt_freq);
       s_pc[wv] = wbest;
-      s_pd[wv] = (unsigned long long)__double_as_longlong(dtl);
+      s_in[wv][0] = se;
+      s_in[wv][1] = sec;
+      s_in[wv][2] = sep;
+      s_in[wv][3] = sepc;
+      s_in[wv][4] = seb;
+      s_in[wv][5] = s_x[par][bk][tid];
     } else if (lane == 0 && wbest == ~0ull) {
       s_pc[wv] = ~0ull;
-      s_pd[wv] = 0ull;
     }
     __syncthreads();
     // 3. wave 0: the workgroup's best; with G > 1 it writes the workgroup's record for this round
@@ -232,9 +234,17 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     //    Wave 0 then advances time and counters (core.py:700-707) and tells the others whether
     //    the winner flips and whether another step follows.
     if (wv == 0) {
-      unsigned long long pd = lane < TT / 64 ? s_pd[lane] : 0ull;
-      unsigned long long pc = hs_wave_min_pay(lane < TT / 64 ? s_pc[lane] : ~0ull, pd);
-      if (pc == ~0ull) pc = NONE;
+      unsigned long long src = (unsigned long long)lane, pd = 0ull;
+      unsigned long long pc = hs_wave_min_pay(lane < TT / 64 ? s_pc[lane] : ~0ull, src);
+      if (pc == ~0ull) {
+        pc = NONE;
+      } else {
+        // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
+        const float *in = s_in[src];
+        const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
+                                     (double)in[4], (double)beta, I.m_perp_factor, I.barrier, PERP, in[5], R.attempt_freq);
+        pd = (unsigned long long)__double_as_longlong(dt);
+      }
       bool bad = false;
       if (G > 1) {
         const unsigned long long tg = rounds & 63u;
@@ -328,7 +338,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         if (k < nval && Kw) {
           const int dy = (int)(pos[k] >> XB) - yw, dx = (int)(pos[k] & XM) - xw;
           if (dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx) {
-            const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
+            const int o = (I.kwy + dy) * KW + (I.kwx + dx);
             kv[j] = Kw[o];
             if (PERP) kpv[j] = Kpw[o];
           }
