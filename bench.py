@@ -5,9 +5,10 @@ magnets), full 511x511 dipolar kernels + 1 mT Zeeman field, Neel-Arrhenius, FP32
 fields with an FP64 recompute every 1024 switches. One benchmark step = 1024 Neel switch
 attempts + the FP64 recompute, i.e. one recompute period; `value` = switch attempts/s.
 
-Prints ONE JSON line (see the keys below). `roofline` is for the dominant kernel (the Neel
-step): algorithmic bytes per launch (neel_bytes_per_attempt) / average launch time measured
-with HIP events on the library's stream. `cpu_baseline` times the FP64 NumPy restatement of
+Prints ONE JSON line (see the keys below). `roofline` is for the dominant kernel (the persistent
+Neel step kernel, one launch per 1024 attempts): algorithmic bytes per launch (attempts per launch
+x neel_bytes_per_attempt) / average launch duration, measured with HIP events recorded on the
+launch stream around every launch in the timed region. `cpu_baseline` times the FP64 NumPy restatement of
 the reference on one host core, on a bounded sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
@@ -89,22 +90,11 @@ def main():
     E_ref = mm.E
     rel_err = float(np.max(np.abs(E_inc - E_ref)) / np.max(np.abs(E_ref)))
 
-    # live average duration of the Neel step kernel: HIP events around launches of that kernel only
-    stream = torch.cuda.current_stream()
-    mm.recompute_every = None
-    nprobe = 4096
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    mm.progress(t_max=math.inf, MCsteps_max=nprobe / n)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    neel_us = ev0.elapsed_time(ev1) * 1e3 / (nprobe + 1)  # nprobe steps launch nprobe+1 step kernels
-    mm.recompute_every = PERIOD
-    mm._device_recompute()
-
-    # timed region: K benchmark steps of PERIOD attempts + FP64 recompute each
-    a0 = mm.attempted_switches
+    # timed region: K benchmark steps of PERIOD attempts + FP64 recompute each; HIP events on the
+    # launch stream bracket every launch of the persistent Neel kernel (k_neel_team) inside it
+    dev = mm._dev
+    dev.launch_events = []
+    a0, s0 = mm.attempted_switches, mm.switches
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -112,10 +102,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     attempts = mm.attempted_switches - a0
+    switches = mm.switches - s0
     value = attempts / wall
+    launches = dev.launch_events
+    dev.launch_events = None
+    launch_us = [e0.elapsed_time(e1) * 1e3 for e0, e1, _ in launches]
+    neel_us = sum(launch_us) / len(launch_us)
+    steps_per_launch = attempts / len(launches)
 
     bytes_per = neel_bytes_per_attempt(n, n, perp=True)
-    achieved = bytes_per / (neel_us * 1e-6) / 1e9
+    bytes_launch = bytes_per * steps_per_launch
+    achieved = bytes_launch / (neel_us * 1e-6) / 1e9
     line = {
         "metric": "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat",
         "value": value,
@@ -132,10 +129,12 @@ def main():
         "config": {"workload": "plain Neel: IP_Pinwheel 256x256 (32768 magnets), full 511x511 dipolar kernel "
                                "+ 1 mT Zeeman at 22.5 deg, T=300 K, E_B lognormal 60 kT/5%, FP64 recompute every "
                                "1024 switches", "attempts_per_step": PERIOD, "attempts": attempts,
-                   "switches": mm.switches, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"},
+                   "switches": switches, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_neel_team", "avg_launch_us": neel_us, "bytes_per_launch": bytes_per},
+                     "kernel": "k_neel_team", "launches": len(launches), "avg_launch_us": neel_us,
+                     "attempts_per_launch": steps_per_launch, "bytes_per_attempt": bytes_per,
+                     "bytes_per_launch": bytes_launch},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -57,6 +57,9 @@ class DeviceState:
         self.inst = _lib.Instance()
 
     # ---- plumbing ----------------------------------------------------------------------
+    # set to a list to collect (start event, end event, state pointers) around every team launch
+    launch_events = None
+
     def stream(self):
         return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -175,7 +178,15 @@ class DeviceState:
             raise _lib.HotspiceError("bad team descriptors")
         if self.team_ws is None or self.team_ws.numel() < need:
             self.team_ws = self.torch.zeros(max(need, 256), dtype=self.torch.uint8, device=self.device)
+        ev = self.launch_events
+        if ev is not None:  # timing hook: HIP events around the launch, on the launch stream
+            s = self.torch.cuda.current_stream(self.device)
+            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
+            e0.record(s)
         _lib.check(run_fn(arr, len(teams), self.ptr(self.team_ws), self.team_ws.numel(), self.stream()))
+        if ev is not None:
+            e1.record(s)
+            ev.append((e0, e1, [t.state for t in teams]))
 
     def metropolis_run(self, sel: _lib.Select, run: _lib.Run, log_cells=None, log_count=None):
         need = int(self.lib.hs_metropolis_ws_bytes(C.byref(self.inst), C.byref(sel)))
