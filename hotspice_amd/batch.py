@@ -20,10 +20,11 @@ def _check_same_scheme(mags):
     return schemes.pop()
 
 
-def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None, **kwargs):
-    """Steps every instance like mm.progress(t_max, MCsteps_max, **kwargs), all in one launch per
-    chunk. `nblocks`: workgroups per instance (int, list, or None for automatic). Returns the
-    list of (dt, dMCsteps)."""
+def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None, per_instance=None, **kwargs):
+    """Steps every instance like mm.progress(t_max, MCsteps_max, **kwargs), all instances
+    together per chunk. `nblocks`: Neel workgroups per instance (int, list, or None for
+    automatic). `per_instance`: optional list of keyword dicts (e.g. Metropolis r / Q) that
+    override **kwargs for one instance each. Returns the list of (dt, dMCsteps)."""
     mags = list(magnets)
     if not mags:
         return []
@@ -41,10 +42,13 @@ def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None,
         st = m._state()
         starts.append((st.t, st.attempts))
     attempt_freq = float(kwargs.get('attempt_freq', 1e10))
+    if per_instance is not None and len(per_instance) != len(mags):
+        raise ValueError("per_instance needs one keyword dict per instance")
+    kws = [dict(kwargs, **(per_instance[i] if per_instance is not None else {})) for i in range(len(mags))]
     if scheme == Scheme.NEEL:
         return _progress_many_neel(mags, starts, t_max, MCsteps_max, attempt_freq, nbl, chunk)
     if scheme == Scheme.METROPOLIS:
-        return _progress_many_metropolis(mags, starts, t_max, MCsteps_max, attempt_freq, nbl, chunk, kwargs)
+        return _progress_many_metropolis(mags, starts, t_max, MCsteps_max, attempt_freq, chunk, kws)
     raise NotImplementedError("batched Wolff updates are not supported")
 
 
@@ -100,17 +104,19 @@ def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk):
     return out
 
 
-def _progress_many_metropolis(mags, starts, t_max, mc, f0, nbl, chunk, kwargs):
+def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws):
     dev = mags[0]._dev
     sels = []
-    for m in mags:
-        r = kwargs.get('r', 0)
-        Q = kwargs.get('Q', 0.05)
+    for m, kw in zip(mags, kws):
+        if kw.get('idx') is not None:
+            raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
+        r = kw.get('r', 0)
+        Q = kw.get('Q', 0.05)
         sels.append(m._select_geometry(m.calc_r(Q) if r == 0 else r))
     first = True
     while True:
         teams = []
-        for m, s0, nb, sel in zip(mags, starts, nbl, sels):
+        for m, s0, sel in zip(mags, starts, sels):
             pend, st = _pending(m, s0, t_max, mc)
             if not pend:
                 continue
@@ -130,7 +136,7 @@ def _progress_many_metropolis(mags, starts, t_max, mc, f0, nbl, chunk, kwargs):
                            begin=0)
             teams.append((m, _lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=_lib.C.addressof(sel), run=run,
                                        state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None,
-                                       nblocks=int(nb))))
+                                       nblocks=0)))
         first = False
         if not teams:
             break

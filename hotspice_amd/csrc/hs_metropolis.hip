@@ -1,21 +1,27 @@
 // Metropolis stepping with grid multisampling (Magnets._update_Metropolis, core.py:659-677).
 //
-// Two launches per step:
-//   k_metro_select (one workgroup): the stop predicate of progress(), the supergrid selection
-//     of _select_grid (core.py:496-558) with its retries (core.py:484-487) and the
+// Two launches per step, for one instance or for a batch of independent instances ("jobs"):
+//   k_metro_select (one workgroup per job): the stop predicate of progress(), the supergrid
+//     selection of _select_grid (core.py:496-558) with its retries (core.py:484-487) and the
 //     _select_single fallback (core.py:489-494, 555-558), acceptance min(1, e^{-beta dE})
 //     clipped to [1e-10, 1e10] (core.py:666, 671), optional METROPOLIS_TIME (core.py:667-669),
 //     the flips with E[w] *= -1 (energies.py:124-126, 332-334), and the order-preserving list of
 //     accepted switches with per-sublattice counts (energies.py:335-343);
-//   k_metro_update (one workgroup per 32x8 output tile): the sparse-source correlation of the
-//     accepted switches with the dipolar kernel -- full window, or the (2k+1)^2 REDUCED_KERNEL_SIZE
-//     window with fill/wrap boundary when a sublattice has more than
+//   k_metro_update (one workgroup per 32x8 output tile of a job): the sparse-source correlation
+//     of the accepted switches with the dipolar kernel -- full window, or the (2k+1)^2
+//     REDUCED_KERNEL_SIZE window with fill/wrap boundary when a sublattice has more than
 //     SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF switches (energies.py:343-361) -- plus the
 //     exchange stencil, added once per cell with Kahan compensation (energies.py:362-366).
-//     Sources are staged in LDS and filtered per tile; tiles no source reaches return at once.
+//     A tile finds its sources directly from the supercell grid (the samples of the supercells
+//     whose cells lie within the window of the tile), or, where that is not shorter (full
+//     kernels, PBC, single-sample steps), by filtering the whole list through LDS. Both visit
+//     the sources in selection order, so every cell sums the same terms in the same order.
+// A batch runs every job's step in the same two launches; each job keeps its own stop
+// predicate, so a job alone and a job inside a batch execute identical arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <vector>
 
 #include "hs_device.h"
 #include "hs_internal.h"
@@ -33,6 +39,18 @@ struct MetroWS {
   float *acc_mmu;    // [nsc] (m mu) after the flip
   int32_t *acc_n;    // [1]  number accepted this step
   int32_t *trunc;    // [HS_MAX_UC] per unit-cell site: 1 = truncated window (convolution branch)
+  int32_t *info;     // [4] grid offset ox, oy of the selection that succeeded, 1 = single sample
+};
+
+// Everything one job's step needs (passed by value for one instance, from an array for a batch).
+struct MetroJob {
+  hs_instance_t I;
+  hs_select_t SEL;
+  hs_run_t R;
+  hs_state_t *st;
+  MetroWS ws;
+  int32_t *log_cells, *log_count;
+  int32_t tiles_x, tiles_y;
 };
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -53,12 +71,14 @@ MetroWS carve(void *base, int nsc) {
   p += 256;
   w.trunc = (int32_t *)p;
   p += align256(HS_MAX_UC * 4);
+  w.info = (int32_t *)p;
+  p += 256;
   return w;
 }
 
 size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
-  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4);
+  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256;
 }
 
 // Block-wide sum of an int (1024 threads).
@@ -116,9 +136,13 @@ HS_INLINE double barrier_min_f64(const hs_instance_t &I, int c) {
   return dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
 }
 
-__global__ void __launch_bounds__(MS_THREADS) k_metro_select(hs_instance_t I, hs_select_t SEL, hs_run_t R,
-                                                             hs_state_t *st, MetroWS ws, int32_t *log_cells,
-                                                             int32_t *log_count) {
+__device__ __forceinline__ void metro_select(const MetroJob &J) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const hs_run_t &R = J.R;
+  hs_state_t *st = J.st;
+  const MetroWS &ws = J.ws;
+  int32_t *log_cells = J.log_cells, *log_count = J.log_count;
   __shared__ int s_i[MS_THREADS / 64];
   __shared__ double s_d[MS_THREADS / 64];
   __shared__ int s_site[HS_MAX_UC];
@@ -148,12 +172,12 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select(hs_instance_t I, hs
   int nvalid = 0;
   bool single = false;
   int attempt = 0;
+  int ox = 0, oy = 0;
   for (; attempt < MAX_ATTEMPTS; ++attempt) {
     const uint64_t b0 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0));
     const uint64_t b1 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1));
     const uint64_t b2 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 2));
     if (SEL.scheme == 0) {
-      int ox, oy;
       if (I.pbc) {  // np.random.randint(nx), randint(ny) (core.py:527)
         ox = (int)floor(hs_u53(b0, 0) * nx);
         oy = (int)floor(hs_u53(b1, 0) * ny);
@@ -275,6 +299,9 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select(hs_instance_t I, hs
     ws.trunc[tid] = (total > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
   }
   if (tid == 0) {
+    ws.info[0] = ox;
+    ws.info[1] = oy;
+    ws.info[2] = single ? 1 : 0;
     *ws.acc_n = total;
     if (R.log) log_count[S.steps_done] = total;
     S.attempts += nvalid;
@@ -310,7 +337,12 @@ HS_INLINE bool reaches(int s, int w, int n, bool wrap, int lo, int hi) {
   return false;
 }
 
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update(hs_instance_t I, hs_select_t SEL, MetroWS ws) {
+HS_INLINE int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int tile_y) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const MetroWS &ws = J.ws;
   __shared__ int s_cell[MU_THREADS];
   __shared__ float s_mmu[MU_THREADS];
   __shared__ int s_site_of[MU_THREADS];
@@ -318,41 +350,120 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update(hs_instance_t I, hs
   const int a = *ws.acc_n;
   if (a == 0) return;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * MU_TX, y0 = blockIdx.y * MU_TY;
+  const int x0 = tile_x * MU_TX, y0 = tile_y * MU_TY;
   const int x = x0 + (tid % MU_TX), y = y0 + (tid / MU_TX);
   const int nx = I.nx, ny = I.ny;
   const bool inside = x < nx && y < ny;
   const int c = inside ? y * nx + x : 0;
-  const int mi = inside ? (int)I.m[c] : 0;
+  int mi = 2;  // the cell's spin, loaded once some source reaches the tile (2 = not loaded)
   const bool has_dip = I.dip_pf != 0.0;
   const int hy = I.nn_h / 2, hx = I.nn_w / 2;
   const bool has_ex = I.J != 0.0 && I.nn_w > 0;
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
   const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
-  const int site_i = hs_site(I, y % ny, x % nx);
-  const float oxi = I.ox_uc[site_i], oyi = I.oy_uc[site_i];
+  float oxi = 0.f, oyi = 0.f;
   float acc = 0.f, accp = 0.f, ex = 0.f, exp_ = 0.f;
   bool touched = false;
-  for (int base = 0; base < a; base += MU_THREADS) {
-    // stage this chunk's sources that reach the tile
+
+  // the contribution of one switched source (cell scell, (m mu) after its flip, unit-cell site ss)
+  auto add_source = [&](int scell, float mm, int ss) {
+    const int sy = scell / nx, sx = scell % nx;
+    if (has_dip) {
+      const int n = I.sub_of[ss];
+      if (n >= 0) {
+        const bool tr = ws.trunc[ss] != 0;
+        const bool wrap = tr && I.pbc;
+        const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
+        int dys[3], dxs[3];
+        const int ndy = disp_candidates(y, sy, ny, wy, wrap, dys);
+        const int ndx = disp_candidates(x, sx, nx, wx, wrap, dxs);
+        const float *Kn = I.K32 + (size_t)n * KS;
+        const float *Kp = I.use_perp ? I.Kpo32 + (size_t)n * KS : nullptr;
+        for (int a1 = 0; a1 < ndy; ++a1)
+          for (int a2 = 0; a2 < ndx; ++a2) {
+            const size_t o = (size_t)(I.kwy + dys[a1]) * KW + (I.kwx + dxs[a2]);
+            acc += mm * Kn[o];
+            if (Kp) accp += mm * Kp[o];
+            touched = true;
+          }
+      }
+    }
+    if (has_ex) {
+      int dy = y - sy, dx = x - sx;
+      if (I.pbc) {
+        dy = hs_pmod(dy, ny);
+        dy = dy > ny / 2 ? dy - ny : dy;
+        dx = hs_pmod(dx, nx);
+        dx = dx > nx / 2 ? dx - nx : dx;
+      }
+      if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
+        const float wq = I.nn_mask[(dy + hy) * I.nn_w + (dx + hx)];
+        if (wq != 0.f) {
+          const float msn = mm > 0.f ? 1.f : -1.f;  // m of the source after its flip
+          if (I.in_plane) {
+            const float oxs = I.ox_uc[ss], oys = I.oy_uc[ss];
+            ex += msn * wq * (oxi * oxs + oyi * oys);
+            exp_ += msn * wq * (oys * oxi - oxs * oyi);
+          } else {
+            ex += msn * wq;
+          }
+          touched = true;
+        }
+      }
+    }
+  };
+
+  // Direct lookup: the sample of chosen supercell j = rs*ncols + cs lies in rows
+  // [2 rs Ry + oy, +Ry) and columns [2 cs Rx + ox, +Rx) (core.py:540-552), so only supercells
+  // within the largest window of the tile can reach it.
+  bool lookup = false;
+  int rs0 = 0, rs1 = -1, cs0 = 0, cs1 = -1;
+  if (SEL.scheme == 0 && !I.pbc && ws.info[2] == 0) {
+    const int ox = ws.info[0], oy = ws.info[1];
+    const int Wx = max(has_dip ? I.kwx : 0, has_ex ? hx : 0), Wy = max(has_dip ? I.kwy : 0, has_ex ? hy : 0);
+    const int nrows = SEL.nsc / SEL.ncols;
+    cs0 = max(0, -floor_div(-(x0 - Wx - ox - SEL.Rx + 1), 2 * SEL.Rx));
+    cs1 = min(SEL.ncols - 1, floor_div(x0 + MU_TX - 1 + Wx - ox, 2 * SEL.Rx));
+    rs0 = max(0, -floor_div(-(y0 - Wy - oy - SEL.Ry + 1), 2 * SEL.Ry));
+    rs1 = min(nrows - 1, floor_div(y0 + MU_TY - 1 + Wy - oy, 2 * SEL.Ry));
+    lookup = (int64_t)max(0, cs1 - cs0 + 1) * max(0, rs1 - rs0 + 1) <= (int64_t)a;
+  }
+  // stage the sources that reach the tile in LDS (in selection order), chunk by chunk, and let
+  // every cell of the tile add their contributions
+  const int ncs = max(0, cs1 - cs0 + 1);
+  const int nsrc = lookup ? ncs * max(0, rs1 - rs0 + 1) : a;
+  for (int base = 0; base < nsrc; base += MU_THREADS) {
     const int sidx = base + tid;
     bool rel = false;
     int sc = 0, ssite = 0;
     float smmu = 0.f;
-    if (sidx < a) {
-      sc = ws.acc_cell[sidx];
-      smmu = ws.acc_mmu[sidx];
-      const int sy = sc / nx, sx = sc % nx;
-      ssite = hs_site(I, sy, sx);
-      if (has_dip && I.sub_of[ssite] >= 0) {
-        const bool tr = ws.trunc[ssite] != 0;
-        const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
-        const bool wrap = tr && I.pbc;
-        rel = reaches(sx, wx, nx, wrap, x0, x0 + MU_TX - 1) && reaches(sy, wy, ny, wrap, y0, y0 + MU_TY - 1);
+    if (sidx < nsrc) {
+      bool have;
+      if (lookup) {
+        const int j = (rs0 + sidx / ncs) * SEL.ncols + cs0 + sidx % ncs;
+        have = ws.accf[j] != 0;
+        if (have) {
+          sc = ws.cand[j];
+          smmu = (float)I.m[sc] * I.mu[sc];  // (m mu) after the flip, as in the selection order list
+        }
+      } else {
+        have = true;
+        sc = ws.acc_cell[sidx];
+        smmu = ws.acc_mmu[sidx];
       }
-      if (!rel && has_ex)
-        rel = reaches(sx, hx, nx, I.pbc, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, I.pbc, y0, y0 + MU_TY - 1);
+      if (have) {
+        const int sy = sc / nx, sx = sc % nx;
+        ssite = hs_site(I, sy, sx);
+        if (has_dip && I.sub_of[ssite] >= 0) {
+          const bool tr = ws.trunc[ssite] != 0;
+          const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
+          const bool wrap = tr && I.pbc;
+          rel = reaches(sx, wx, nx, wrap, x0, x0 + MU_TX - 1) && reaches(sy, wy, ny, wrap, y0, y0 + MU_TY - 1);
+        }
+        if (!rel && has_ex)
+          rel = reaches(sx, hx, nx, I.pbc, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, I.pbc, y0, y0 + MU_TY - 1);
+      }
     }
     const unsigned long long ball = __ballot(rel);
     const int lane = tid & 63, wid = tid >> 6;
@@ -371,58 +482,19 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update(hs_instance_t I, hs
       s_site_of[p] = ssite;
     }
     __syncthreads();
-    if (nrel == 0 || !inside || mi == 0) continue;
-    for (int r = 0; r < nrel; ++r) {
-      const int scell = s_cell[r];
-      const int sy = scell / nx, sx = scell % nx;
-      const int ss = s_site_of[r];
-      const float mm = s_mmu[r];
-      if (has_dip) {
-        const int n = I.sub_of[ss];
-        if (n >= 0) {
-          const bool tr = ws.trunc[ss] != 0;
-          const bool wrap = tr && I.pbc;
-          const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
-          int dys[3], dxs[3];
-          const int ndy = disp_candidates(y, sy, ny, wy, wrap, dys);
-          const int ndx = disp_candidates(x, sx, nx, wx, wrap, dxs);
-          const float *Kn = I.K32 + (size_t)n * KS;
-          const float *Kp = I.use_perp ? I.Kpo32 + (size_t)n * KS : nullptr;
-          for (int a1 = 0; a1 < ndy; ++a1)
-            for (int a2 = 0; a2 < ndx; ++a2) {
-              const size_t o = (size_t)(I.kwy + dys[a1]) * KW + (I.kwx + dxs[a2]);
-              acc += mm * Kn[o];
-              if (Kp) accp += mm * Kp[o];
-              touched = true;
-            }
-        }
-      }
-      if (has_ex) {
-        int dy = y - sy, dx = x - sx;
-        if (I.pbc) {
-          dy = hs_pmod(dy, ny);
-          dy = dy > ny / 2 ? dy - ny : dy;
-          dx = hs_pmod(dx, nx);
-          dx = dx > nx / 2 ? dx - nx : dx;
-        }
-        if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
-          const float wq = I.nn_mask[(dy + hy) * I.nn_w + (dx + hx)];
-          if (wq != 0.f) {
-            const float msn = mm > 0.f ? 1.f : -1.f;  // m of the source after its flip
-            if (I.in_plane) {
-              const float oxs = I.ox_uc[ss], oys = I.oy_uc[ss];
-              ex += msn * wq * (oxi * oxs + oyi * oys);
-              exp_ += msn * wq * (oys * oxi - oxs * oyi);
-            } else {
-              ex += msn * wq;
-            }
-            touched = true;
-          }
-        }
+    if (nrel == 0) continue;
+    if (mi == 2) {  // first chunk with sources: load the cell's own spin and site
+      mi = inside ? (int)I.m[c] : 0;
+      if (mi != 0) {
+        const int site_i = hs_site(I, y, x);
+        oxi = I.ox_uc[site_i];
+        oyi = I.oy_uc[site_i];
       }
     }
+    if (!inside || mi == 0) continue;
+    for (int r = 0; r < nrel; ++r) add_source(s_cell[r], s_mmu[r], s_site_of[r]);
   }
-  if (!inside || mi == 0 || !touched) return;
+  if (!inside || mi == 0 || mi == 2 || !touched) return;
   const float mmui = (float)mi * I.mu[c];
   // E += 2*prefactor*(m mu)_i*conv (energies.py:362-363); exchange: -J m_i * 2 m_s * NN
   const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
@@ -439,12 +511,72 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update(hs_instance_t I, hs
   }
 }
 
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J) { metro_select(J); }
+
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob *__restrict__ jobs) {
+  metro_select(jobs[blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J) { metro_update(J, blockIdx.x, blockIdx.y); }
+
+// One workgroup per tile of every job: tile_job[b] is the job of block b, tile_first[j] its first block.
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob *__restrict__ jobs,
+                                                                  const int32_t *__restrict__ tile_job,
+                                                                  const int32_t *__restrict__ tile_first) {
+  const int jb = tile_job[blockIdx.x];
+  const MetroJob &J = jobs[jb];
+  const int t = blockIdx.x - tile_first[jb];
+  metro_update(J, t % J.tiles_x, t / J.tiles_x);
+}
+
+__global__ void k_state_begin_many(const MetroJob *__restrict__ jobs, int njobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  hs_state_t s = jobs[j].st[0];
+  if (jobs[j].R.begin) {
+    s.t0 = s.t;
+    s.attempts0 = s.attempts;
+  }
+  s.done = 0;
+  s.steps_done = 0;
+  s.err = 0;
+  jobs[j].st[0] = s;
+}
+
+int check_select(const hs_select_t *sel) {
+  if (sel->scheme == 0) {
+    if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
+      return hs_set_error(HS_ERR_ARG, "bad selection geometry");
+    if (sel->Rx > 0xffff || sel->Ry > 0x7fff) return hs_set_error(HS_ERR_ARG, "supercell too large");
+    if (sel->nsc % sel->ncols) return hs_set_error(HS_ERR_ARG, "nsc is not a multiple of ncols");
+  } else if (sel->scheme != 1) {
+    return hs_set_error(HS_ERR_ARG, "unsupported MULTISAMPLING_SCHEME");
+  }
+  return HS_OK;
+}
+
+MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run, hs_state_t *state, void *ws,
+                  int32_t *log_cells, int32_t *log_count) {
+  MetroJob J;
+  J.I = *inst;
+  J.SEL = *sel;
+  J.SEL.nsc = sel->scheme == 0 ? sel->nsc : 1;
+  J.R = *run;
+  J.st = state;
+  J.ws = carve(ws, J.SEL.nsc);
+  J.log_cells = log_cells;
+  J.log_count = log_count;
+  J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
+  J.tiles_y = (inst->ny + MU_TY - 1) / MU_TY;
+  return J;
+}
+
 }  // namespace
 
 extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
   (void)inst;
   if (!sel) return -1;
-  return (int64_t)ws_bytes(sel->nsc);
+  return (int64_t)ws_bytes(sel->scheme == 0 ? sel->nsc : 1);
 }
 
 extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
@@ -455,26 +587,106 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   if (!sel || !run || !state || !ws) return hs_set_error(HS_ERR_ARG, "missing sel/run/state/ws");
   if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
   if (run->log && (!log_cells || !log_count)) return hs_set_error(HS_ERR_ARG, "log requested without arrays");
-  if (sel->scheme == 0) {
-    if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
-      return hs_set_error(HS_ERR_ARG, "bad selection geometry");
-    if (sel->Rx > 0xffff || sel->Ry > 0x7fff) return hs_set_error(HS_ERR_ARG, "supercell too large");
-  } else if (sel->scheme != 1) {
-    return hs_set_error(HS_ERR_ARG, "unsupported MULTISAMPLING_SCHEME");
-  }
+  rc = check_select(sel);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  const int nsc = sel->scheme == 0 ? sel->nsc : 1;
-  hs_select_t S = *sel;
-  S.nsc = nsc;
-  MetroWS w = carve(ws, nsc);
+  const MetroJob J = make_job(inst, sel, run, state, ws, log_cells, log_count);
   k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
   HS_LAUNCH_CHECK("k_state_begin");
-  const dim3 grid((inst->nx + MU_TX - 1) / MU_TX, (inst->ny + MU_TY - 1) / MU_TY);
+  const dim3 grid(J.tiles_x, J.tiles_y);
   for (int64_t j = 0; j < run->max_steps; ++j) {
-    k_metro_select<<<1, MS_THREADS, 0, s>>>(*inst, S, *run, state, w, log_cells, log_count);
+    k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J);
     HS_LAUNCH_CHECK("k_metro_select");
-    k_metro_update<<<grid, MU_THREADS, 0, s>>>(*inst, S, w);
+    k_metro_update_one<<<grid, MU_THREADS, 0, s>>>(J);
     HS_LAUNCH_CHECK("k_metro_update");
+  }
+  return HS_OK;
+}
+
+namespace {
+
+struct MetroLayout {
+  size_t jobs_off, tile_job_off, tile_first_off, total;
+  std::vector<size_t> ws_off;
+  std::vector<int32_t> first;
+  int32_t ntiles;
+};
+
+void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
+  L.jobs_off = 0;
+  L.ntiles = 0;
+  L.first.resize(n);
+  for (int i = 0; i < n; ++i) {
+    L.first[i] = L.ntiles;
+    const hs_instance_t &in = *teams[i].inst;
+    L.ntiles += ((in.nx + MU_TX - 1) / MU_TX) * ((in.ny + MU_TY - 1) / MU_TY);
+  }
+  L.tile_job_off = align256(sizeof(MetroJob) * (size_t)n);
+  L.tile_first_off = L.tile_job_off + align256(4 * (size_t)L.ntiles);
+  size_t off = L.tile_first_off + align256(4 * (size_t)n);
+  L.ws_off.resize(n);
+  for (int i = 0; i < n; ++i) {
+    L.ws_off[i] = off;
+    off += ws_bytes(teams[i].sel->scheme == 0 ? teams[i].sel->nsc : 1);
+  }
+  L.total = off;
+}
+
+}  // namespace
+
+extern "C" int64_t hs_metro_teams_ws_bytes(const hs_team_t *teams, int32_t nteams) {
+  if (!teams || nteams <= 0) return -1;
+  for (int i = 0; i < nteams; ++i)
+    if (!teams[i].inst || !teams[i].sel) return -1;
+  MetroLayout L;
+  metro_layout(teams, nteams, L);
+  return (int64_t)L.total;
+}
+
+extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes_, void *stream) {
+  hs_clear_error();
+  if (!teams || nteams <= 0 || !ws) return hs_set_error(HS_ERR_ARG, "missing teams / workspace");
+  int64_t steps = 0;
+  for (int i = 0; i < nteams; ++i) {
+    const hs_team_t &t = teams[i];
+    int rc = hs_check_instance(t.inst);
+    if (rc) return rc;
+    if (!t.sel || !t.state) return hs_set_error(HS_ERR_ARG, "team %d has no selection / state", i);
+    rc = check_select(t.sel);
+    if (rc) return rc;
+    if (t.run.max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
+    if (t.run.log && (!t.log_cells || !t.log_idx)) return hs_set_error(HS_ERR_ARG, "team %d: log without arrays", i);
+    steps = t.run.max_steps > steps ? t.run.max_steps : steps;
+  }
+  MetroLayout L;
+  metro_layout(teams, nteams, L);
+  if (ws_bytes_ < (int64_t)L.total)
+    return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes_, (long long)L.total);
+  char *base = (char *)ws;
+  std::vector<char> host(L.tile_first_off + align256(4 * (size_t)nteams), 0);
+  MetroJob *jobs = (MetroJob *)(host.data() + L.jobs_off);
+  int32_t *tile_job = (int32_t *)(host.data() + L.tile_job_off);
+  int32_t *tile_first = (int32_t *)(host.data() + L.tile_first_off);
+  for (int i = 0; i < nteams; ++i) {
+    const hs_team_t &t = teams[i];
+    jobs[i] = make_job(t.inst, t.sel, &t.run, t.state, base + L.ws_off[i], t.log_cells, t.log_idx);
+    tile_first[i] = L.first[i];
+    const int nt = jobs[i].tiles_x * jobs[i].tiles_y;
+    for (int b = 0; b < nt; ++b) tile_job[L.first[i] + b] = i;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
+  HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  const MetroJob *dj = (const MetroJob *)(base + L.jobs_off);
+  const int32_t *dtj = (const int32_t *)(base + L.tile_job_off);
+  const int32_t *dtf = (const int32_t *)(base + L.tile_first_off);
+  k_state_begin_many<<<(nteams + 255) / 256, 256, 0, s>>>(dj, nteams);
+  HS_LAUNCH_CHECK("k_state_begin_many");
+  for (int64_t j = 0; j < steps; ++j) {
+    k_metro_select_many<<<nteams, MS_THREADS, 0, s>>>(dj);
+    HS_LAUNCH_CHECK("k_metro_select_many");
+    k_metro_update_many<<<L.ntiles, MU_THREADS, 0, s>>>(dj, dtj, dtf);
+    HS_LAUNCH_CHECK("k_metro_update_many");
   }
   return HS_OK;
 }

@@ -264,14 +264,30 @@ int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel
 
 /*
  * Neel-Arrhenius steps for many instances in ONE persistent cooperative launch (same
- * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760). Each
- * team keeps its magnets' fields in registers, draws the same Philox streams and reduces its
- * winner with an atomic (key, index) minimum plus a team barrier per step; teams never wait
- * for each other. ws: device scratch of hs_neel_teams_ws_bytes bytes. Blocking for the small
- * descriptor upload, otherwise asynchronous.
+ * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760, and
+ * bit-identical results). Each team keeps its magnets' fields in registers and draws the same
+ * Philox streams; a team of several workgroups exchanges one tagged record per workgroup per
+ * step (its best candidate and that cell's waiting time) and polls for the round's records
+ * (bounded spin, err = 3 on timeout); teams never wait for each other. ws: device scratch of
+ * hs_neel_teams_ws_bytes bytes. Blocking for the small descriptor upload, otherwise
+ * asynchronous. Grids need ny * 2^ceil(log2 nx) <= 2^24.
  */
 int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
 int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
+
+/*
+ * Metropolis steps for many instances at once (same semantics per instance as
+ * hs_metropolis_run: core.py:659-677 with select core.py:456-558 and the update_multiple
+ * branch rule energies.py:331-366, looped as core.py:736-760, and bit-identical results):
+ * every step is one selection launch with a workgroup per instance and one update launch over
+ * the output tiles of all instances; each instance stops on its own predicate. team.sel is
+ * required; team.log_cells / team.log_idx are the optional accepted-cell / count logs of
+ * hs_metropolis_run; nblocks and log_dt are unused. ws: device scratch of
+ * hs_metro_teams_ws_bytes bytes. Blocking for the small descriptor upload, otherwise
+ * asynchronous.
+ */
+int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
+int64_t hs_metro_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 
 #ifdef __cplusplus
 }

@@ -39,45 +39,73 @@ CASES = {
 }
 
 
+# Oracle-only cases (no reference fixture): seeded synthetic pattern and E_B. The window case
+# has a short dipolar cutoff with a matching REDUCED_KERNEL_SIZE and no PBC, so the Metropolis
+# update reaches its sources both by supercell lookup and by filtering the whole list.
+SYNTH = {
+    "metro_oop_window": dict(lattice="OOP_Square", a=1e-6, n=64, T=900, moment=1.1e-15,
+                             energies=[("dipolar", dict(cutoff_range=3e-6)), ("exchange", dict(J_kT=0.1))],
+                             params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=3,
+                                         SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
+                             update_kw=dict(r=6e-6), nsteps=150, seed=31),
+    "metro_ipsq_window": dict(lattice="IP_Square", a=1e-6, n=48, T=400, moment=2.37e-16,
+                              energies=[("dipolar", dict(cutoff_range=3e-6)), ("zeeman", dict(magnitude=1e-3, angle=0.7))],
+                              params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,
+                                          SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=1),
+                              update_kw=dict(r=5e-6), nsteps=150, seed=32),
+}
+
+
+def spec(name):
+    return CASES[name] if name in CASES else SYNTH[name]
+
+
 def load(name):
+    if name in SYNTH:
+        sp = SYNTH[name]
+        rng = np.random.default_rng(sp["seed"])
+        n = sp["n"]
+        kT = kB * sp["T"]
+        return dict(pattern=np.where(rng.random((n, n)) < 0.5, -1.0, 1.0),
+                    E_B=20 * kT * rng.normal(1, 0.05, size=(n, n)))
     return dict(np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False))
 
 
 def oracle_magnets(name, fx=None):
     """OracleMagnets for a fixture case, with the fixture's pattern and E_B."""
     from oracle import model
-    spec = CASES[name]
+    sp = spec(name)
     fx = load(name) if fx is None else fx
-    kT = kB * spec["T"]
+    kT = kB * sp["T"]
     ens = []
-    for kind, kw in spec["energies"]:
+    for kind, kw in sp["energies"]:
         if kind == "dipolar":
             ens.append(model.Dipolar(**kw))
         elif kind == "zeeman":
             ens.append(model.Zeeman(**kw))
         else:
             ens.append(model.Exchange(J=kw["J_kT"] * kT))
-    params = dict(spec["params"])
+    params = dict(sp["params"])
     params["UPDATE_SCHEME"] = "Neel" if params["UPDATE_SCHEME"] in ("Néel", "Neel") else "Metropolis"
-    return model.make_magnets(spec["lattice"], spec["a"], n=spec["n"], T=spec["T"], E_B=fx["E_B"],
-                              moment=spec["moment"], pattern=fx["pattern"], energies=ens, params=params,
-                              seed=spec["seed"], PBC=spec.get("kw", {}).get("PBC", False))
+    return model.make_magnets(sp["lattice"], sp["a"], n=sp["n"], T=sp["T"], E_B=fx["E_B"],
+                              moment=sp["moment"], pattern=fx["pattern"], energies=ens, params=params,
+                              seed=sp["seed"], PBC=sp.get("kw", {}).get("PBC", False))
 
 
 def product_magnets(name, fx=None, **extra):
     """hotspice_amd Magnets for a fixture case (same pattern, E_B and Philox seed)."""
     import hotspice_amd as hs
-    spec = CASES[name]
+    sp = spec(name)
     fx = load(name) if fx is None else fx
-    kT = kB * spec["T"]
+    kT = kB * sp["T"]
     ens = []
-    for kind, kw in spec["energies"]:
+    for kind, kw in sp["energies"]:
         if kind == "dipolar":
             ens.append(hs.DipolarEnergy(**kw))
         elif kind == "zeeman":
             ens.append(hs.ZeemanEnergy(**kw))
         else:
             ens.append(hs.ExchangeEnergy(J=kw["J_kT"] * kT))
-    cls = getattr(hs.ASI, spec["lattice"])
-    return cls(spec["a"], spec["n"], T=spec["T"], E_B=fx["E_B"], moment=spec["moment"], pattern=fx["pattern"],
-               energies=ens, params=hs.SimParams(**spec["params"]), seed=spec["seed"], **spec.get("kw", {}), **extra)
+    cls = getattr(hs.ASI, sp["lattice"])
+    return cls(sp["a"], sp["n"], T=sp["T"], E_B=fx["E_B"], moment=sp["moment"], pattern=fx["pattern"],
+               energies=ens, params=hs.SimParams(**sp["params"]), seed=sp["seed"], **sp.get("kw", {}), **extra)

@@ -14,7 +14,7 @@ import math
 import numpy as np
 import pytest
 
-from cases import CASES, load, oracle_magnets, product_magnets
+from cases import CASES, SYNTH, load, oracle_magnets, product_magnets, spec as case_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,7 @@ def _neel_near_tie(om, g_idx, o_idx):
 
 def lockstep(name, nsteps=None, check_every=25):
     """Steps product and oracle side by side; returns (steps matched, near-tie step or None)."""
-    spec = CASES[name]
+    spec = case_spec(name)
     fx = load(name)
     mm = product_magnets(name, fx)
     om = oracle_magnets(name, fx)
@@ -81,7 +81,7 @@ def test_initial_fields_match_golden(name):
     assert mm.E_tot == pytest.approx(float(fx["E_tot0"]), rel=1e-9, abs=1e-9 * scale)
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("name", sorted(CASES) + sorted(SYNTH))
 def test_trajectory_matches_oracle(name):
     done, tie = lockstep(name)
     assert done >= 1

@@ -76,3 +76,29 @@ def test_batch_equals_alone():
     for m1, m2, r in zip(alone, mixed, res):
         _same(_snapshot(m1), _snapshot(m2))
         assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
+
+
+
+METRO = ["metro_oop_window", "metro_ipsq_window", "metro_pinwheel", "metro_oop_conv"]
+
+
+def _metro(name):
+    from cases import load, product_magnets
+    return product_magnets(name, load(name))
+
+
+def test_metropolis_batch_equals_alone():
+    """Every instance of a Metropolis batch (one selection launch with a workgroup per instance
+    and one update launch over all instances' tiles per step) ends bit-identical to stepping it
+    alone through Magnets.progress."""
+    from cases import spec
+    from hotspice_amd import batch
+    kws = [spec(n)["update_kw"] for n in METRO]
+    alone = [_metro(n) for n in METRO]
+    for m, kw in zip(alone, kws):
+        m.progress(t_max=math.inf, MCsteps_max=2.0, **kw)
+    mixed = [_metro(n) for n in METRO]
+    res = batch.progress_many(mixed, t_max=math.inf, MCsteps_max=2.0, per_instance=kws)
+    for m1, m2, r in zip(alone, mixed, res):
+        _same(_snapshot(m1), _snapshot(m2))
+        assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
