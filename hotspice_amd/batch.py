@@ -1,10 +1,13 @@
-"""Batched stepping: many independent Magnets instances in one persistent launch.
+"""Batched stepping: many independent Magnets instances advanced together.
 
 progress_many([...]) runs Magnets.progress() (hotspice/core.py:736-760) on every instance,
-with each instance's own t_max / MCsteps_max, temperature, fields and Philox seed, in ONE
-cooperative launch per chunk: every instance is a team of workgroups that steps on its own,
-so the results are those of stepping each instance alone (the Philox streams are keyed by
-the instance's seed and step counter, not by its position in the batch).
+with each instance's own t_max / MCsteps_max, temperature, fields and Philox seed. Neel
+instances step in ONE persistent cooperative launch per chunk (every instance is a team of
+workgroups that steps on its own); Metropolis instances step together, one selection launch
+and one update launch per step for all of them. Results are those of stepping each instance
+alone: the Philox streams are keyed by the instance's seed and step counter, not by its
+position in the batch, and every instance's launches are chunked as Magnets.progress chunks
+them. launch_neel / launch_metropolis are the single-launch building blocks (used by serve.py).
 """
 import math
 
@@ -58,54 +61,104 @@ def _pending(m, start, t_max, mc):
     return lower_than(st.t - t0, t_max) and lower_than(st.attempts / m.n - a0 / m.n, mc), st
 
 
-def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk):
-    dev = mags[0]._dev
-    first = True
-    while True:
-        teams = []
-        for m, s0, nb in zip(mags, starts, nbl):
-            pend, st = _pending(m, s0, t_max, mc)
-            if not pend:
-                continue
-            if not m._uniform_beta:
-                raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-            if math.isfinite(mc):
-                steps = int(math.ceil(mc * m.n - (st.attempts - s0[1]))) + 1
-            else:
-                steps = 4096
-            if chunk:
-                steps = min(steps, chunk)
-            if math.isfinite(t_max):
-                steps = min(steps, 4096)
-            if m.recompute_every:
-                steps = min(steps, max(1, m.recompute_every - m._since_recompute))
-            if first:
-                m._set_progress_origin(*s0)
-            run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=f0, max_steps=steps, log=0, nblocks=0,
-                           begin=0)
-            teams.append((m, _lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=None, run=run,
-                                       state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None,
-                                       nblocks=int(nb))))
-        first = False
-        if not teams:
-            break
-        dev.run_teams([t for _, t in teams], "neel")
-        for m, _ in teams:
-            st = m._state()
-            if st.err:
-                raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
-            m._since_recompute += int(st.steps_done)
-            m._maybe_recompute()
-    out = []
-    for m, (t0, a0) in zip(mags, starts):
-        m._after_device_step()
+def neel_team_size(m, nblocks=0):
+    """Workgroups of an instance's Neel team: `nblocks`, else the library's default of one grid
+    cell per thread (1024 per workgroup), at most 128."""
+    if nblocks:
+        return int(nblocks)
+    return max(1, min(128, -(-(m.nx * m.ny) // 1024)))
+
+
+def launch_neel(items, attempt_freq=1e10):
+    """ONE persistent launch stepping every (mm, max_steps, t_max, mc_max, nblocks) item (the
+    progress origin of each instance must be set). Afterwards checks each instance's error
+    word, counts its steps towards the periodic FP64 recompute and runs that when due."""
+    if not items:
+        return
+    dev = items[0][0]._dev
+    teams = []
+    for m, steps, t_max, mc, nb in items:
+        if not m._uniform_beta:
+            raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
+        run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=float(attempt_freq), max_steps=int(steps),
+                       log=0, nblocks=0, begin=0)
+        teams.append(_lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=None, run=run, state=m._dev.state.data_ptr(),
+                               log_idx=None, log_dt=None, log_cells=None, nblocks=int(nb)))
+    dev.run_teams(teams, "neel")
+    _after_launch([it[0] for it in items], "Neel")
+
+
+def launch_metropolis(items, attempt_freq=1e10):
+    """One batched Metropolis call stepping every (mm, sel, max_steps, t_max, mc_max) item: a
+    selection launch and an update launch per step for all instances together."""
+    if not items:
+        return
+    dev = items[0][0]._dev
+    teams = []
+    for m, sel, steps, t_max, mc in items:
+        run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=float(attempt_freq), max_steps=int(steps),
+                       log=0, nblocks=0, begin=0)
+        teams.append(_lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=_lib.C.addressof(sel), run=run,
+                               state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None, nblocks=0))
+    dev.run_teams(teams, "metropolis")
+    _after_launch([it[0] for it in items], "Metropolis")
+
+
+def _after_launch(mags, what):
+    for m in mags:
         st = m._state()
-        out.append((st.t - t0, st.attempts / m.n - a0 / m.n))
-    return out
+        if st.err:
+            raise _lib.HotspiceError(f"device {what} step failed (err={st.err})")
+        m._since_recompute += int(st.steps_done)
+        m._maybe_recompute()
+
+
+def neel_chunk(m, start, t_max, mc, chunk=None):
+    """Steps of the next Neel launch for an instance inside progress(t_max, mc) started at
+    `start` = (t0, attempts0): as Magnets.progress chunks it."""
+    st = m._state()
+    if math.isfinite(mc):
+        steps = int(math.ceil(mc * m.n - (st.attempts - start[1]))) + 1
+    else:
+        steps = 4096
+    if chunk:
+        steps = min(steps, chunk)
+    if math.isfinite(t_max):
+        steps = min(steps, 4096)
+    if m.recompute_every:
+        steps = min(steps, max(1, m.recompute_every - m._since_recompute))
+    return max(1, steps)
+
+
+def metropolis_chunk(m, sel, start, mc, chunk=None):
+    """Steps of the next Metropolis launch (as Magnets.progress chunks it)."""
+    st = m._state()
+    per_step = max(1.0, 0.25 * sel.nsc)
+    if math.isfinite(mc):
+        steps = int(math.ceil((mc * m.n - (st.attempts - start[1])) / per_step)) + 1
+        steps = max(1, min(steps, 1 << 16))
+    else:
+        steps = 1024
+    if chunk:
+        steps = min(steps, chunk)
+    if m.recompute_every and m._incremental_exact():
+        steps = min(steps, max(1, m.recompute_every - m._since_recompute))
+    return max(1, steps)
+
+
+def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk):
+    for m, s0 in zip(mags, starts):
+        m._set_progress_origin(*s0)
+    while True:
+        items = [(m, neel_chunk(m, s0, t_max, mc, chunk), t_max, mc, nb)
+                 for m, s0, nb in zip(mags, starts, nbl) if _pending(m, s0, t_max, mc)[0]]
+        if not items:
+            break
+        launch_neel(items, f0)
+    return _results(mags, starts)
 
 
 def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws):
-    dev = mags[0]._dev
     sels = []
     for m, kw in zip(mags, kws):
         if kw.get('idx') is not None:
@@ -113,40 +166,18 @@ def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws):
         r = kw.get('r', 0)
         Q = kw.get('Q', 0.05)
         sels.append(m._select_geometry(m.calc_r(Q) if r == 0 else r))
-    first = True
+    for m, s0 in zip(mags, starts):
+        m._set_progress_origin(*s0)
     while True:
-        teams = []
-        for m, s0, sel in zip(mags, starts, sels):
-            pend, st = _pending(m, s0, t_max, mc)
-            if not pend:
-                continue
-            per_step = max(1.0, 0.25 * sel.nsc)
-            if math.isfinite(mc):
-                steps = int(math.ceil((mc * m.n - (st.attempts - s0[1])) / per_step)) + 1
-                steps = max(1, min(steps, 1 << 16))
-            else:
-                steps = 1024
-            if chunk:
-                steps = min(steps, chunk)
-            if m.recompute_every and m._incremental_exact():
-                steps = min(steps, max(1, m.recompute_every - m._since_recompute))
-            if first:
-                m._set_progress_origin(*s0)
-            run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=f0, max_steps=steps, log=0, nblocks=0,
-                           begin=0)
-            teams.append((m, _lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=_lib.C.addressof(sel), run=run,
-                                       state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None,
-                                       nblocks=0)))
-        first = False
-        if not teams:
+        items = [(m, sel, metropolis_chunk(m, sel, s0, mc, chunk), t_max, mc)
+                 for m, s0, sel in zip(mags, starts, sels) if _pending(m, s0, t_max, mc)[0]]
+        if not items:
             break
-        dev.run_teams([t for _, t in teams], "metropolis")
-        for m, _ in teams:
-            st = m._state()
-            if st.err:
-                raise _lib.HotspiceError(f"device Metropolis step failed (err={st.err})")
-            m._since_recompute += int(st.steps_done)
-            m._maybe_recompute()
+        launch_metropolis(items, f0)
+    return _results(mags, starts)
+
+
+def _results(mags, starts):
     out = []
     for m, (t0, a0) in zip(mags, starts):
         m._after_device_step()

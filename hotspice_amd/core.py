@@ -26,6 +26,13 @@ from .utils import as_2D_array, check_repetition, full_obj_name, lower_than
 kB = 1.380649e-23
 
 
+
+class _Released:
+    """Stands in for the device state of an instance whose device memory was released."""
+
+    def __getattr__(self, name):
+        raise _lib.HotspiceError("this Magnets instance released its device memory (release_device())")
+
 class Scheme(Enum):
     """core.py:29-32"""
     NEEL = auto()
@@ -396,6 +403,14 @@ class Magnets(ABC):
             self._device_params_changed()
             if not used and self.USE_PERP_ENERGY:
                 self._device_recompute()
+
+    def release_device(self):
+        """Frees this instance's device arrays (a server retiring a finished job). Reading host
+        attributes still works; stepping or reading device state afterwards raises."""
+        self._m_host = self.m.copy()
+        self._m_ref = self._m_host.copy()
+        self._dev = _Released()
+        self._sel, self._sel_tensors, self._sel_key = None, (), None
 
     # ---- counters (device run state) ------------------------------------------------------------
     def _state(self):

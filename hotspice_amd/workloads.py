@@ -73,3 +73,85 @@ def neel_bytes_per_attempt(n_occ, window_occ, perp=True):
     compensation (8 + 8 B), reads m (1 B) and, with perp channels, E_perp + compensation (16 B)
     for every magnet in the kernel window."""
     return n_occ * (4 + 4 + (4 if perp else 0)) + window_occ * (8 + 8 + 1 + (16 if perp else 0))
+
+
+# ---- batched and served workloads ------------------------------------------------------------------
+
+def ipsq_batch_specs(count=64, seed=3):
+    """64 independent in-plane square ASI: n_i in {32, 48, ..., 512} drawn log-uniform, T_i ~ U(100, 400) K,
+    Zeeman B_i ~ U(0, 5) mT at theta_i ~ U(0, 2 pi); a = 1 um, moment 2.37e-16 A m^2 (borrowed from the
+    pinwheel workload), dipolar cutoff 10 cell pitches with REDUCED_KERNEL_SIZE = 10 and
+    SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF = 0, Metropolis grid with Q = 0.05."""
+    rng = np.random.default_rng(seed)
+    sizes = np.arange(32, 513, 16)
+    logn = rng.uniform(np.log(32), np.log(512), size=count)
+    ns = [int(sizes[np.argmin(np.abs(np.log(sizes) - v))]) for v in logn]
+    out = []
+    for i, n in enumerate(ns):
+        out.append(dict(n=n, T=float(rng.uniform(100, 400)), B=float(rng.uniform(0, 5e-3)),
+                        theta=float(rng.uniform(0, 2 * math.pi)), seed=int(seed * 1000 + i)))
+    return out
+
+
+def ipsq_batch_instance(spec, a=1e-6):
+    import hotspice_amd as hs
+    n = spec["n"]
+    rng = np.random.default_rng(spec["seed"])
+    kT = kB * spec["T"]
+    mm = hs.ASI.IP_Square(a, n, T=spec["T"], E_B=20 * kT * rng.normal(1, 0.05, size=(n, n)), moment=2.37e-16,
+                          pattern=np.where(rng.random((n, n)) < 0.5, -1.0, 1.0), seed=spec["seed"],
+                          energies=[hs.DipolarEnergy(cutoff_range=10 * a / 2), hs.ZeemanEnergy(spec["B"], spec["theta"])],
+                          params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=10,
+                                              SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0))
+    return mm
+
+
+def clocking_fields(steps=100, peak=3e-3):
+    """Field steps of the served jobs: magnitude 1.5 mT (1 - cos(2 pi i / steps)) (0 -> 3 -> 0 mT),
+    angle 0 for even i and 45 deg for odd i."""
+    return [(peak / 2 * (1 - math.cos(2 * math.pi * i / steps)), 0.0 if i % 2 == 0 else math.pi / 4)
+            for i in range(steps)]
+
+
+def served_jobs(count=32, rate=200.0, seed=4, n_range=(64, 1024), field_steps=100, neel_work=1000,
+                metro_sweeps=4.0):
+    """Jobs of the served workload: Poisson arrivals at `rate` jobs/s; 50 % IP_Kagome / 50 %
+    IP_Pinwheel with n log-uniform in n_range; 50 % Neel (`neel_work` switch attempts per field
+    step) / 50 % Metropolis (`metro_sweeps` MC sweeps per field step); `field_steps` clocking
+    field steps; other parameters as the pinwheel workload (a = 1 um, moment 2.37e-16, T = 300 K,
+    E_B lognormal median 60 kT / 5 %, full dipolar kernels)."""
+    from .serve import Job, footprint_bytes
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    fields = clocking_fields(field_steps)
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        lattice = "IP_Kagome" if rng.random() < 0.5 else "IP_Pinwheel"
+        scheme = "Néel" if rng.random() < 0.5 else "Metropolis"
+        jseed = int(seed * 100000 + i)
+
+        def make(n=n, lattice=lattice, scheme=scheme, jseed=jseed):
+            import hotspice_amd as hs
+            r = np.random.default_rng(jseed)
+            kT = kB * 300
+            cls = getattr(hs.ASI, lattice)
+            shape = _lattice_shape(lattice, n)
+            return cls(1e-6, n, T=300, E_B=60 * kT * np.exp(0.05 * r.normal(0, 1, size=shape)), moment=2.37e-16,
+                       pattern=np.where(r.random(shape) < 0.5, -1.0, 1.0), seed=jseed,
+                       energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(0.0, 0.0)],
+                       params=hs.SimParams(UPDATE_SCHEME=scheme))
+        nx, ny = _lattice_shape(lattice, n)[::-1]
+        n_sub = 6 if lattice == "IP_Kagome" else 2
+        work = float(neel_work) if scheme == "Néel" else float(metro_sweeps)
+        jobs.append(Job(make=make, fields=fields, work=work, arrival=t, footprint=footprint_bytes(nx, ny, n_sub),
+                        name=f"{lattice} n={n} {scheme}"))
+    return jobs
+
+
+def _lattice_shape(lattice, n):
+    """(ny, nx) of the lattices the served jobs use (ASI.py:371-382 for Kagome)."""
+    if lattice == "IP_Kagome":
+        return (int(n / math.sqrt(3)) // 4 * 4 - 1, n)
+    return (n, n)

@@ -1,0 +1,46 @@
+"""Continuous-batching server: every job's result is the same served alone and served in a mix
+of jobs that arrive while others run (bit-identical spins, counters, time and energy), and the
+server reports latencies for every job."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    from hotspice_amd.workloads import served_jobs
+    # small instances and few field steps so the test stays short; arrivals 2 ms apart on average
+    return served_jobs(count=8, rate=500.0, seed=7, n_range=(16, 40), field_steps=4, neel_work=150, metro_sweeps=1.0)
+
+
+def _serve(jobs, **kw):
+    from hotspice_amd.serve import Server
+    srv = Server(**kw)
+    for j in jobs:
+        srv.submit(j)
+    return srv.run(timeout=300), srv
+
+
+def test_served_job_equals_alone():
+    mixed, srv = _serve(_jobs())
+    assert len(mixed) == 8 and srv.iterations > 0
+    by_name = {(j.name, j.arrival): j.result for j in mixed}
+    for job in _jobs():
+        job.arrival = 0.0
+        (alone,), _ = _serve([job])
+        match = [r for (name, arr), r in by_name.items() if name == job.name]
+        assert any(np.array_equal(r["m"], alone.result["m"]) and r["t"] == alone.result["t"]
+                   and r["attempts"] == alone.result["attempts"] and r["switches"] == alone.result["switches"]
+                   and r["E_tot"] == alone.result["E_tot"] for r in match), job.name
+
+
+def test_latency_stats_and_budget():
+    from hotspice_amd.serve import latency_stats
+    jobs = _jobs()
+    # a budget that holds about two jobs at a time forces admission to wait for retirements
+    budget = 2 * max(j.footprint for j in jobs)
+    done, srv = _serve(jobs, budget_bytes=budget)
+    assert len(done) == len(jobs)
+    st = latency_stats(done)
+    assert 0 < st["p50"] <= st["p99"] <= st["max"]
+    assert all(j.admitted >= j.arrival - 1e-9 and j.finished >= j.admitted for j in done)
