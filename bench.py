@@ -1,17 +1,28 @@
 """Benchmark of the Hotspice stepping path on one MI355X.
 
-Default workload (the plain Neel configuration): in-plane pinwheel ASI 256x256 (32768
-magnets), full 511x511 dipolar kernels + 1 mT Zeeman field, Neel-Arrhenius, FP32 + Kahan
-fields with an FP64 recompute every 1024 switches. One benchmark step = 1024 Neel switch
-attempts + the FP64 recompute, i.e. one recompute period; `value` = switch attempts/s.
+Default workload (`--workload neel`, the plain Neel configuration): in-plane pinwheel ASI
+256x256 (32768 magnets), full 511x511 dipolar kernels + 1 mT Zeeman field, Neel-Arrhenius,
+FP32 + Kahan fields with an FP64 recompute every 1024 switches. One benchmark step = 1024 Neel
+switch attempts + the FP64 recompute, i.e. one recompute period; `value` = switch attempts/s.
 
-Prints ONE JSON line (see the keys below). `roofline` is for the dominant kernel (the persistent
-Neel step kernel, one launch per 1024 attempts): algorithmic bytes per launch (attempts per launch
-x neel_bytes_per_attempt) / average launch duration, measured with HIP events recorded on the
-launch stream around every launch in the timed region. `cpu_baseline` times the FP64 NumPy restatement of
+Other workloads (one JSON line each, same keys):
+  metro64     plain Metropolis: OOP square 64x64, full 127x127 kernel, grid r = 8a; step = 1 MC sweep
+  metro2048   plain Metropolis: OOP square 2048x2048, 41x41 window + exchange, grid r = 20a;
+              step = 1 MC sweep
+  batched     64 IP_Square instances (n 32..480, own T and field) stepped together by Metropolis;
+              step = 1 MC sweep of every instance; also times the same instances one by one
+              through the plain path (`vs_plain`)
+  neel_batched 256 pinwheel 32x32 instances, one persistent Neel launch; step = 1024 attempts
+              of every instance; `vs_plain` against the same instances stepped one by one
+  served      continuous batching of clocking jobs (Poisson arrivals, Kagome / pinwheel,
+              Neel / Metropolis); reports jobs/s and p50/p99 latency beside attempts/s
+
+Prints ONE JSON line. `roofline` is for the dominant kernel: algorithmic bytes per launch /
+average launch duration, measured with HIP events recorded on the launch stream around every
+launch in the timed region. `cpu_baseline` (plain workloads) times the FP64 NumPy restatement of
 the reference on one host core, on a bounded sample of the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
+    python bench.py [--workload W] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
 """
 import argparse
 import json
@@ -26,19 +37,63 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PERIOD = 1024
+METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
+WORKLOADS = ("neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=1000)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--workload", choices=WORKLOADS, default="neel")
+    p.add_argument("--steps", type=int, default=None)
+    p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--jobs", type=int, default=32, help="served: number of jobs")
+    p.add_argument("--field-steps", type=int, default=100, help="served: field steps per job")
+    p.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     return p.parse_args()
 
 
-def cpu_baseline(seconds):
+def cores_note():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return f"host has {cores} cores, {platform.processor() or platform.machine()}"
+
+
+def base_line(args, value, steps, warmup, wall, dtype, config, unit="switch attempts/s"):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / max(steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded random initial state and E_B)",
+            "config": config}
+
+
+def roofline(kernel, launches, bytes_total, extra=None):
+    us = [e0.elapsed_time(e1) * 1e3 for e0, e1, _ in launches]
+    avg = sum(us) / len(us)
+    per_launch = bytes_total / len(us)
+    achieved = per_launch / (avg * 1e-6) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "traffic": None, "kernel": kernel, "launches": len(us), "avg_launch_us": avg, "bytes_per_launch": per_launch}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def rel_field_error(mm):
+    """Largest |E_incremental - E_FP64| / max|E_FP64| after the steps since the last recompute."""
+    import numpy as np
+    E_inc = mm.E
+    mm._device_recompute()
+    E_ref = mm.E
+    return float(np.max(np.abs(E_inc - E_ref)) / np.max(np.abs(E_ref)))
+
+
+# ---- plain Neel (default) ---------------------------------------------------------------------
+
+def cpu_baseline_neel(seconds):
     """FP64 NumPy restatement of the reference (oracle/model.py) on the same workload, one core,
     with NumPy generators as in the reference (no replay needed for timing)."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
@@ -56,39 +111,26 @@ def cpu_baseline(seconds):
         om.update(t_max=math.inf)
     dt = time.perf_counter() - t0
     n = om.attempted_switches - a0
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count()
     return dict(value=n / dt, unit="switch attempts/s", cores=1, kind="port",
                 sample=f"{n} Neel steps of the same 256x256 pinwheel workload in {dt:.1f} s, FP64 NumPy, "
-                       f"1 thread (host has {cores} cores, {platform.processor() or platform.machine()})")
+                       f"1 thread ({cores_note()})")
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one MI355X).",
-              file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
+def bench_neel(args):
     import torch
-    import hotspice_amd as hs
     from hotspice_amd.workloads import neel_bytes_per_attempt, pinwheel_neel
-
+    steps = 1000 if args.steps is None else args.steps
+    warmup = 5 if args.warmup is None else args.warmup
     mm = pinwheel_neel(recompute_every=PERIOD)
     n = mm.n
     mc = PERIOD / n
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         mm.progress(t_max=math.inf, MCsteps_max=mc)
 
     # error of the incremental FP32 + Kahan field vs an FP64 recompute, at the end of one period
     mm._device_recompute()
     mm.progress(t_max=math.inf, MCsteps_max=(PERIOD - 1) / n)  # PERIOD-1 steps: no recompute yet
-    E_inc = mm.E
-    mm._device_recompute()
-    E_ref = mm.E
-    rel_err = float(np.max(np.abs(E_inc - E_ref)) / np.max(np.abs(E_ref)))
+    rel_err = rel_field_error(mm)
 
     # timed region: K benchmark steps of PERIOD attempts + FP64 recompute each; HIP events on the
     # launch stream bracket every launch of the persistent Neel kernel (k_neel_team) inside it
@@ -97,48 +139,244 @@ def main():
     a0, s0 = mm.attempted_switches, mm.switches
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         mm.progress(t_max=math.inf, MCsteps_max=mc)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     attempts = mm.attempted_switches - a0
-    switches = mm.switches - s0
-    value = attempts / wall
-    launches = dev.launch_events
-    dev.launch_events = None
-    launch_us = [e0.elapsed_time(e1) * 1e3 for e0, e1, _ in launches]
-    neel_us = sum(launch_us) / len(launch_us)
-    steps_per_launch = attempts / len(launches)
-
+    launches, dev.launch_events = dev.launch_events, None
     bytes_per = neel_bytes_per_attempt(n, n, perp=True)
-    bytes_launch = bytes_per * steps_per_launch
-    achieved = bytes_launch / (neel_us * 1e-6) / 1e9
-    line = {
-        "metric": "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat",
-        "value": value,
-        "unit": "switch attempts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+kahan (fields), f64 (recompute, time)",
-        "data": "synthetic (seeded random initial state and E_B)",
-        "config": {"workload": "plain Neel: IP_Pinwheel 256x256 (32768 magnets), full 511x511 dipolar kernel "
-                               "+ 1 mT Zeeman at 22.5 deg, T=300 K, E_B lognormal 60 kT/5%, FP64 recompute every "
-                               "1024 switches", "attempts_per_step": PERIOD, "attempts": attempts,
-                   "switches": switches, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_neel_team", "launches": len(launches), "avg_launch_us": neel_us,
-                     "attempts_per_launch": steps_per_launch, "bytes_per_attempt": bytes_per,
-                     "bytes_per_launch": bytes_launch},
-    }
+    line = base_line(args, attempts / wall, steps, warmup, wall, "f32+kahan (fields), f64 (recompute, time)",
+                     {"workload": "plain Neel: IP_Pinwheel 256x256 (32768 magnets), full 511x511 dipolar kernel "
+                                  "+ 1 mT Zeeman at 22.5 deg, T=300 K, E_B lognormal 60 kT/5%, FP64 recompute every "
+                                  "1024 switches", "attempts_per_step": PERIOD, "attempts": attempts,
+                      "switches": mm.switches - s0, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"})
+    line["roofline"] = roofline("k_neel_team", launches, bytes_per * attempts,
+                                {"attempts_per_launch": attempts / len(launches), "bytes_per_attempt": bytes_per})
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-    print(json.dumps(line), flush=True)
+        line["cpu_baseline"] = cpu_baseline_neel(args.cpu_seconds)
+    return line
+
+
+# ---- plain Metropolis -------------------------------------------------------------------------
+
+def metro_window_cells(mm, idx_list, kw):
+    """Mean number of occupied cells within the kernel window of at least one accepted switch,
+    per step, over the given accepted-index arrays (2, a)."""
+    import numpy as np
+    occ = np.asarray(mm.occupation).astype(bool)
+    tot = 0
+    for idx in idx_list:
+        mark = np.zeros(occ.shape, bool)
+        for y, x in zip(*np.asarray(idx).reshape(2, -1)):
+            mark[max(0, y - kw):y + kw + 1, max(0, x - kw):x + kw + 1] = True
+        tot += int((mark & occ).sum())
+    return tot / max(1, len(idx_list))
+
+
+def cpu_baseline_metro(make, kw, seconds, label):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import model
+    p = make()
+    om = model.make_magnets("OOP_Square", p["a"], n=p["n"], T=p["T"], E_B=p["E_B"], moment=p["moment"],
+                            pattern=p["pattern"], energies=p["energies"], params=p["params"], seed=1, rng="numpy")
+    om.update(**kw)
+    a0 = om.attempted_switches
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        om.update(**kw)
+    dt = time.perf_counter() - t0
+    n = om.attempted_switches - a0
+    return dict(value=n / dt, unit="switch attempts/s", cores=1, kind="port",
+                sample=f"{n} Metropolis attempts of {label} in {dt:.1f} s, FP64 NumPy, 1 thread ({cores_note()})")
+
+
+def bench_metro(args, large):
+    import numpy as np
+    import torch
+    from hotspice_amd.workloads import oop_large, oop_metropolis, oop_metropolis_inputs, kB
+    a = 200e-9
+    if large:
+        mm, kw, sweeps, warm = oop_large(), dict(r=20 * a), 4, 0.2
+        mm.recompute_every = max(1, int(16 * mm.n / 2850))  # every ~16 sweeps (about 2850 samples per step)
+        desc = ("plain Metropolis: OOP_Square 2048x2048 (4.19M magnets), a=200 nm, moment 1.1e-15, T=350 K, "
+                "E_B N(30 kT, 5%), dipolar cutoff 20a (41x41 window, REDUCED_KERNEL_SIZE=20) + exchange J=0.5 kT, "
+                "grid r=20a, FP32+Kahan, FP64 recompute every ~16 sweeps")
+        kwin = 20
+    else:
+        mm, kw, sweeps, warm = oop_metropolis(), dict(r=8 * a), 200, 1
+        desc = ("plain Metropolis: OOP_Square 64x64 (4096 magnets), a=200 nm, moment 1.1e-15, T=300 K, "
+                "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a, FP32+Kahan")
+        kwin = 10 ** 6
+    steps = sweeps if args.steps is None else args.steps
+    mm.progress(t_max=math.inf, MCsteps_max=warm, **kw)
+    dev = mm._dev
+    dev.launch_events = []
+    a0, s0 = mm.attempted_switches, mm.switches
+    step0 = mm._state().step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mm.progress(t_max=math.inf, MCsteps_max=1.0, **kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    attempts, accepted = mm.attempted_switches - a0, mm.switches - s0
+    launches, dev.launch_events = dev.launch_events, None
+    nsteps = mm._state().step - step0
+    # window cells of the accepted switches (N_T), sampled over 20 single steps after the timed region
+    sample = [np.asarray(mm.update(**kw)).reshape(2, -1) for _ in range(20)]
+    acc_s = sum(i.shape[1] for i in sample)
+    if large:
+        nt_per_acc = metro_window_cells(mm, sample, kwin) * len(sample) / max(1, acc_s)
+        bytes_total = attempts * 5 + accepted * nt_per_acc * 17
+    else:  # full kernel: a step with any accepted switch touches every magnet
+        frac = sum(i.shape[1] > 0 for i in sample) / len(sample)
+        nt_per_acc = mm.n * frac * len(sample) / max(1, acc_s)
+        bytes_total = attempts * 5 + nsteps * frac * mm.n * 17
+    err = rel_field_error(mm)
+    line = base_line(args, attempts / wall, steps, 0, wall, "f32+kahan (fields), f64 (acceptance)",
+                     {"workload": desc, "attempts": attempts, "accepted": accepted, "device_steps": nsteps,
+                      "rel_E_err_vs_fp64": err, "window_cells_per_accept": nt_per_acc, "parallelism": "single GPU"})
+    line["roofline"] = roofline("k_metro_select_one + k_metro_update_one (per progress call)", launches, bytes_total)
+    if not args.no_cpu:
+        if large:
+            line["cpu_baseline"] = None
+        else:
+            def make():
+                p = oop_metropolis_inputs()
+                from oracle import model
+                p.update(energies=[model.Dipolar(recompute="fft")],
+                         params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0))
+                return p
+            line["cpu_baseline"] = cpu_baseline_metro(make, kw, args.cpu_seconds, "the same 64x64 workload")
+    return line
+
+
+# ---- batched ----------------------------------------------------------------------------------
+
+def bench_batched(args):
+    import numpy as np
+    import torch
+    from hotspice_amd import batch
+    from hotspice_amd.workloads import ipsq_batch_instance, ipsq_batch_specs
+    sweeps = 10 if args.steps is None else args.steps
+    specs = ipsq_batch_specs()
+    mags = [ipsq_batch_instance(s) for s in specs]
+    batch.progress_many(mags, t_max=math.inf, MCsteps_max=0.5)
+    dev = mags[0]._dev
+    dev.launch_events = []
+    a0 = sum(m.attempted_switches for m in mags)
+    s0 = sum(m.switches for m in mags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        batch.progress_many(mags, t_max=math.inf, MCsteps_max=1.0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    attempts = sum(m.attempted_switches for m in mags) - a0
+    accepted = sum(m.switches for m in mags) - s0
+    launches, dev.launch_events = dev.launch_events, None
+    err = max(rel_field_error(m) for m in mags[::8])
+    # plain path: the same instances (fresh copies) stepped one at a time, one sweep each
+    plain = [ipsq_batch_instance(s) for s in specs]
+    for m in plain:
+        m.progress(t_max=math.inf, MCsteps_max=0.5)
+    torch.cuda.synchronize()
+    p0 = sum(m.attempted_switches for m in plain)
+    t1 = time.perf_counter()
+    for m in plain:
+        m.progress(t_max=math.inf, MCsteps_max=1.0)
+    torch.cuda.synchronize()
+    wall_plain = time.perf_counter() - t1
+    plain_rate = (sum(m.attempted_switches for m in plain) - p0) / wall_plain
+    # algorithmic bytes: k*(4+1) per attempt + 17 B per occupied window cell of each accepted
+    # switch (21x21 windows, half of the IP_Square cells occupied)
+    bytes_total = attempts * 5 + accepted * 21 * 21 * 0.5 * 17
+    line = base_line(args, attempts / wall, sweeps, 1, wall, "f32+kahan (fields), f64 (acceptance)",
+                     {"workload": "batched Metropolis: 64 IP_Square instances, n in {32..512} log-uniform (seed 3), "
+                                  "T~U(100,400) K, B~U(0,5) mT at random angle, a=1 um, moment 2.37e-16, dipolar cutoff "
+                                  "10 cells + REDUCED_KERNEL_SIZE=10, cutoff 0, grid multisampling Q=0.05, FP32+Kahan",
+                      "instances": len(mags), "magnets": int(sum(m.n for m in mags)), "attempts": attempts,
+                      "accepted": accepted, "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
+                      "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
+    line["roofline"] = roofline("k_metro_select_many + k_metro_update_many (per batched call)", launches, bytes_total)
+    return line
+
+
+def bench_neel_batched(args):
+    import torch
+    from hotspice_amd import batch
+    from hotspice_amd.workloads import neel_bytes_per_attempt, pinwheel_neel
+    steps = 20 if args.steps is None else args.steps
+    count = 256
+    mags = [pinwheel_neel(n=32, seed=100 + i, recompute_every=PERIOD) for i in range(count)]
+    n = mags[0].n
+    batch.progress_many(mags, t_max=math.inf, MCsteps_max=PERIOD / n)
+    dev = mags[0]._dev
+    dev.launch_events = []
+    a0 = sum(m.attempted_switches for m in mags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.progress_many(mags, t_max=math.inf, MCsteps_max=PERIOD / n)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    attempts = sum(m.attempted_switches for m in mags) - a0
+    launches, dev.launch_events = dev.launch_events, None
+    err = max(rel_field_error(m) for m in mags[::32])
+    one = mags[0]
+    torch.cuda.synchronize()
+    p0 = one.attempted_switches
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        one.progress(t_max=math.inf, MCsteps_max=PERIOD / n)
+    torch.cuda.synchronize()
+    plain_rate = (one.attempted_switches - p0) / (time.perf_counter() - t1)
+    bytes_total = neel_bytes_per_attempt(n, n, perp=True) * attempts
+    line = base_line(args, attempts / wall, steps, 1, wall, "f32+kahan (fields), f64 (recompute, time)",
+                     {"workload": "batched Neel: 256 IP_Pinwheel 32x32 instances (512 magnets each), parameters of "
+                                  "the 256x256 pinwheel workload, one persistent launch, FP64 recompute every 1024 "
+                                  "switches per instance", "instances": count, "attempts": attempts,
+                      "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
+                      "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
+    line["roofline"] = roofline("k_neel_team (256 teams of 1 workgroup)", launches, bytes_total)
+    return line
+
+
+def bench_served(args):
+    import torch
+    from hotspice_amd.serve import Server, latency_stats
+    from hotspice_amd.workloads import served_jobs
+    jobs = served_jobs(count=args.jobs, rate=args.rate, field_steps=args.field_steps, cutoff_cells=10)
+    srv = Server()
+    for j in jobs:
+        srv.submit(j)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = srv.run(timeout=900)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = latency_stats(done)
+    line = base_line(args, srv.attempts / wall, srv.iterations, 0, wall, "f32+kahan (fields), f64 (recompute, time)",
+                     {"workload": f"served: {args.jobs} clocking jobs, Poisson arrivals at {args.rate:g}/s, 50% IP_Kagome "
+                                  f"/ 50% IP_Pinwheel, n log-uniform 64..1024, 50% Neel (1000 attempts per field step) / "
+                                  f"50% Metropolis (4 MC sweeps per field step), {args.field_steps} field steps "
+                                  "(1.5 mT (1-cos), 0/45 deg), dipolar window 21x21 (cutoff 10 cells)",
+                      "jobs": len(done), "jobs_per_s": len(done) / wall, "latency_s": st,
+                      "attempts": srv.attempts, "parallelism": "single GPU"})
+    line["roofline"] = None
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one MI355X).",
+              file=sys.stderr)
+        sys.exit(2)
+    fn = {"neel": bench_neel, "metro64": lambda a: bench_metro(a, False), "metro2048": lambda a: bench_metro(a, True),
+          "batched": bench_batched, "neel_batched": bench_neel_batched, "served": bench_served}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

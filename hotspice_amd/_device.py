@@ -178,15 +178,25 @@ class DeviceState:
             raise _lib.HotspiceError("bad team descriptors")
         if self.team_ws is None or self.team_ws.numel() < need:
             self.team_ws = self.torch.zeros(max(need, 256), dtype=self.torch.uint8, device=self.device)
-        ev = self.launch_events
-        if ev is not None:  # timing hook: HIP events around the launch, on the launch stream
-            s = self.torch.cuda.current_stream(self.device)
-            e0, e1 = self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)
-            e0.record(s)
+        ev = self._event_start()
         _lib.check(run_fn(arr, len(teams), self.ptr(self.team_ws), self.team_ws.numel(), self.stream()))
-        if ev is not None:
-            e1.record(s)
-            ev.append((e0, e1, [t.state for t in teams]))
+        self._event_end(ev, [t.state for t in teams])
+
+    def _event_start(self):
+        """Timing hook: when launch_events is a list, HIP events on the launch stream bracket the
+        library call and (start, end, state pointers) is appended to it."""
+        if self.launch_events is None:
+            return None
+        e0 = self.torch.cuda.Event(enable_timing=True)
+        e0.record(self.torch.cuda.current_stream(self.device))
+        return e0
+
+    def _event_end(self, e0, states):
+        if e0 is None:
+            return
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e1.record(self.torch.cuda.current_stream(self.device))
+        self.launch_events.append((e0, e1, states))
 
     def metropolis_run(self, sel: _lib.Select, run: _lib.Run, log_cells=None, log_count=None):
         need = int(self.lib.hs_metropolis_ws_bytes(C.byref(self.inst), C.byref(sel)))
@@ -194,9 +204,11 @@ class DeviceState:
             raise _lib.HotspiceError("bad selection geometry")
         if self.metro_ws is None or self.metro_ws.numel() < need:
             self.metro_ws = self.torch.zeros(need, dtype=self.torch.uint8, device=self.device)
+        ev = self._event_start()
         _lib.check(self.lib.hs_metropolis_run(C.byref(self.inst), C.byref(sel), C.byref(run), self.ptr(self.state),
                                               self.ptr(self.metro_ws), self.ptr(log_cells), self.ptr(log_count),
                                               self.stream()))
+        self._event_end(ev, [self.state.data_ptr()])
 
     def philox_uniform53(self, seed, step, tag, index0, n, open_zero=False):
         out = self.torch.empty(max(1, n), dtype=self.torch.float64, device=self.device)

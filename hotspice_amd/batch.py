@@ -100,15 +100,18 @@ def launch_metropolis(items, attempt_freq=1e10):
                        log=0, nblocks=0, begin=0)
         teams.append(_lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=_lib.C.addressof(sel), run=run,
                                state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None, nblocks=0))
+    before = [it[0]._state().attempts for it in items]
     dev.run_teams(teams, "metropolis")
-    _after_launch([it[0] for it in items], "Metropolis")
+    _after_launch([it[0] for it in items], "Metropolis", before)
 
 
-def _after_launch(mags, what):
-    for m in mags:
+def _after_launch(mags, what, before=None):
+    for i, m in enumerate(mags):
         st = m._state()
         if st.err:
             raise _lib.HotspiceError(f"device {what} step failed (err={st.err})")
+        if before is not None and st.steps_done > 0:  # samples per step, to size the next launch
+            m._metro_rate = max(1.0, (st.attempts - before[i]) / st.steps_done)
         m._since_recompute += int(st.steps_done)
         m._maybe_recompute()
 
@@ -133,7 +136,7 @@ def neel_chunk(m, start, t_max, mc, chunk=None):
 def metropolis_chunk(m, sel, start, mc, chunk=None):
     """Steps of the next Metropolis launch (as Magnets.progress chunks it)."""
     st = m._state()
-    per_step = max(1.0, 0.25 * sel.nsc)
+    per_step = m._metro_rate or max(1.0, 0.25 * sel.nsc)
     if math.isfinite(mc):
         steps = int(math.ceil((mc * m.n - (st.attempts - start[1])) / per_step)) + 1
         steps = max(1, min(steps, 1 << 16))

@@ -112,6 +112,7 @@ class Magnets(ABC):
         self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
         self.params = SimParams() if params is None else params
         self.recompute_every = recompute_every
+        self._metro_rate = None  # measured Metropolis samples per step (sizes launches only)
         self._energies = list()
         self.in_plane = in_plane
         self.nx, self.ny = int(nx), int(ny)
@@ -876,8 +877,8 @@ class Magnets(ABC):
             if kwargs.get('idx') is not None:
                 raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
             sel = self._select_geometry(self.calc_r(Q) if r == 0 else r)
-            per_step = max(1.0, 0.25 * sel.nsc)
             while True:
+                per_step = self._metro_rate or max(1.0, 0.25 * sel.nsc)
                 st = self._state()
                 if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
                     break
@@ -892,11 +893,14 @@ class Magnets(ABC):
                                max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
                 if begin:
                     self._set_progress_origin(t0, a0)
+                a_before = st.attempts
                 self._dev.metropolis_run(sel, run)
                 begin = False
                 st = self._state()
                 if st.err:
                     raise _lib.HotspiceError(f"device Metropolis step failed (err={st.err})")
+                if st.steps_done > 0:  # samples per step, to size the next launch
+                    self._metro_rate = max(1.0, (st.attempts - a_before) / st.steps_done)
                 self._since_recompute += int(st.steps_done)
                 self._maybe_recompute()
         self._after_device_step()

@@ -10,13 +10,25 @@
                   127x127 kernel, Metropolis with grid multisampling r = 8a, random +-1 start.
   oop_large       Out-of-plane square ASI 2048x2048 (4.19M magnets), a = 200 nm, moment 1.1e-15,
                   E_B normal (30 kT, 5 %), T = 350 K, dipolar with cutoff 20a (41x41 window,
-                  REDUCED_KERNEL_SIZE = 20) + exchange J = 0.5 kT, Metropolis grid r = 20a.
+                  REDUCED_KERNEL_SIZE = 20; see edge_cutoff) + exchange J = 0.5 kT, Metropolis
+                  grid r = 20a.
 """
 import math
 
 import numpy as np
 
 kB = 1.380649e-23
+
+
+def edge_cutoff(r):
+    """A cutoff_range of r lattice-pitch multiples, widened by 1e-9 relative. The kernel's
+    distances are FP64 cumulative sums of the cell pitches, so an offset of exactly r can land a
+    few ulp above r on one side of the centre and below it on the other: the kernel then keeps
+    the -r entry and drops the +r one. The reference's full recompute correlates with K (source -
+    target) while its incremental updates use K (target - source), so such a kernel makes the two
+    disagree by that entry. Widening the cutoff keeps both entries (the next lattice distance is
+    far above 1e-9 relative)."""
+    return r * (1 + 1e-9)
 
 
 def pinwheel_neel_inputs(n=256, seed=0):
@@ -63,7 +75,7 @@ def oop_large(n=2048, seed=0):
     shape = (n, n)
     return hs.ASI.OOP_Square(a, n, T=T, E_B=30 * kT * rng.normal(1, 0.05, size=shape), moment=1.1e-15,
                              pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0), seed=seed,
-                             energies=[hs.DipolarEnergy(cutoff_range=20 * a), hs.ExchangeEnergy(J=0.5 * kT)],
+                             energies=[hs.DipolarEnergy(cutoff_range=edge_cutoff(20 * a)), hs.ExchangeEnergy(J=0.5 * kT)],
                              params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=20))
 
 
@@ -100,7 +112,8 @@ def ipsq_batch_instance(spec, a=1e-6):
     kT = kB * spec["T"]
     mm = hs.ASI.IP_Square(a, n, T=spec["T"], E_B=20 * kT * rng.normal(1, 0.05, size=(n, n)), moment=2.37e-16,
                           pattern=np.where(rng.random((n, n)) < 0.5, -1.0, 1.0), seed=spec["seed"],
-                          energies=[hs.DipolarEnergy(cutoff_range=10 * a / 2), hs.ZeemanEnergy(spec["B"], spec["theta"])],
+                          energies=[hs.DipolarEnergy(cutoff_range=edge_cutoff(10 * a / 2)),
+                                    hs.ZeemanEnergy(spec["B"], spec["theta"])],
                           params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=10,
                                               SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0))
     return mm
@@ -114,12 +127,13 @@ def clocking_fields(steps=100, peak=3e-3):
 
 
 def served_jobs(count=32, rate=200.0, seed=4, n_range=(64, 1024), field_steps=100, neel_work=1000,
-                metro_sweeps=4.0):
+                metro_sweeps=4.0, cutoff_cells=None):
     """Jobs of the served workload: Poisson arrivals at `rate` jobs/s; 50 % IP_Kagome / 50 %
     IP_Pinwheel with n log-uniform in n_range; 50 % Neel (`neel_work` switch attempts per field
     step) / 50 % Metropolis (`metro_sweeps` MC sweeps per field step); `field_steps` clocking
     field steps; other parameters as the pinwheel workload (a = 1 um, moment 2.37e-16, T = 300 K,
-    E_B lognormal median 60 kT / 5 %, full dipolar kernels)."""
+    E_B lognormal median 60 kT / 5 %). Dipolar kernels are full, or cut off at `cutoff_cells` cell
+    pitches with the matching REDUCED_KERNEL_SIZE."""
     from .serve import Job, footprint_bytes
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -138,15 +152,20 @@ def served_jobs(count=32, rate=200.0, seed=4, n_range=(64, 1024), field_steps=10
             kT = kB * 300
             cls = getattr(hs.ASI, lattice)
             shape = _lattice_shape(lattice, n)
+            dx = 1e-6 / 4 if lattice == "IP_Kagome" else 1e-6 / 2
+            dip = hs.DipolarEnergy() if cutoff_cells is None else \
+                hs.DipolarEnergy(cutoff_range=edge_cutoff(cutoff_cells * dx))
+            params = hs.SimParams(UPDATE_SCHEME=scheme) if cutoff_cells is None else \
+                hs.SimParams(UPDATE_SCHEME=scheme, REDUCED_KERNEL_SIZE=cutoff_cells)
             return cls(1e-6, n, T=300, E_B=60 * kT * np.exp(0.05 * r.normal(0, 1, size=shape)), moment=2.37e-16,
                        pattern=np.where(r.random(shape) < 0.5, -1.0, 1.0), seed=jseed,
-                       energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(0.0, 0.0)],
-                       params=hs.SimParams(UPDATE_SCHEME=scheme))
+                       energies=[dip, hs.ZeemanEnergy(0.0, 0.0)], params=params)
         nx, ny = _lattice_shape(lattice, n)[::-1]
         n_sub = 6 if lattice == "IP_Kagome" else 2
         work = float(neel_work) if scheme == "Néel" else float(metro_sweeps)
-        jobs.append(Job(make=make, fields=fields, work=work, arrival=t, footprint=footprint_bytes(nx, ny, n_sub),
-                        name=f"{lattice} n={n} {scheme}"))
+        fp = footprint_bytes(nx, ny, n_sub) if cutoff_cells is None else \
+            footprint_bytes(nx, ny, n_sub, kernel_full=False, kwx=cutoff_cells, kwy=cutoff_cells)
+        jobs.append(Job(make=make, fields=fields, work=work, arrival=t, footprint=fp, name=f"{lattice} n={n} {scheme}"))
     return jobs
 
 

@@ -174,3 +174,31 @@ def test_incremental_equals_full_recompute(lat):
     np.testing.assert_allclose(E_inc, mm.E, rtol=0, atol=1e-12 * scale)
     if Ep_inc is not None:
         np.testing.assert_allclose(Ep_inc, mm.E_perp, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("widen,consistent", [(1.0, False), (1 + 1e-9, True)])
+def test_cutoff_at_a_lattice_distance(widen, consistent):
+    """A cutoff_range equal to a lattice distance (10 pitches on IP_Square 22x22): the FP64
+    cumulative-sum offsets put +10 pitches a few ulp above the cutoff and -10 below it, so the
+    kernel keeps only one of the two entries. The full recompute (correlation with K, as
+    energies.py:303) and the incremental updates (K indexed by target - source, as
+    energies.py:320 and 350) then differ by that entry; a cutoff widened by 1e-9 keeps both and
+    the two agree again. This is the reason the cutoff workloads widen their cutoffs."""
+    g = lattices.make("IP_Square", 1e-6, n=22)
+    rng = np.random.default_rng(5)
+    pat = np.where(rng.random((g.ny, g.nx)) < .5, -1., 1.)
+    mm = model.OracleMagnets(g, T=300, moment=2.37e-16, pattern=pat,
+                             energies=[model.Dipolar(cutoff_range=10 * 0.5e-6 * widen)],
+                             params=dict(REDUCED_KERNEL_SIZE=10, SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0))
+    occ = np.argwhere(g.occupation)
+    for it in range(10):
+        sel = occ[rng.choice(len(occ), 6, replace=False)].T
+        mm.m[sel[0], sel[1]] *= -1
+        mm.update_energy(sel)
+    E_inc = mm.E.copy()
+    mm.update_energy()
+    err = np.abs(E_inc - mm.E).max() / np.abs(mm.E).max()
+    if consistent:
+        assert err < 1e-12
+    else:
+        assert err > 1e-5
