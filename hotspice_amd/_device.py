@@ -59,6 +59,7 @@ class DeviceState:
     # ---- plumbing ----------------------------------------------------------------------
     # set to a list to collect (start event, end event, state pointers) around every team launch
     launch_events = None
+    many_ws = None
 
     def stream(self):
         return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
@@ -129,6 +130,31 @@ class DeviceState:
             self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
         _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.ptr(self.rec_scratch),
                                                 self.rec_scratch.numel(), self.stream()))
+
+    def recompute_scratch(self):
+        """The recompute scratch tensor of this instance, sized for it."""
+        need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
+        if need < 0:
+            raise _lib.HotspiceError("bad instance for recompute")
+        if self.rec_scratch is None or self.rec_scratch.numel() < need:
+            self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
+        return self.rec_scratch
+
+    def recompute_many(self, devs, which):
+        """hs_recompute_fields for several instances (DeviceStates on this device) in one call."""
+        n = len(devs)
+        insts = (C.c_void_p * n)(*[C.addressof(d.inst) for d in devs])
+        wh = (C.c_int32 * n)(*which)
+        scr = [d.recompute_scratch() for d in devs]
+        ptrs = (C.c_void_p * n)(*[t.data_ptr() for t in scr])
+        sizes = (C.c_int64 * n)(*[t.numel() for t in scr])
+        need = int(self.lib.hs_recompute_many_ws_bytes(insts, wh, n))
+        if need < 0:
+            raise _lib.HotspiceError("bad instances for recompute")
+        if self.many_ws is None or self.many_ws.numel() < need:
+            self.many_ws = self.torch.empty(max(need, 256), dtype=self.torch.uint8, device=self.device)
+        _lib.check(self.lib.hs_recompute_fields_many(insts, wh, ptrs, sizes, n, self.ptr(self.many_ws),
+                                                     self.many_ws.numel(), self.stream()))
 
     def components(self):
         out = self.torch.empty((6, self.ny, self.nx), dtype=self.torch.float64, device=self.device)

@@ -296,12 +296,11 @@ HS_INLINE void coarse_range(int dy0, int u, int kw, int &lo, int &hi) {
   hi = div_floor(kw - dy0, u);
 }
 
-__global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan plan, const double *Kbase,
-                                                       double *part) {
+__device__ __forceinline__ void rec_dip(const hs_instance_t &I, const RecPlan &plan, const double *Kbase,
+                                        double *part, int tile, int p, int g) {
   __shared__ double sK[KBR * KBC];
   __shared__ double sS[RA * RB];
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x, p = blockIdx.y, g = blockIdx.z;
   const int a0 = (tile / plan.tB) * RA, b0 = (tile % plan.tB) * RB;
   const int py = plan.sites_y[p], px = plan.sites_x[p];
   const int KW = 2 * I.kwx + 1;
@@ -388,9 +387,31 @@ __global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan 
   }
 }
 
-__global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, const double *partP, int which) {
+__global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan plan, const double *Kbase,
+                                                       double *part) {
+  rec_dip(I, plan, Kbase, part, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Several recomputes in one launch: block b works for job block_job[b] (its blocks in the order
+// tile, then sublattice p, then source group g).
+struct RecJob {
+  hs_instance_t I;
+  RecPlan plan;
+  const double *K;
+  double *part;
+  int32_t first;
+};
+
+__global__ void __launch_bounds__(256) k_recompute_dip_many(const RecJob *__restrict__ jobs,
+                                                            const int32_t *__restrict__ block_job) {
+  const RecJob &J = jobs[block_job[blockIdx.x]];
+  const int lb = blockIdx.x - J.first, nt = J.plan.tA * J.plan.tB;
+  rec_dip(J.I, J.plan, J.K, J.part, lb % nt, (lb / nt) % J.I.n_sub, lb / (nt * J.I.n_sub));
+}
+
+__device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const double *partE, const double *partP,
+                                           int which, int c) {
   const int ncell = I.nx * I.ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int y = c / I.nx, x = c % I.nx, site = hs_site(I, y, x);
   const int n = I.dip_pf != 0.0 ? I.sub_of[site] : -1;
@@ -430,6 +451,21 @@ __global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, 
     I.Ep[c] = h;
     I.Epc[c] = (float)((double)h - ep);
   }
+}
+
+__global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, const double *partP, int which) {
+  rec_finish(I, F, partE, partP, which, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+struct FinJob {
+  hs_instance_t I;
+  const double *partE, *partP;
+  int32_t F, which, first;
+};
+
+__global__ void k_recompute_finish_many(const FinJob *__restrict__ jobs, const int32_t *__restrict__ block_job) {
+  const FinJob &J = jobs[block_job[blockIdx.x]];
+  rec_finish(J.I, J.F, J.partE, J.partP, J.which, (blockIdx.x - J.first) * blockDim.x + threadIdx.x);
 }
 
 static RecPlan make_plan(const hs_instance_t *I) {
@@ -492,6 +528,126 @@ extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, voi
   }
   k_recompute_finish<<<(ncell + 255) / 256, 256, 0, s>>>(*inst, dip ? pl.F : 0, partE, partP, which);
   HS_LAUNCH_CHECK("k_recompute_finish");
+  return HS_OK;
+}
+
+namespace {
+
+struct ManyLayout {
+  size_t dip_off, fin_off, dmap_off, fmap_off, total;
+  int ndip, dip_blocks, fin_blocks;
+};
+
+size_t al256c(size_t v) { return (v + 255) & ~(size_t)255; }
+
+ManyLayout many_layout(const hs_instance_t *const *insts, const int32_t *which, int n) {
+  ManyLayout L{};
+  for (int i = 0; i < n; ++i) {
+    const hs_instance_t &I = *insts[i];
+    const RecPlan pl = make_plan(&I);
+    if (I.dip_pf != 0.0) {
+      const int per = pl.tA * pl.tB * I.n_sub * pl.F;
+      if (which[i] & 1) {
+        ++L.ndip;
+        L.dip_blocks += per;
+      }
+      if ((which[i] & 2) && I.use_perp) {
+        ++L.ndip;
+        L.dip_blocks += per;
+      }
+    }
+    L.fin_blocks += (I.nx * I.ny + 255) / 256;
+  }
+  L.dip_off = 0;
+  L.fin_off = al256c(sizeof(RecJob) * (size_t)(L.ndip > 0 ? L.ndip : 1));
+  L.dmap_off = L.fin_off + al256c(sizeof(FinJob) * (size_t)n);
+  L.fmap_off = L.dmap_off + al256c(4 * (size_t)(L.dip_blocks > 0 ? L.dip_blocks : 1));
+  L.total = L.fmap_off + al256c(4 * (size_t)L.fin_blocks);
+  return L;
+}
+
+}  // namespace
+
+extern "C" int64_t hs_recompute_many_ws_bytes(const hs_instance_t *const *insts, const int32_t *which, int32_t n) {
+  if (!insts || !which || n <= 0) return -1;
+  for (int i = 0; i < n; ++i)
+    if (!insts[i] || insts[i]->nx <= 0 || insts[i]->ny <= 0 || insts[i]->ux <= 0 || insts[i]->uy <= 0) return -1;
+  return (int64_t)many_layout(insts, which, n).total;
+}
+
+extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const int32_t *which, void *const *scratch,
+                                        const int64_t *scratch_bytes, int32_t n, void *ws, int64_t ws_bytes,
+                                        void *stream) {
+  hs_clear_error();
+  if (!insts || !which || !scratch || !scratch_bytes || n <= 0 || !ws) return hs_set_error(HS_ERR_ARG, "missing arrays");
+  for (int i = 0; i < n; ++i) {
+    int rc = hs_check_instance(insts[i]);
+    if (rc) return rc;
+    const hs_instance_t &I = *insts[i];
+    if (I.dip_pf != 0.0) {
+      const int64_t need = hs_recompute_scratch_bytes(&I);
+      if (!scratch[i] || scratch_bytes[i] < need)
+        return hs_set_error(HS_ERR_ARG, "instance %d: recompute scratch of %lld bytes is below the %lld needed", i,
+                            (long long)scratch_bytes[i], (long long)need);
+      for (int k = 0; k < I.n_sub; ++k) {
+        bool found = false;
+        for (int q = 0; q < I.ux * I.uy; ++q) found |= I.sub_of[q] == k;
+        if (!found) return hs_set_error(HS_ERR_ARG, "instance %d: kernel %d has no unit-cell site", i, k);
+      }
+    }
+  }
+  const ManyLayout L = many_layout(insts, which, n);
+  if (ws_bytes < (int64_t)L.total)
+    return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)L.total);
+  std::string host(L.total, '\0');
+  RecJob *dj = (RecJob *)&host[L.dip_off];
+  FinJob *fj = (FinJob *)&host[L.fin_off];
+  int32_t *dmap = (int32_t *)&host[L.dmap_off];
+  int32_t *fmap = (int32_t *)&host[L.fmap_off];
+  int nd = 0, db = 0, fb = 0;
+  for (int i = 0; i < n; ++i) {
+    const hs_instance_t &I = *insts[i];
+    const RecPlan pl = make_plan(&I);
+    const int ncell = I.nx * I.ny;
+    const bool dip = I.dip_pf != 0.0;
+    double *partE = dip ? (double *)scratch[i] : nullptr;
+    double *partP = dip ? partE + (size_t)pl.F * ncell : nullptr;
+    if (dip) {
+      const int per = pl.tA * pl.tB * I.n_sub * pl.F;
+      for (int w = 1; w <= 2; w <<= 1) {
+        if (!(which[i] & w) || (w == 2 && !I.use_perp)) continue;
+        RecJob &J = dj[nd];
+        J.I = I;
+        J.plan = pl;
+        J.K = w == 1 ? I.K64 : I.Kps64;
+        J.part = w == 1 ? partE : partP;
+        J.first = db;
+        for (int b = 0; b < per; ++b) dmap[db + b] = nd;
+        db += per;
+        ++nd;
+      }
+    }
+    FinJob &F = fj[i];
+    F.I = I;
+    F.partE = partE;
+    F.partP = partP;
+    F.F = dip ? pl.F : 0;
+    F.which = which[i];
+    F.first = fb;
+    const int nb = (ncell + 255) / 256;
+    for (int b = 0; b < nb; ++b) fmap[fb + b] = i;
+    fb += nb;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char *base = (char *)ws;
+  HS_HIP_TRY(hipMemcpyAsync(base, host.data(), L.total, hipMemcpyHostToDevice, s));
+  HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  if (db > 0) {
+    k_recompute_dip_many<<<db, 256, 0, s>>>((const RecJob *)(base + L.dip_off), (const int32_t *)(base + L.dmap_off));
+    HS_LAUNCH_CHECK("k_recompute_dip_many");
+  }
+  k_recompute_finish_many<<<fb, 256, 0, s>>>((const FinJob *)(base + L.fin_off), (const int32_t *)(base + L.fmap_off));
+  HS_LAUNCH_CHECK("k_recompute_finish_many");
   return HS_OK;
 }
 

@@ -86,8 +86,8 @@ class _Active:
         self.start = (st.t, st.attempts)
         self.mm._set_progress_origin(*self.start)
 
-    def step_done(self):
-        st = self.mm._state()
+    def step_done(self, st=None):
+        st = self.mm._state() if st is None else st
         if self.scheme == Scheme.NEEL:
             return st.attempts - self.start[1] >= int(self.job.work)
         return not lower_than(st.attempts / self.mm.n - self.start[1] / self.mm.n, self.job.work)
@@ -148,38 +148,37 @@ class Server:
             order = neel[self._rr % len(neel):] + neel[:self._rr % len(neel)]
             self._rr += 1
             items, used = [], 0
-            for a in order:
+            for a, st in zip(order, batch.read_states([a.mm for a in order])):
                 g = batch.neel_team_size(a.mm)
                 if items and used + g > self.max_wg:
                     continue
-                st = a.mm._state()
                 left = int(a.job.work) - (st.attempts - a.start[1])
                 steps = min(left, self.neel_chunk)
                 if a.mm.recompute_every:
                     steps = min(steps, max(1, a.mm.recompute_every - a.mm._since_recompute))
                 items.append((a.mm, steps, math.inf, math.inf, 0))
                 used += g
-            before = sum(it[0]._state().attempts for it in items)
+                self.attempts -= st.attempts
             batch.launch_neel(items, 1e10)
-            self.attempts += sum(it[0]._state().attempts for it in items) - before
+            self.attempts += sum(st.attempts for st in batch.read_states([it[0] for it in items]))
         if metro:
             items = []
-            before = 0
-            for a in metro:
-                steps = batch.metropolis_chunk(a.mm, a.sel, a.start, a.job.work, self.metro_chunk)
+            for a, st in zip(metro, batch.read_states([a.mm for a in metro])):
+                steps = batch.metropolis_chunk(a.mm, a.sel, a.start, a.job.work, self.metro_chunk, st)
                 items.append((a.mm, a.sel, steps, math.inf, a.job.work))
-                before += a.mm._state().attempts
-            batch.launch_metropolis(items, a.job.kwargs.get('attempt_freq', 1e10))
-            self.attempts += sum(a.mm._state().attempts for a in metro) - before
+                self.attempts -= st.attempts
+            batch.launch_metropolis(items, metro[0].job.kwargs.get('attempt_freq', 1e10))
+            self.attempts += sum(st.attempts for st in batch.read_states([a.mm for a in metro]))
 
     def _retire(self, now):
         keep = []
-        for a in self.active:
+        for a, st in zip(self.active, batch.read_states([a.mm for a in self.active])):
             finished = False
-            while a.step_done():
+            while a.step_done(st):
                 if a.advance():
                     finished = True
                     break
+                st = a.mm._state()
             if finished:
                 a.mm._after_device_step()
                 a.job.finished = now

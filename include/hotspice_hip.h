@@ -212,6 +212,17 @@ int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *scratch,
 int64_t hs_recompute_scratch_bytes(const hs_instance_t *inst);
 
 /*
+ * hs_recompute_fields for n instances at once (one launch for all dipolar correlations, one for
+ * all finishing passes): insts[i] (HOST pointers to the descriptors), which[i] as above,
+ * scratch[i] / scratch_bytes[i] the recompute scratch of instance i. ws: device scratch of
+ * hs_recompute_many_ws_bytes bytes for the descriptors. Same results as n single calls.
+ * Blocking for the small descriptor upload, otherwise asynchronous.
+ */
+int hs_recompute_fields_many(const hs_instance_t *const *insts, const int32_t *which, void *const *scratch,
+                             const int64_t *scratch_bytes, int32_t n, void *ws, int64_t ws_bytes, void *stream);
+int64_t hs_recompute_many_ws_bytes(const hs_instance_t *const *insts, const int32_t *which, int32_t n);
+
+/*
  * Per-energy FP64 fields for the host API (Energy.E / Energy.E_perp): out[0..3] planes =
  * Zeeman E, Zeeman E_perp, exchange E, exchange E_perp (closed forms, energies.py:106-118,
  * 771-781) and out[4], out[5] = combined E and E_perp (value - compensation). Dipolar fields

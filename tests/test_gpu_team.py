@@ -102,3 +102,23 @@ def test_metropolis_batch_equals_alone():
     for m1, m2, r in zip(alone, mixed, res):
         _same(_snapshot(m1), _snapshot(m2))
         assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
+
+
+def test_recompute_many_equals_single():
+    """hs_recompute_fields_many gives every instance exactly the fields of its own
+    hs_recompute_fields call."""
+    mk = [lambda: _pinwheel(24, 5), lambda: _oop(20, 6), lambda: _pinwheel(40, 7)]
+    a = [f() for f in mk]
+    b = [f() for f in mk]
+    for x, y in zip(a, b):
+        x.progress(t_max=math.inf, MCsteps_max=0.3)
+        y.progress(t_max=math.inf, MCsteps_max=0.3)
+    for x in a:
+        x._dev.recompute(3)
+    b[0]._dev.recompute_many([y._dev for y in b], [3, 3, 1])
+    for x, y, w in zip(a, b, [3, 3, 1]):
+        np.testing.assert_array_equal(x._dev.E.cpu().numpy(), y._dev.E.cpu().numpy())
+        np.testing.assert_array_equal(x._dev.Ec.cpu().numpy(), y._dev.Ec.cpu().numpy())
+        if w == 3:
+            np.testing.assert_array_equal(x._dev.Ep.cpu().numpy(), y._dev.Ep.cpu().numpy())
+            np.testing.assert_array_equal(x._dev.Epc.cpu().numpy(), y._dev.Epc.cpu().numpy())
