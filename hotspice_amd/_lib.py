@@ -72,7 +72,7 @@ class Select(C.Structure):
     _fields_ = [("pat_off", C.c_void_p), ("pat_cells", C.c_void_p), ("Rx", C.c_int32), ("Ry", C.c_int32),
                 ("snx", C.c_int32), ("sny", C.c_int32), ("ncols", C.c_int32), ("nsc", C.c_int32),
                 ("scheme", C.c_int32), ("cutoff", C.c_int32), ("reduced_k", C.c_int32),
-                ("metropolis_time", C.c_int32)]
+                ("metropolis_time", C.c_int32), ("r_x", C.c_double), ("r_y", C.c_double)]
 
 
 class Team(C.Structure):

@@ -782,7 +782,7 @@ class Magnets(ABC):
         """Supergrid of _select_grid (core.py:505-540) for distance r: sizes and the occupied-cell
         pattern of one supercell for every unit-cell offset, uploaded once per (r, occupation)."""
         scheme = self.params.MULTISAMPLING_SCHEME
-        if scheme not in ('grid', 'single'):
+        if scheme not in ('grid', 'single', 'Poisson'):
             raise NotImplementedError(f"MULTISAMPLING_SCHEME='{scheme}' has no device implementation.")
         key = (float(r), scheme, self.params.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF,
                self.params.REDUCED_KERNEL_SIZE, bool(self.params.METROPOLIS_TIME))
@@ -796,6 +796,16 @@ class Magnets(ABC):
         if scheme == 'single':
             sel.scheme = 1
             sel.nsc = 1
+            self._sel_tensors = ()
+        elif scheme == 'Poisson':
+            # minimal distance r in cells along each axis (at least 1.5 cells, so snapped samples
+            # stay distinct); background cells of r/sqrt(2) hold at most one sample each
+            sel.scheme = 2
+            sel.r_x = max(1.5, r / float(np.min(self.dx)))
+            sel.r_y = max(1.5, r / float(np.min(self.dy)))
+            sel.snx = int(self.nx / (sel.r_x * 0.70710678)) + 1  # one spare column/row: covers the grid
+            sel.sny = int(self.ny / (sel.r_y * 0.70710678)) + 1  # whatever the FP32 cell size rounds to
+            sel.nsc = sel.snx * sel.sny
             self._sel_tensors = ()
         else:
             sel.scheme = 0

@@ -40,6 +40,10 @@ struct MetroWS {
   int32_t *acc_n;    // [1]  number accepted this step
   int32_t *trunc;    // [HS_MAX_UC] per unit-cell site: 1 = truncated window (convolution branch)
   int32_t *info;     // [4] grid offset ox, oy of the selection that succeeded, 1 = single sample
+  float *pax, *pay;  // [nsc] Poisson: accepted point of each background cell (NaN = none)
+  float *pcx, *pcy;  // [nsc] Poisson: this round's candidate point
+  uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
+  int32_t *pwin;     // [nsc] Poisson: 1 = this round's candidate is accepted
 };
 
 // Everything one job's step needs (passed by value for one instance, from an array for a batch).
@@ -73,12 +77,25 @@ MetroWS carve(void *base, int nsc) {
   p += align256(HS_MAX_UC * 4);
   w.info = (int32_t *)p;
   p += 256;
+  w.pax = (float *)p;
+  p += align256(n * 4);
+  w.pay = (float *)p;
+  p += align256(n * 4);
+  w.pcx = (float *)p;
+  p += align256(n * 4);
+  w.pcy = (float *)p;
+  p += align256(n * 4);
+  w.pcp = (uint32_t *)p;
+  p += align256(n * 4);
+  w.pwin = (int32_t *)p;
+  p += align256(n * 4);
   return w;
 }
 
 size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
-  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256;
+  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
+         6 * align256(n * 4);
 }
 
 // Block-wide sum of an int (1024 threads).
@@ -136,6 +153,95 @@ HS_INLINE double barrier_min_f64(const hs_instance_t &I, int c) {
   return dE > 4.0 * eb ? dE : fmin(eb * (q * q), dE + 4.0 * eb);
 }
 
+// Poisson-disc selection (the scheme of Magnets._select_Poisson, core.py:560-570, with the
+// minimal distance r of its docstring, core.py:464-467): parallel dart throwing on a background
+// grid of cells (r_x, r_y)/sqrt(2), so a cell holds at most one sample and every conflict lies
+// within +-2 cells. Each round, every empty background cell proposes a uniform point with a
+// Philox priority; a proposal survives if no accepted sample and no proposal of lower priority
+// (lower cell index on ties) lies within the ellipse (dx/r_x)^2 + (dy/r_y)^2 < 1 (wrapped with
+// PBC). Four rounds; the samples are then snapped to the nearest grid cell and kept where
+// occupied (Magnets.select, core.py:484-487). Writes cand[b] for every background cell b and
+// returns the number of valid samples (block-wide).
+__device__ int poisson_select(const hs_instance_t &I, const hs_select_t &SEL, const MetroWS &ws, uint64_t step,
+                              int attempt, int *s_tmp) {
+  const int tid = threadIdx.x;
+  const int gx = SEL.snx, gy = SEL.sny, nbg = SEL.nsc;
+  const float rx = (float)SEL.r_x, ry = (float)SEL.r_y;
+  const float sx = rx * 0.70710678f, sy = ry * 0.70710678f;
+  const float nxf = (float)I.nx, nyf = (float)I.ny;
+  auto close = [&](float x1, float y1, float x2, float y2) {
+    float dx = x1 - x2, dy = y1 - y2;
+    if (I.pbc) {
+      dx -= nxf * rintf(dx / nxf);
+      dy -= nyf * rintf(dy / nyf);
+    }
+    const float qx = dx / rx, qy = dy / ry;
+    return qx * qx + qy * qy < 1.f;
+  };
+  for (int b = tid; b < nbg; b += blockDim.x) ws.pax[b] = NAN;
+  __syncthreads();
+  for (int round = 0; round < 4; ++round) {
+    for (int b = tid; b < nbg; b += blockDim.x) {
+      uint32_t pr = 0xffffffffu;
+      float cx = 0.f, cy = 0.f;
+      if (isnan(ws.pax[b])) {
+        const uint4 w = hs_philox(make_uint4((uint32_t)b, (uint32_t)step, (uint32_t)(step >> 32),
+                                             6u | ((uint32_t)attempt << 8) | ((uint32_t)round << 24)),
+                                  (uint32_t)I.seed, (uint32_t)(I.seed >> 32));
+        cx = ((float)(b % gx) + (float)w.x * 0x1p-32f) * sx;
+        cy = ((float)(b / gx) + (float)w.y * 0x1p-32f) * sy;
+        if (cx < nxf && cy < nyf) pr = w.z == 0xffffffffu ? 0xfffffffeu : w.z;
+      }
+      ws.pcx[b] = cx;
+      ws.pcy[b] = cy;
+      ws.pcp[b] = pr;
+    }
+    __syncthreads();
+    for (int b = tid; b < nbg; b += blockDim.x) {
+      const uint32_t pr = ws.pcp[b];
+      bool win = pr != 0xffffffffu;
+      const int bx = b % gx, by = b / gx;
+      const float cx = ws.pcx[b], cy = ws.pcy[b];
+      for (int oy = -2; oy <= 2 && win; ++oy)
+        for (int ox = -2; ox <= 2 && win; ++ox) {
+          int qx = bx + ox, qy = by + oy;
+          if (I.pbc) {
+            qx = hs_pmod(qx, gx);
+            qy = hs_pmod(qy, gy);
+          } else if (qx < 0 || qx >= gx || qy < 0 || qy >= gy) {
+            continue;
+          }
+          const int q = qy * gx + qx;
+          if (q == b) continue;
+          const float ax = ws.pax[q];
+          if (!isnan(ax) && close(cx, cy, ax, ws.pay[q])) win = false;
+          const uint32_t pq = ws.pcp[q];
+          if (pq != 0xffffffffu && (pq < pr || (pq == pr && q < b)) && close(cx, cy, ws.pcx[q], ws.pcy[q])) win = false;
+        }
+      ws.pwin[b] = win ? 1 : 0;  // applied after every cell has read this round's points
+    }
+    __syncthreads();
+    for (int b = tid; b < nbg; b += blockDim.x)
+      if (ws.pwin[b]) {
+        ws.pax[b] = ws.pcx[b];
+        ws.pay[b] = ws.pcy[b];
+      }
+    __syncthreads();
+  }
+  int val = 0;
+  for (int b = tid; b < nbg; b += blockDim.x) {
+    int c = -1;
+    const float ax = ws.pax[b];
+    if (!isnan(ax)) {
+      const int x = min(I.nx - 1, (int)floorf(ax + 0.5f)), y = min(I.ny - 1, (int)floorf(ws.pay[b] + 0.5f));
+      if (I.flags[y * I.nx + x] & HS_FLAG_OCC) c = y * I.nx + x;
+    }
+    ws.cand[b] = c;
+    val += c >= 0;
+  }
+  return block_sum_int(val, s_tmp);
+}
+
 __device__ __forceinline__ void metro_select(const MetroJob &J) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
@@ -177,6 +283,14 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
     const uint64_t b0 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0));
     const uint64_t b1 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1));
     const uint64_t b2 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 2));
+    if (SEL.scheme == 2) {
+      const int tot_val = poisson_select(I, SEL, ws, step, attempt, s_i);
+      if (tot_val > 0) {
+        nvalid = tot_val;
+        break;
+      }
+      continue;  // no occupied sample: select() again (core.py:487)
+    }
     if (SEL.scheme == 0) {
       if (I.pbc) {  // np.random.randint(nx), randint(ny) (core.py:527)
         ox = (int)floor(hs_u53(b0, 0) * nx);
@@ -549,6 +663,11 @@ int check_select(const hs_select_t *sel) {
       return hs_set_error(HS_ERR_ARG, "bad selection geometry");
     if (sel->Rx > 0xffff || sel->Ry > 0x7fff) return hs_set_error(HS_ERR_ARG, "supercell too large");
     if (sel->nsc % sel->ncols) return hs_set_error(HS_ERR_ARG, "nsc is not a multiple of ncols");
+  } else if (sel->scheme == 2) {
+    if (sel->snx < 1 || sel->sny < 1 || (int64_t)sel->snx * sel->sny != sel->nsc || !(sel->r_x >= 1.5) ||
+        !(sel->r_y >= 1.5))
+      return hs_set_error(HS_ERR_ARG, "bad Poisson geometry (background %dx%d, r %g x %g cells)", sel->snx, sel->sny,
+                          sel->r_x, sel->r_y);
   } else if (sel->scheme != 1) {
     return hs_set_error(HS_ERR_ARG, "unsupported MULTISAMPLING_SCHEME");
   }
@@ -560,7 +679,7 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
   MetroJob J;
   J.I = *inst;
   J.SEL = *sel;
-  J.SEL.nsc = sel->scheme == 0 ? sel->nsc : 1;
+  J.SEL.nsc = sel->scheme != 1 ? sel->nsc : 1;
   J.R = *run;
   J.st = state;
   J.ws = carve(ws, J.SEL.nsc);
@@ -576,7 +695,7 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
 extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
   (void)inst;
   if (!sel) return -1;
-  return (int64_t)ws_bytes(sel->scheme == 0 ? sel->nsc : 1);
+  return (int64_t)ws_bytes(sel->scheme != 1 ? sel->nsc : 1);
 }
 
 extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
@@ -627,7 +746,7 @@ void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
   L.ws_off.resize(n);
   for (int i = 0; i < n; ++i) {
     L.ws_off[i] = off;
-    off += ws_bytes(teams[i].sel->scheme == 0 ? teams[i].sel->nsc : 1);
+    off += ws_bytes(teams[i].sel->scheme != 1 ? teams[i].sel->nsc : 1);
   }
   L.total = off;
 }

@@ -127,17 +127,20 @@ typedef struct hs_state {
   int32_t err;          /* nonzero on a device-side error */
 } hs_state_t;
 
-/* Metropolis selection geometry for one r (Magnets._select_grid, core.py:496-558). */
+/* Metropolis selection geometry for one r (Magnets._select_grid, core.py:496-558, or the
+ * Poisson-disc scheme of Magnets._select_Poisson, core.py:560-570). */
 typedef struct hs_select {
   const int32_t *pat_off;   /* [uy*ux+1] offsets into pat_cells for supercell pattern (dy,dx) */
   const int32_t *pat_cells; /* occupied (oy*Rx+ox) cells of occupation[dy:dy+Ry, dx:dx+Rx], row-major */
   int32_t Rx, Ry;           /* supercell size in cells */
   int32_t snx, sny;         /* supercells along x/y: n//R + 1 */
-  int32_t ncols, nsc;       /* chosen (every other) supercells per row and in total */
-  int32_t scheme;           /* 0 = 'grid', 1 = 'single' */
+  int32_t ncols, nsc;       /* chosen (every other) supercells per row and in total; Poisson: background cells */
+  int32_t scheme;           /* 0 = 'grid', 1 = 'single', 2 = 'Poisson' */
   int32_t cutoff;           /* SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF */
   int32_t reduced_k;        /* REDUCED_KERNEL_SIZE (0 = full kernel) */
   int32_t metropolis_time;  /* SimParams.METROPOLIS_TIME */
+  double r_x, r_y;          /* Poisson: minimal sample distance r in cells along x (r/dx) and y (r/dy);
+                               the background grid is snx x sny cells of (r_x, r_y)/sqrt(2) */
 } hs_select_t;
 
 /*
