@@ -159,33 +159,30 @@ class IP_Square_Closed(IP_ASI):
         return 'afm'
 
     def vertex_counts(self):
-        """Fractions of the four square-ice vertex types (the counting of
-        examples/IP_SquareVertices.py:19-35): over every vertex with four magnets, type I
-        (2-in/2-out, alternating), II (2-in/2-out, adjacent), III (3-1) and IV (4-0)."""
+        """Fractions (N1, N2, N3, N4) of the four square-ice vertex types, counted as in
+        examples/IP_SquareVertices.py:19-35: at every vertex (even x and y), the in-pointing count
+        is the wrapped correlation of m with [[0,1,0],[1,0,-1],[0,-1,0]] (|.| = 2: type III,
+        4: type IV) and the 'flower' sum the correlation with [[0,1,0],[-1,0,1],[0,-1,0]]
+        (0 in-pointing and |flower| = 4: type I, otherwise type II)."""
         m = self.m
-        o = self.orientation
-        ny, nx = self.shape
-        counts = np.zeros(4)
-        # a vertex sits at an even (x, y) cell: neighbours left/right (horizontal magnets, odd x)
-        # and below/above (vertical magnets, odd y)
-        for y in range(0, ny, 2):
-            for x in range(0, nx, 2):
-                if not (0 < x < nx - 1 and 0 < y < ny - 1):
-                    continue
-                left, right = m[y, x - 1] * o[y, x - 1, 0], m[y, x + 1] * o[y, x + 1, 0]
-                down, up = m[y - 1, x] * o[y - 1, x, 1], m[y + 1, x] * o[y + 1, x, 1]
-                ins = [left > 0, right < 0, down > 0, up < 0]  # pointing into the vertex
-                k = sum(ins)
-                if k in (0, 4):
-                    counts[3] += 1
-                elif k in (1, 3):
-                    counts[2] += 1
-                elif ins[0] == ins[1]:  # both horizontal in (or out): opposite pair -> type I
-                    counts[0] += 1
-                else:
-                    counts[1] += 1
-        tot = counts.sum()
-        return counts / tot if tot else counts
+
+        def corr_wrap(mask):  # convolve2d(m, mask[::-1, ::-1], 'same', boundary='wrap')
+            out = np.zeros_like(m)
+            for i in range(3):
+                for j in range(3):
+                    if mask[i][j]:
+                        out += mask[i][j] * np.roll(m, (1 - i, 1 - j), axis=(0, 1))
+            return out
+
+        vertices = (self.ixx % 2 == 0) & (self.iyy % 2 == 0)
+        n_in = corr_wrap([[0, 1, 0], [1, 0, -1], [0, -1, 0]])[vertices]
+        flower = corr_wrap([[0, 1, 0], [-1, 0, 1], [0, -1, 0]])[vertices]
+        n3 = np.sum(np.abs(n_in) == 2)
+        n4 = np.sum(np.abs(n_in) == 4)
+        n1 = np.sum((np.abs(n_in) == 0) & (np.abs(flower) == 4))
+        n2 = np.sum((np.abs(n_in) == 0) & (np.abs(flower) != 4))
+        total = n1 + n2 + n3 + n4
+        return np.array([n1, n2, n3, n4], dtype=float) / total if total else np.zeros(4)
 
 
 class IP_Pinwheel_Diamond(IP_Square_Closed):
