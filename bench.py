@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PERIOD = 1024
+# HBM bytes per launch of k_neel_team on the 256x256 pinwheel workload (1024 attempts), from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy of known size; the
+# commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
+NEEL_TEAM_TRAFFIC_PER_LAUNCH = 66.6e6
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
@@ -52,6 +56,9 @@ def parse():
     p.add_argument("--jobs", type=int, default=32, help="served: number of jobs")
     p.add_argument("--field-steps", type=int, default=100, help="served: field steps per job")
     p.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                   help="first copy MB MiB between two device buffers (a known read + write volume that "
+                        "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     return p.parse_args()
 
 
@@ -153,6 +160,8 @@ def bench_neel(args):
                       "switches": mm.switches - s0, "rel_E_err_vs_fp64": rel_err, "parallelism": "single GPU"})
     line["roofline"] = roofline("k_neel_team", launches, bytes_per * attempts,
                                 {"attempts_per_launch": attempts / len(launches), "bytes_per_attempt": bytes_per})
+    if abs(attempts / len(launches) - PERIOD) < 1:
+        line["roofline"]["traffic"] = NEEL_TEAM_TRAFFIC_PER_LAUNCH
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_neel(args.cpu_seconds)
     return line
@@ -374,6 +383,13 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run here uses one MI355X).",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fn = {"neel": bench_neel, "metro64": lambda a: bench_metro(a, False), "metro2048": lambda a: bench_metro(a, True),
           "batched": bench_batched, "neel_batched": bench_neel_batched, "served": bench_served}[args.workload]
     print(json.dumps(fn(args)), flush=True)
