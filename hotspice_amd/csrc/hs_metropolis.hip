@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <vector>
 
 #include "hs_device.h"
@@ -31,6 +32,8 @@ namespace {
 constexpr int MS_THREADS = 1024;
 constexpr int MU_TX = 32, MU_TY = 8, MU_THREADS = MU_TX * MU_TY;
 constexpr int MAX_ATTEMPTS = 4096;
+// instances up to this many grid cells step in the persistent one-workgroup kernel
+constexpr int PERSIST_CELLS = 80 * 80;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -105,6 +108,16 @@ HS_INLINE int block_sum_int(int v, int *s_tmp) {
   if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
   __syncthreads();
   int t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_tmp[w];
+  return t;
+}
+
+HS_INLINE long long block_sum_i64(long long v, long long *s_tmp) {
+  v = hs_wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  long long t = 0;
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_tmp[w];
   return t;
 }
@@ -242,7 +255,7 @@ __device__ int poisson_select(const hs_instance_t &I, const hs_select_t &SEL, co
   return block_sum_int(val, s_tmp);
 }
 
-__device__ __forceinline__ void metro_select(const MetroJob &J) {
+__device__ __forceinline__ bool metro_select(const MetroJob &J) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const hs_run_t &R = J.R;
@@ -250,6 +263,7 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
   const MetroWS &ws = J.ws;
   int32_t *log_cells = J.log_cells, *log_count = J.log_count;
   __shared__ int s_i[MS_THREADS / 64];
+  __shared__ long long s_l[MS_THREADS / 64];
   __shared__ double s_d[MS_THREADS / 64];
   __shared__ int s_site[HS_MAX_UC];
   __shared__ int s_single;
@@ -268,7 +282,7 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
         st[0] = S;
       }
     }
-    return;
+    return false;
   }
   const uint64_t step = (uint64_t)S.step;
   const int nsc = SEL.nsc;
@@ -324,8 +338,9 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
         inb += ok;
         val += v;
       }
-      const int tot_inb = block_sum_int(inb, s_i);
-      const int tot_val = block_sum_int(val, s_i);
+      // one block reduction of both counts (in-range samples in the high word, valid ones in the low)
+      const long long both = block_sum_i64(((long long)inb << 32) + val, s_l);
+      const int tot_inb = (int)(both >> 32), tot_val = (int)(both & 0xffffffffll);
       if (tot_inb > 0) {
         if (tot_val > 0) {
           nvalid = tot_val;
@@ -352,7 +367,7 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
       st[0] = S;
       *ws.acc_n = 0;
     }
-    return;
+    return false;
   }
   // acceptance (core.py:662-671); with a single sample thread 0 plays candidate 0
   int my_acc = 0;
@@ -364,7 +379,9 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
     if (c >= 0) {
       const double e = (double)I.E[c] - (double)I.Ec[c];
       const double beta = (double)I.beta[c];
-      double p = exp(2.0 * e * beta);  // exp(-switch_energy*beta), switch_energy = -2E
+      // exp(-switch_energy*beta) with switch_energy = -2E, relative error ~3e-7 (FP64 exponent
+      // split, FP32 mantissa), clipped to [1e-10, 1e10]
+      double p = hs_scaled_exp(1.f, 2.0 * e * beta, 1.f, 1.0);
       p = fmin(fmax(p, 1e-10), 1e10);
       if (SEL.metropolis_time) {
         const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
@@ -425,6 +442,7 @@ __device__ __forceinline__ void metro_select(const MetroJob &J) {
     if (SEL.metropolis_time) S.t += tsum;
     st[0] = S;
   }
+  return true;
 }
 
 // Displacement candidates d = i - s (+k*n) with |d| <= w: wrap enumerates images, fill only k = 0.
@@ -453,7 +471,10 @@ HS_INLINE bool reaches(int s, int w, int n, bool wrap, int lo, int hi) {
 
 HS_INLINE int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
+// Update of one TX x TY output tile by a workgroup of TX*TY threads (one cell per thread).
+template <int TX, int TY>
 __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int tile_y) {
+  constexpr int MU_TX = TX, MU_TY = TY, MU_THREADS = TX * TY;
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const MetroWS &ws = J.ws;
@@ -627,11 +648,15 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
 
 __global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J) { metro_select(J); }
 
-__global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob *__restrict__ jobs) {
-  metro_select(jobs[blockIdx.x]);
+// One selection workgroup per job of the list `which`.
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob *__restrict__ jobs,
+                                                                  const int32_t *__restrict__ which) {
+  metro_select(jobs[which[blockIdx.x]]);
 }
 
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J) { metro_update(J, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J) {
+  metro_update<MU_TX, MU_TY>(J, blockIdx.x, blockIdx.y);
+}
 
 // One workgroup per tile of every job: tile_job[b] is the job of block b, tile_first[j] its first block.
 __global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob *__restrict__ jobs,
@@ -640,7 +665,29 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob
   const int jb = tile_job[blockIdx.x];
   const MetroJob &J = jobs[jb];
   const int t = blockIdx.x - tile_first[jb];
-  metro_update(J, t % J.tiles_x, t / J.tiles_x);
+  metro_update<MU_TX, MU_TY>(J, t % J.tiles_x, t / J.tiles_x);
+}
+
+// Persistent Metropolis stepping of small instances: one 1024-thread workgroup per job runs the
+// selection, then the update of every 32x32 tile of its grid, step after step, until the job's
+// stop predicate or step budget ends it. The arithmetic is that of the two-launch step (the same
+// selection code; every cell adds the same sources in the same order whatever the tile shape).
+__device__ __forceinline__ void metro_persist(const MetroJob &J) {
+  const int tx = (J.I.nx + 31) / 32, ty = (J.I.ny + 31) / 32;
+  for (int64_t s = 0; s < J.R.max_steps; ++s) {
+    const bool stepped = metro_select(J);
+    __syncthreads();
+    if (!stepped) break;
+    for (int t = 0; t < tx * ty; ++t) metro_update<32, 32>(J, t % tx, t / tx);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(MS_THREADS) k_metro_persist_one(MetroJob J) { metro_persist(J); }
+
+__global__ void __launch_bounds__(MS_THREADS) k_metro_persist_many(const MetroJob *__restrict__ jobs,
+                                                                   const int32_t *__restrict__ which) {
+  metro_persist(jobs[which[blockIdx.x]]);
 }
 
 __global__ void k_state_begin_many(const MetroJob *__restrict__ jobs, int njobs) {
@@ -712,6 +759,11 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   const MetroJob J = make_job(inst, sel, run, state, ws, log_cells, log_count);
   k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
   HS_LAUNCH_CHECK("k_state_begin");
+  if (inst->nx * inst->ny <= PERSIST_CELLS) {
+    k_metro_persist_one<<<1, MS_THREADS, 0, s>>>(J);
+    HS_LAUNCH_CHECK("k_metro_persist_one");
+    return HS_OK;
+  }
   const dim3 grid(J.tiles_x, J.tiles_y);
   for (int64_t j = 0; j < run->max_steps; ++j) {
     k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J);
@@ -725,24 +777,40 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
 namespace {
 
 struct MetroLayout {
-  size_t jobs_off, tile_job_off, tile_first_off, total;
+  size_t jobs_off, tile_job_off, tile_first_off, small_off, large_off, total;
   std::vector<size_t> ws_off;
-  std::vector<int32_t> first;
+  std::vector<int32_t> first, small, large;
   int32_t ntiles;
 };
 
+bool persistent(const hs_instance_t &in) { return in.nx * in.ny <= PERSIST_CELLS; }
+
+// A batch of small instances only runs in the persistent kernel (one launch for all steps).
+// Once the batch has a large instance, its two launches per step pace the batch anyway, and the
+// small instances ride along in them at no extra cost.
 void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
   L.jobs_off = 0;
   L.ntiles = 0;
-  L.first.resize(n);
+  L.first.assign(n, 0);
+  L.small.clear();
+  L.large.clear();
+  bool all_small = true;
+  for (int i = 0; i < n; ++i) all_small = all_small && persistent(*teams[i].inst);
   for (int i = 0; i < n; ++i) {
-    L.first[i] = L.ntiles;
     const hs_instance_t &in = *teams[i].inst;
+    if (all_small) {
+      L.small.push_back(i);
+      continue;
+    }
+    L.large.push_back(i);
+    L.first[i] = L.ntiles;
     L.ntiles += ((in.nx + MU_TX - 1) / MU_TX) * ((in.ny + MU_TY - 1) / MU_TY);
   }
   L.tile_job_off = align256(sizeof(MetroJob) * (size_t)n);
-  L.tile_first_off = L.tile_job_off + align256(4 * (size_t)L.ntiles);
-  size_t off = L.tile_first_off + align256(4 * (size_t)n);
+  L.tile_first_off = L.tile_job_off + align256(4 * (size_t)(L.ntiles > 0 ? L.ntiles : 1));
+  L.small_off = L.tile_first_off + align256(4 * (size_t)n);
+  L.large_off = L.small_off + align256(4 * (size_t)n);
+  size_t off = L.large_off + align256(4 * (size_t)n);
   L.ws_off.resize(n);
   for (int i = 0; i < n; ++i) {
     L.ws_off[i] = off;
@@ -782,17 +850,24 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   if (ws_bytes_ < (int64_t)L.total)
     return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes_, (long long)L.total);
   char *base = (char *)ws;
-  std::vector<char> host(L.tile_first_off + align256(4 * (size_t)nteams), 0);
+  std::vector<char> host(L.large_off + align256(4 * (size_t)nteams), 0);
   MetroJob *jobs = (MetroJob *)(host.data() + L.jobs_off);
   int32_t *tile_job = (int32_t *)(host.data() + L.tile_job_off);
   int32_t *tile_first = (int32_t *)(host.data() + L.tile_first_off);
+  int32_t *small = (int32_t *)(host.data() + L.small_off);
+  int64_t large_steps = 0;
   for (int i = 0; i < nteams; ++i) {
     const hs_team_t &t = teams[i];
     jobs[i] = make_job(t.inst, t.sel, &t.run, t.state, base + L.ws_off[i], t.log_cells, t.log_idx);
     tile_first[i] = L.first[i];
+    if (L.large.empty() || std::find(L.large.begin(), L.large.end(), i) == L.large.end()) continue;
+    large_steps = t.run.max_steps > large_steps ? t.run.max_steps : large_steps;
     const int nt = jobs[i].tiles_x * jobs[i].tiles_y;
     for (int b = 0; b < nt; ++b) tile_job[L.first[i] + b] = i;
   }
+  for (size_t k = 0; k < L.small.size(); ++k) small[k] = L.small[k];
+  int32_t *large = (int32_t *)(host.data() + L.large_off);
+  for (size_t k = 0; k < L.large.size(); ++k) large[k] = L.large[k];
   hipStream_t s = (hipStream_t)stream;
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
@@ -801,8 +876,13 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   const int32_t *dtf = (const int32_t *)(base + L.tile_first_off);
   k_state_begin_many<<<(nteams + 255) / 256, 256, 0, s>>>(dj, nteams);
   HS_LAUNCH_CHECK("k_state_begin_many");
+  if (!L.small.empty()) {  // a batch of small instances: one persistent workgroup each
+    k_metro_persist_many<<<(unsigned)L.small.size(), MS_THREADS, 0, s>>>(dj, (const int32_t *)(base + L.small_off));
+    HS_LAUNCH_CHECK("k_metro_persist_many");
+  }
+  steps = L.ntiles > 0 ? large_steps : 0;
   for (int64_t j = 0; j < steps; ++j) {
-    k_metro_select_many<<<nteams, MS_THREADS, 0, s>>>(dj);
+    k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, (const int32_t *)(base + L.large_off));
     HS_LAUNCH_CHECK("k_metro_select_many");
     k_metro_update_many<<<L.ntiles, MU_THREADS, 0, s>>>(dj, dtj, dtf);
     HS_LAUNCH_CHECK("k_metro_update_many");

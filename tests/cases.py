@@ -48,6 +48,12 @@ SYNTH = {
                              params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=3,
                                          SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
                              update_kw=dict(r=6e-6), nsteps=150, seed=31),
+    # above the persistent-kernel size (80 x 80 cells): the two-launch step
+    "metro_oop_window_large": dict(lattice="OOP_Square", a=1e-6, n=136, T=900, moment=1.1e-15,
+                                   energies=[("dipolar", dict(cutoff_range=3e-6)), ("exchange", dict(J_kT=0.1))],
+                                   params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=3,
+                                               SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
+                                   update_kw=dict(r=6e-6), nsteps=60, seed=33),
     "metro_ipsq_window": dict(lattice="IP_Square", a=1e-6, n=48, T=400, moment=2.37e-16,
                               energies=[("dipolar", dict(cutoff_range=3e-6)), ("zeeman", dict(magnitude=1e-3, angle=0.7))],
                               params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,
