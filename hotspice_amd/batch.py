@@ -40,6 +40,8 @@ def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None,
             raise ValueError("T=0 somewhere in an instance: no switch can be simulated")
         m._pre_step()
     nbl = nblocks if isinstance(nblocks, (list, tuple)) else [nblocks or 0] * len(mags)
+    if scheme == Scheme.NEEL:  # per-instance team sizes ('auto' instances are tuned first)
+        nbl = [nb or (m._neel_nblocks() if m.neel_blocks else 0) for m, nb in zip(mags, nbl)]
     starts = [(st.t, st.attempts) for st in read_states(mags)]
     attempt_freq = float(kwargs.get('attempt_freq', 1e10))
     if per_instance is not None and len(per_instance) != len(mags):
@@ -68,8 +70,9 @@ def _pending(m, start, t_max, mc, st=None):
 
 
 def neel_team_size(m, nblocks=0):
-    """Workgroups of an instance's Neel team: `nblocks`, else the library's default of one grid
-    cell per thread (1024 per workgroup), at most 128."""
+    """Workgroups of an instance's Neel team: `nblocks` (or the instance's own setting), else the
+    library's default of one grid cell per thread (1024 per workgroup), at most 128."""
+    nblocks = nblocks or (m.neel_blocks if isinstance(m.neel_blocks, int) else 0)
     if nblocks:
         return int(nblocks)
     return max(1, min(128, -(-(m.nx * m.ny) // 1024)))

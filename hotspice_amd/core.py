@@ -108,7 +108,7 @@ class Magnets(ABC):
                  recompute_every: int | None = 1024):
         self.seed = int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed) & 0xFFFFFFFFFFFFFFFF
         self.neel_persistent = True  # one persistent launch per progress() chunk (False: one launch per step)
-        self.neel_blocks = 0         # workgroups in the instance's team (0: automatic)
+        self.neel_blocks = 0         # workgroups in the instance's team (0: default, 'auto': autotuned)
         self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
         self.params = SimParams() if params is None else params
         self.recompute_every = recompute_every
@@ -714,6 +714,13 @@ class Magnets(ABC):
             case other:
                 raise ValueError(f"The Magnets.params object has an invalid value for UPDATE_SCHEME='{other}'.")
 
+    def _neel_nblocks(self):
+        """Team size of the next Neel launch ('auto' runs the autotuner once)."""
+        if self.neel_blocks == 'auto':
+            from .tune import tune_neel_blocks
+            tune_neel_blocks(self)
+        return self.neel_blocks
+
     def _update_Néel(self, t_max=1, attempt_freq=1e10):
         """core.py:679-709: returns the (y, x) divmod tuple of the switched magnet, or None."""
         self._pre_step()
@@ -722,7 +729,7 @@ class Magnets(ABC):
                        nblocks=0, begin=1)
         if not self._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-        self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self.neel_blocks)
+        self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self._neel_nblocks())
         self._after_device_step()
         idx = int(self._dev.log_idx[0].item())
         st = self._state()
@@ -874,7 +881,7 @@ class Magnets(ABC):
                     raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
                 if begin:
                     self._set_progress_origin(t0, a0)
-                self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self.neel_blocks)
+                self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self._neel_nblocks())
                 begin = False
                 st = self._state()
                 if st.err:

@@ -156,7 +156,7 @@ class Server:
                 steps = min(left, self.neel_chunk)
                 if a.mm.recompute_every:
                     steps = min(steps, max(1, a.mm.recompute_every - a.mm._since_recompute))
-                items.append((a.mm, steps, math.inf, math.inf, 0))
+                items.append((a.mm, steps, math.inf, math.inf, g))
                 used += g
                 self.attempts -= st.attempts
             batch.launch_neel(items, 1e10)

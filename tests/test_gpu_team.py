@@ -122,3 +122,18 @@ def test_recompute_many_equals_single():
         if w == 3:
             np.testing.assert_array_equal(x._dev.Ep.cpu().numpy(), y._dev.Ep.cpu().numpy())
             np.testing.assert_array_equal(x._dev.Epc.cpu().numpy(), y._dev.Epc.cpu().numpy())
+
+
+def test_autotuner_keeps_state_and_results():
+    """tune_neel_blocks times team sizes on the instance itself, restores its state exactly and
+    leaves a choice that gives the same trajectory as the default team."""
+    from hotspice_amd import tune
+    a = _pinwheel(64, 9)
+    b = _pinwheel(64, 9)
+    before = _snapshot(a)
+    timing = tune.tune_neel_blocks(a, steps=64, reuse=False)
+    assert set(timing) == set(tune.neel_candidates(a)) and a.neel_blocks in timing
+    _same(_snapshot(a), before)
+    a.progress(t_max=math.inf, MCsteps_max=0.5)
+    b.progress(t_max=math.inf, MCsteps_max=0.5)
+    _same(_snapshot(a), _snapshot(b))
