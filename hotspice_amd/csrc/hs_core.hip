@@ -394,18 +394,25 @@ __global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan 
 
 // Several recomputes in one launch: block b works for job block_job[b] (its blocks in the order
 // tile, then sublattice p, then source group g).
+__device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *Kbase, double *part, int c);
+
 struct RecJob {
   hs_instance_t I;
   RecPlan plan;
   const double *K;
   double *part;
-  int32_t first;
+  int32_t first, small;
 };
 
 __global__ void __launch_bounds__(256) k_recompute_dip_many(const RecJob *__restrict__ jobs,
                                                             const int32_t *__restrict__ block_job) {
   const RecJob &J = jobs[block_job[blockIdx.x]];
-  const int lb = blockIdx.x - J.first, nt = J.plan.tA * J.plan.tB;
+  const int lb = blockIdx.x - J.first;
+  if (J.small) {
+    rec_small(J.I, J.K, J.part, lb * 256 + threadIdx.x);
+    return;
+  }
+  const int nt = J.plan.tA * J.plan.tB;
   rec_dip(J.I, J.plan, J.K, J.part, lb % nt, (lb / nt) % J.I.n_sub, lb / (nt * J.I.n_sub));
 }
 
@@ -457,6 +464,35 @@ __global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, 
   rec_finish(I, F, partE, partP, which, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Small grids: one thread per target cell sums K[source - target] * (m mu)[source] over the
+// stored window directly (the tiled kernel's 32x64 tiles would be mostly padding there). Only
+// occupied targets on a kernel's sublattice are computed, as the finish pass reads no others.
+__device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *Kbase, double *part, int c) {
+  const int nx = I.nx, ny = I.ny;
+  if (c >= nx * ny) return;
+  const int y = c / nx, x = c % nx, n = I.sub_of[hs_site(I, y, x)];
+  double acc = 0.0;
+  if (n >= 0 && I.m[c] != 0) {
+    const int KW = 2 * I.kwx + 1;
+    const double *Kn = Kbase + (size_t)n * KW * (2 * I.kwy + 1);
+    const int y0 = max(0, y - I.kwy), y1 = min(ny - 1, y + I.kwy);
+    const int x0 = max(0, x - I.kwx), x1 = min(nx - 1, x + I.kwx);
+    for (int ys = y0; ys <= y1; ++ys) {
+      const double *krow = Kn + (size_t)(ys - y + I.kwy) * KW + (I.kwx - x);
+      for (int xs = x0; xs <= x1; ++xs) {
+        const int s = ys * nx + xs;
+        const double v = (double)I.m[s] * I.mu64[s];
+        if (v != 0.0) acc = fma(krow[xs], v, acc);
+      }
+    }
+  }
+  part[c] = acc;
+}
+
+__global__ void k_recompute_small(hs_instance_t I, const double *Kbase, double *part) {
+  rec_small(I, Kbase, part, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 struct FinJob {
   hs_instance_t I;
   const double *partE, *partP;
@@ -468,6 +504,10 @@ __global__ void k_recompute_finish_many(const FinJob *__restrict__ jobs, const i
   rec_finish(J.I, J.F, J.partE, J.partP, J.which, (blockIdx.x - J.first) * blockDim.x + threadIdx.x);
 }
 
+constexpr int SMALL_REC_CELLS = 4096;  // grids up to this many cells use rec_small
+
+static bool small_rec(const hs_instance_t *I) { return I->nx * I->ny <= SMALL_REC_CELLS; }
+
 static RecPlan make_plan(const hs_instance_t *I) {
   RecPlan pl{};
   pl.Ay = (I->ny + I->uy - 1) / I->uy;
@@ -477,6 +517,7 @@ static RecPlan make_plan(const hs_instance_t *I) {
   const int tiles = pl.tA * pl.tB * (I->n_sub > 0 ? I->n_sub : 1);
   int F = (768 + tiles - 1) / tiles;
   pl.F = F < 1 ? 1 : (F > 64 ? 64 : F);
+  if (small_rec(I)) pl.F = 1;
   for (int s = 0; s < I->ux * I->uy; ++s) {
     const int n = I->sub_of[s];
     if (n >= 0 && n < HS_MAX_UC) {
@@ -517,12 +558,15 @@ extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, voi
     partE = (double *)scratch;
     partP = partE + (size_t)pl.F * ncell;
     const dim3 grid(pl.tA * pl.tB, inst->n_sub, pl.F);
+    const unsigned sb = (unsigned)((ncell + 255) / 256);
     if (which & 1) {
-      k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->K64, partE);
+      if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->K64, partE);
+      else k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->K64, partE);
       HS_LAUNCH_CHECK("k_recompute_dip");
     }
     if (do_p) {
-      k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->Kps64, partP);
+      if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->Kps64, partP);
+      else k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->Kps64, partP);
       HS_LAUNCH_CHECK("k_recompute_dip");
     }
   }
@@ -546,7 +590,7 @@ ManyLayout many_layout(const hs_instance_t *const *insts, const int32_t *which, 
     const hs_instance_t &I = *insts[i];
     const RecPlan pl = make_plan(&I);
     if (I.dip_pf != 0.0) {
-      const int per = pl.tA * pl.tB * I.n_sub * pl.F;
+      const int per = small_rec(&I) ? (I.nx * I.ny + 255) / 256 : pl.tA * pl.tB * I.n_sub * pl.F;
       if (which[i] & 1) {
         ++L.ndip;
         L.dip_blocks += per;
@@ -613,7 +657,7 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
     double *partE = dip ? (double *)scratch[i] : nullptr;
     double *partP = dip ? partE + (size_t)pl.F * ncell : nullptr;
     if (dip) {
-      const int per = pl.tA * pl.tB * I.n_sub * pl.F;
+      const int per = small_rec(&I) ? (ncell + 255) / 256 : pl.tA * pl.tB * I.n_sub * pl.F;
       for (int w = 1; w <= 2; w <<= 1) {
         if (!(which[i] & w) || (w == 2 && !I.use_perp)) continue;
         RecJob &J = dj[nd];
@@ -622,6 +666,7 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
         J.K = w == 1 ? I.K64 : I.Kps64;
         J.part = w == 1 ? partE : partP;
         J.first = db;
+        J.small = small_rec(&I) ? 1 : 0;
         for (int b = 0; b < per; ++b) dmap[db + b] = nd;
         db += per;
         ++nd;
