@@ -3,20 +3,21 @@
 //
 // Every instance gets a team of G co-resident workgroups (1024 threads). Each thread owns up to
 // MAXC grid cells (c = rank*1024 + tid + k*G*1024) and keeps their spin, combined fields (+Kahan
-// compensations), E_B and beta in registers for the whole launch. Per step:
-//   1. keys log(tau) = log(-ln u) - log(rate) for the owned cells (Philox counter = cell, same
-//      streams as the single-instance path; -ln u of the next step is drawn while the winner's
-//      kernel row is being loaded), packed as (key bits | cell | spin) so one 64-bit minimum per
-//      wave, then per workgroup, gives the lowest key with nanargmin's first-index tie break;
-//   2. the owner of the workgroup's best cell evaluates its FP64 waiting time. G == 1: that is
-//      the winner. G > 1: the owner writes the record {candidate, time} into the round's slot
-//      and arrives at the team's counter barrier (bounded spin, error on timeout); one wave
-//      then reads the G records and reduces them. Records rotate over three slots, and every
-//      workgroup reads round k's slot before it arrives at round k+1, so no write of a later
-//      round reaches a slot that is still being read;
-//   3. every thread applies the identical time/counter update (core.py:700-707) and the
-//      rank-1 field update of the winner (energies.py:310-329 + exchange stencil) to its
-//      registers, reading the winner's kernel row from L2.
+// compensations), E_B and moment in registers for the whole launch. Per step:
+//   1. keys log(tau) = log(-ln u) - log(rate) for the owned cells (Philox counter = cell, the
+//      streams of the single-instance path), packed as (key bits | position | spin) so one 64-bit
+//      DPP minimum per wave gives the lowest key with nanargmin's first-index tie break; the lane
+//      holding its wave's best leaves that cell's fields and draw in LDS;
+//   2. wave 0 reduces the 16 wave candidates and evaluates the FP64 waiting time of the
+//      workgroup's best from its compensated fields. G == 1: that is the winner. G > 1: wave 0
+//      writes the workgroup's record {candidate, time}, both words tagged with the round, into
+//      the round's slot and polls (bounded spin, error on timeout) until all G records of the
+//      round carry the tag, then reduces them. Slots rotate over three rounds; a workgroup reads
+//      round k's records before it writes round k+1's, so no slot is overwritten while read.
+//      Wave 0 then advances time and counters (core.py:700-707). Meanwhile waves 1-15 draw
+//      -ln u for the coming steps of all the workgroup's cells;
+//   3. every thread applies the rank-1 field update of the winner (energies.py:310-329 +
+//      exchange stencil) to its registers, reading the winner's kernel row from L2.
 // State is written back once at the end. Teams never wait for each other.
 #include <hip/hip_runtime.h>
 
@@ -162,29 +163,30 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   // neighbouring lanes because base and stride are even. For an even step s the even lane
   // evaluates call j at step s and the odd lane at step s+1, and each hands the other the half
   // it needs (DPP swap of adjacent lanes): one Philox call per cell covers two steps. A single
-  // step (an odd first step) is drawn by both lanes evaluating the same call.
-  auto draw = [&](int64_t s, bool two) {
-    const uint32_t odd = (uint32_t)tid & 1u;
+  // step (an odd first step) is drawn by both lanes evaluating the same call. `v` is the thread
+  // whose cells are drawn (any thread can draw any thread's cells; lane parity must match).
+  auto draw = [&](int64_t s, bool two, int v) {
+    const uint32_t odd = (uint32_t)v & 1u;
 #pragma unroll 1
     for (int k = 0; k < MAXC; ++k) {
-      const uint32_t j = (uint32_t)(base + k * stride) >> 1;
+      const uint32_t j = (uint32_t)(rank * TT + v + k * stride) >> 1;
       const uint64_t sk = (uint64_t)s + (two ? odd : 0u);
       const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
                                 (uint32_t)(I.seed >> 32));
       const uint32_t ma = odd ? W.z : W.x, mb = odd ? W.w : W.y;  // this lane's cell in this call
       if (!two) {
-        s_x[s & 1][k][tid] = hs_exp1_fast(ma, mb);
+        s_x[s & 1][k][v] = hs_exp1_fast(ma, mb);
       } else {
         const uint32_t ra = hs_dpp_u32<0xB1>(0u, odd ? W.x : W.z);  // quad_perm [1,0,3,2]: lane ^ 1
         const uint32_t rb = hs_dpp_u32<0xB1>(0u, odd ? W.y : W.w);
-        s_x[0][k][tid] = hs_exp1_fast(odd ? ra : ma, odd ? rb : mb);
-        s_x[1][k][tid] = hs_exp1_fast(odd ? ma : ra, odd ? mb : rb);
+        s_x[0][k][v] = hs_exp1_fast(odd ? ra : ma, odd ? rb : mb);
+        s_x[1][k][v] = hs_exp1_fast(odd ? ma : ra, odd ? mb : rb);
       }
     }
   };
 
   bool go = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
-  if (go) draw(step, (step & 1) == 0);
+  if (go) draw(step, (step & 1) == 0, tid);
   while (go) {
     const int par = (int)(step & 1);
     // 1. keys of the owned candidates, packed (key bits | position | m+1) so one 64-bit minimum
@@ -302,17 +304,19 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         s_win = pc;
         s_ctl = ctl;
       }
+    } else if (step & 1) {
+      // meanwhile waves 1-15 draw the next (even) step and the one after for all 16 waves' cells
+      // (the keys of this step and wave 0's copy of the best cell's draw were read before the
+      // barrier above, so both buffers are free)
+      draw(step + 1, true, tid - 64);
+      if (wv == 1) draw(step + 1, true, tid + (TT - 128));
     }
     __syncthreads();
     const uint32_t ctl = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ctl);
     if (ctl & 4u) break;
     step += 1;
     go = (ctl & 2u) != 0;
-    const bool two = (step & 1) == 0;  // an even step draws itself and the next
-    if (!(ctl & 1u)) {
-      if (two) draw(step, true);
-      continue;
-    }
+    if (!(ctl & 1u)) continue;
     // 4. rank-1 field update of the winner (energies.py:310-329 + exchange stencil)
     const unsigned long long win = s_win;
     const uint32_t wpos = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)(win >> 2) & 0xFFFFFFu));
@@ -344,7 +348,6 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
         }
       }
-      if (k0 == 0 && two) draw(step, true);  // the next draws overlap the loads above
 #pragma unroll
       for (int j = 0; j < CH; ++j) {
         const int k = k0 + j;
