@@ -306,7 +306,9 @@ __device__ __forceinline__ void rec_dip(const hs_instance_t &I, const RecPlan &p
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
   const double *Kn = Kbase + (size_t)p * KS;
-  const int ta = tid / (RB / RR), tbg = tid % (RB / RR);
+  // consecutive lanes take consecutive target rows: their kernel reads are one row (127 doubles)
+  // apart, which spreads a wave over all LDS banks (column groups would hit the same banks)
+  const int ta = tid % RA, tbg = tid / RA;
   double acc[RR];
 #pragma unroll
   for (int j = 0; j < RR; ++j) acc[j] = 0.0;
