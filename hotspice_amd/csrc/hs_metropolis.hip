@@ -47,6 +47,7 @@ struct MetroWS {
   float *pcx, *pcy;  // [nsc] Poisson: this round's candidate point
   uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
   int32_t *pwin;     // [nsc] Poisson: 1 = this round's candidate is accepted
+  float *cmmu;       // [nsc] (m mu) of each candidate after its flip (read where accf is set)
 };
 
 // Everything one job's step needs (passed by value for one instance, from an array for a batch).
@@ -92,13 +93,15 @@ MetroWS carve(void *base, int nsc) {
   p += align256(n * 4);
   w.pwin = (int32_t *)p;
   p += align256(n * 4);
+  w.cmmu = (float *)p;
+  p += align256(n * 4);
   return w;
 }
 
 size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
   return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
-         6 * align256(n * 4);
+         7 * align256(n * 4);
 }
 
 // Block-wide sum of an int (1024 threads).
@@ -267,6 +270,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
   __shared__ double s_d[MS_THREADS / 64];
   __shared__ int s_site[HS_MAX_UC];
   __shared__ int s_single;
+  __shared__ float s_single_mmu;
   const int tid = threadIdx.x;
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
@@ -376,9 +380,11 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
   for (int j = a0; j < a1; ++j) {
     const int c = single ? s_single : ws.cand[j];
     bool acc = false;
+    float mmu = 0.f;
     if (c >= 0) {
       const double e = (double)I.E[c] - (double)I.Ec[c];
       const double beta = (double)I.beta[c];
+      mmu = (float)(-(int)I.m[c]) * I.mu[c];  // (m mu) after a flip
       // exp(-switch_energy*beta) with switch_energy = -2E, relative error ~3e-7 (FP64 exponent
       // split, FP32 mantissa), clipped to [1e-10, 1e10]
       double p = hs_scaled_exp(1.f, 2.0 * e * beta, 1.f, 1.0);
@@ -391,9 +397,13 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
       acc = ua < p;
     }
     if (single) {
-      if (tid == 0) s_single = acc ? c : -1;
+      if (tid == 0) {
+        s_single = acc ? c : -1;
+        s_single_mmu = mmu;
+      }
     } else {
       ws.accf[j] = acc ? 1 : 0;
+      ws.cmmu[j] = mmu;
     }
     my_acc += acc;
   }
@@ -401,24 +411,21 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
   int total = 0;
   int pos = block_exclusive_scan(my_acc, s_i, total);
   __syncthreads();
+  // the accepted switches in selection order; the flips themselves (m *= -1, E *= -1,
+  // energies.py:124-126, 332-334) are applied by the update pass, by the tile holding the cell
   for (int j = a0; j < a1; ++j) {
     int c;
+    float mmu;
     if (single) {
       c = s_single;
+      mmu = s_single_mmu;
     } else {
       c = ws.accf[j] ? ws.cand[j] : -1;
+      mmu = ws.cmmu[j];
     }
     if (c < 0) continue;
-    const int mnew = -(int)I.m[c];
-    I.m[c] = (int8_t)mnew;
-    I.E[c] = -I.E[c];
-    I.Ec[c] = -I.Ec[c];
-    if (I.use_perp) {
-      I.Ep[c] = -I.Ep[c];
-      I.Epc[c] = -I.Epc[c];
-    }
     ws.acc_cell[pos] = c;
-    ws.acc_mmu[pos] = (float)mnew * I.mu[c];
+    ws.acc_mmu[pos] = mmu;
     atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
     if (R.log) log_cells[(size_t)S.steps_done * (size_t)(nsc > 0 ? nsc : 1) + pos] = c;
     ++pos;
@@ -499,7 +506,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
   float oxi = 0.f, oyi = 0.f;
   float acc = 0.f, accp = 0.f, ex = 0.f, exp_ = 0.f;
-  bool touched = false;
+  bool touched = false, flipped = false;
 
   // the contribution of one switched source (cell scell, (m mu) after its flip, unit-cell site ss)
   auto add_source = [&](int scell, float mm, int ss) {
@@ -580,7 +587,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         have = ws.accf[j] != 0;
         if (have) {
           sc = ws.cand[j];
-          smmu = (float)I.m[sc] * I.mu[sc];  // (m mu) after the flip, as in the selection order list
+          smmu = ws.cmmu[j];  // (m mu) after the flip, as in the selection order list
         }
       } else {
         have = true;
@@ -598,6 +605,8 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         }
         if (!rel && has_ex)
           rel = reaches(sx, hx, nx, I.pbc, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, I.pbc, y0, y0 + MU_TY - 1);
+        // a switch always reaches its own tile, which applies the flip
+        rel = rel || (sx >= x0 && sx < x0 + MU_TX && sy >= y0 && sy < y0 + MU_TY);
       }
     }
     const unsigned long long ball = __ballot(rel);
@@ -627,20 +636,32 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       }
     }
     if (!inside || mi == 0) continue;
-    for (int r = 0; r < nrel; ++r) add_source(s_cell[r], s_mmu[r], s_site_of[r]);
+    for (int r = 0; r < nrel; ++r) {
+      flipped = flipped || s_cell[r] == c;
+      add_source(s_cell[r], s_mmu[r], s_site_of[r]);
+    }
   }
-  if (!inside || mi == 0 || mi == 2 || !touched) return;
+  if (!inside || mi == 0 || mi == 2 || !(touched || flipped)) return;
+  if (flipped) {  // the flip first (E[w] *= -1 before the sum, energies.py:332-334)
+    mi = -mi;
+    I.m[c] = (int8_t)mi;
+  }
+  const float sgn = flipped ? -1.f : 1.f;
   const float mmui = (float)mi * I.mu[c];
   // E += 2*prefactor*(m mu)_i*conv (energies.py:362-363); exchange: -J m_i * 2 m_s * NN
-  const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
-  float e = I.E[c], ec = I.Ec[c];
-  hs_kahan_add(e, ec, de);
+  float e = sgn * I.E[c], ec = sgn * I.Ec[c];
+  if (touched) {
+    const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
+    hs_kahan_add(e, ec, de);
+  }
   I.E[c] = e;
   I.Ec[c] = ec;
   if (I.use_perp) {
-    const float dep = (float)(2.0 * I.dip_pf) * (mmui * accp) + (float)(-2.0 * I.J) * ((float)mi * exp_);
-    float p = I.Ep[c], pc = I.Epc[c];
-    hs_kahan_add(p, pc, dep);
+    float p = sgn * I.Ep[c], pc = sgn * I.Epc[c];
+    if (touched) {
+      const float dep = (float)(2.0 * I.dip_pf) * (mmui * accp) + (float)(-2.0 * I.J) * ((float)mi * exp_);
+      hs_kahan_add(p, pc, dep);
+    }
     I.Ep[c] = p;
     I.Epc[c] = pc;
   }
