@@ -1,22 +1,24 @@
 // Persistent multi-instance ("team") stepping: the Neel-Arrhenius step of many spin ices in
 // one cooperative launch.
 //
-// Every instance gets a team of G co-resident workgroups (1024 threads). Each thread owns up to
-// MAXC grid cells (c = rank*1024 + tid + k*G*1024) and keeps their spin, combined fields (+Kahan
-// compensations), E_B and moment in registers for the whole launch. Per step:
-//   1. keys log(tau) = log(-ln u) - log(rate) for the owned cells (Philox counter = cell, the
-//      streams of the single-instance path), packed as (key bits | position | spin) so one 64-bit
-//      DPP minimum per wave gives the lowest key with nanargmin's first-index tie break; the lane
-//      holding its wave's best leaves that cell's fields and draw in LDS;
-//   2. wave 0 reduces the 16 wave candidates and evaluates the FP64 waiting time of the
+// Every instance gets a team of G co-resident workgroups (1024 threads). The team's magnets are
+// the instance's occupied cells (occ_list, row-major): owner thread t < 512 of workgroup `rank`
+// holds list entries e = k*G*512 + rank*512 + t (k < MAXC) and keeps their spin, combined fields
+// (+Kahan compensations), E_B and moment in registers for the whole launch. Thread 512 + t is
+// owner t's drawer: it draws -ln u of the same magnets for the coming steps into LDS (Philox
+// counter = cell, the streams of the single-instance path), up to two steps ahead. Per step:
+//   1. owners form keys log(tau) = log(-ln u) - log(rate), packed as (key bits | position | spin)
+//      so one 64-bit DPP minimum per wave gives the lowest key with nanargmin's first-index tie
+//      break; the lane holding its wave's best leaves that cell's fields and draw in LDS;
+//   2. wave 0 reduces the 8 owner-wave candidates and evaluates the FP64 waiting time of the
 //      workgroup's best from its compensated fields. G == 1: that is the winner. G > 1: wave 0
 //      writes the workgroup's record {candidate, time}, both words tagged with the round, into
 //      the round's slot and polls (bounded spin, error on timeout) until all G records of the
 //      round carry the tag, then reduces them. Slots rotate over three rounds; a workgroup reads
 //      round k's records before it writes round k+1's, so no slot is overwritten while read.
-//      Wave 0 then advances time and counters (core.py:700-707). Meanwhile waves 1-15 draw
-//      -ln u for the coming steps of all the workgroup's cells;
-//   3. every thread applies the rank-1 field update of the winner (energies.py:310-329 +
+//      Wave 0 then advances time and counters (core.py:700-707). Meanwhile (G > 1) the drawers
+//      draw the next two steps;
+//   3. every owner applies the rank-1 field update of the winner (energies.py:310-329 +
 //      exchange stencil) to its registers, reading the winner's kernel row from L2.
 // State is written back once at the end. Teams never wait for each other.
 #include <hip/hip_runtime.h>
@@ -29,7 +31,9 @@
 
 namespace {
 
-constexpr int TT = 1024;  // threads per workgroup (16 waves: latency hiding)
+constexpr int TT = 1024;   // threads per workgroup
+constexpr int OW = TT / 2;  // owner threads (waves 0-7); thread OW + t draws for owner t
+constexpr int NBUF = 4;     // -ln(u) buffers: step s uses s % 4, drawers fill s+1 and s+2
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 // a packed candidate is key bits (32) | round tag (6) | cell (24) | m+1 (2)
 constexpr unsigned long long TAG_MASK = 63ull << 26;
@@ -37,8 +41,10 @@ constexpr unsigned long long NONE = ~0ull & ~TAG_MASK;  // "no candidate in this
 
 // One workgroup's contribution to a round: its best packed candidate and that cell's waiting
 // time (low 6 mantissa bits cleared), each word carrying the round tag, so a reader that sees the
-// tag in both words has the whole record even if the two words landed separately.
-struct alignas(16) TeamRec {
+// tag in both words has the whole record even if the two words landed separately. Each record has
+// a 128-byte line of its own: records that share a line serialise their device-scope stores and
+// polls (64 workgroups exchange in about 1.0 us with one line each, 1.6 us at 16 bytes apart).
+struct alignas(128) TeamRec {
   unsigned long long c, d;
 };
 
@@ -82,11 +88,11 @@ template <int MAXC, bool PERP>
 __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const int32_t *block_team, int32_t *err_out) {
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
-  __shared__ unsigned long long s_pc[TT / 64];
-  __shared__ float s_in[TT / 64][8];  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each wave's best
+  __shared__ unsigned long long s_pc[OW / 64];
+  __shared__ float s_in[OW / 64][8];  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each owner wave's best
   __shared__ unsigned long long s_win;
   __shared__ uint32_t s_ctl;
-  __shared__ float s_x[2][MAXC][TT];  // -ln(u) of the owned cells for the even / odd step
+  __shared__ float s_x[NBUF][MAXC][OW];  // -ln(u) of the owned magnets, step s in buffer s % 4
   const int tid = threadIdx.x;
   const int team = block_team[blockIdx.x];
   if (tid == 0) sT = teams[team];
@@ -101,7 +107,9 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   const hs_run_t R = sT.run;
   const int G = sT.G, rank = blockIdx.x - sT.first;
   const int lane = tid & 63, wv = tid >> 6;
-  const int nx = I.nx, ncell = I.nx * I.ny;
+  const bool owner = tid < OW;
+  const int ot = owner ? tid : tid - OW;  // the owner this thread is or draws for
+  const int nx = I.nx, nocc_i = I.n_occ;
   const int XB = nx > 1 ? 32 - __clz(nx - 1) : 0;  // positions y << XB | x order like y*nx + x
   const uint32_t XM = (1u << XB) - 1u;
   const int KW = 2 * I.kwx + 1;
@@ -110,19 +118,25 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   const double nocc = (double)I.n_occ;
   const float beta = I.beta[0];  // uniform over the grid (the host checks)
 
-  // register-resident state of the owned cells c_k = base + k*stride (valid for k < nval)
-  const int base = rank * TT + tid, stride = G * TT;
+  // register-resident state of the magnets of list entries e_k = base + k*stride (k < nval).
+  // Owners keep pos = y << XB | x and the fields; drawers keep the flat cell index in pos.
+  const int base = rank * OW + ot, stride = G * OW;
   int nval = 0;
   uint32_t pos[MAXC];
   float e[MAXC], ec[MAXC], ep[MAXC], epc[MAXC], eb[MAXC], mu[MAXC];
   uint32_t occ = 0, neg = 0, cand = 0;  // bit k: occupied, m = -1, Neel candidate
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
-    const int c = base + k * stride;
+    const int li = base + k * stride;
     pos[k] = 0;
     e[k] = ec[k] = ep[k] = epc[k] = eb[k] = mu[k] = 0.f;
-    if (c < ncell) {
+    if (li < nocc_i) {
       nval = k + 1;
+      const int c = I.occ_list[li];
+      if (!owner) {
+        pos[k] = (uint32_t)c;
+        continue;
+      }
       pos[k] = ((uint32_t)(c / nx) << XB) | (uint32_t)(c % nx);
       const int m = I.m[c];
       occ |= (m != 0 ? 1u : 0u) << k;
@@ -159,36 +173,50 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     att_stop = a;
   }
 
-  // -ln(u) draws. Philox call j serves cells 2j and 2j+1 (words 0,1 and 2,3), which sit in
-  // neighbouring lanes because base and stride are even. For an even step s the even lane
-  // evaluates call j at step s and the odd lane at step s+1, and each hands the other the half
-  // it needs (DPP swap of adjacent lanes): one Philox call per cell covers two steps. A single
-  // step (an odd first step) is drawn by both lanes evaluating the same call. `v` is the thread
-  // whose cells are drawn (any thread can draw any thread's cells; lane parity must match).
-  auto draw = [&](int64_t s, bool two, int v) {
-    const uint32_t odd = (uint32_t)v & 1u;
+  // -ln(u) draws by the drawers into buffers s % 4 (and (s+1) % 4 when `two`). Philox call j
+  // serves cells 2j and 2j+1 (words 0,1 and 2,3). When every drawn magnet of a wave has the other
+  // cell of its call in the neighbouring lane (fully occupied rows), the even lane evaluates call
+  // j at step s and the odd lane at step s+1 and each hands the other the half it needs (DPP swap
+  // of adjacent lanes): one call per magnet covers two steps. Otherwise every lane evaluates its
+  // own call for each step.
+  auto draw = [&](int64_t s, bool two) {
+    const uint32_t odd = (uint32_t)lane & 1u;
 #pragma unroll 1
     for (int k = 0; k < MAXC; ++k) {
-      const uint32_t j = (uint32_t)(rank * TT + v + k * stride) >> 1;
-      const uint64_t sk = (uint64_t)s + (two ? odd : 0u);
-      const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
-                                (uint32_t)(I.seed >> 32));
-      const uint32_t ma = odd ? W.z : W.x, mb = odd ? W.w : W.y;  // this lane's cell in this call
-      if (!two) {
-        s_x[s & 1][k][v] = hs_exp1_fast(ma, mb);
-      } else {
-        const uint32_t ra = hs_dpp_u32<0xB1>(0u, odd ? W.x : W.z);  // quad_perm [1,0,3,2]: lane ^ 1
+      const bool valid = k < nval;
+      const uint32_t c = valid ? pos[k] : 0u, j = c >> 1;
+      const uint32_t pj = hs_dpp_u32<0xB1>(~0u, valid ? j : ~0u);  // quad_perm [1,0,3,2]: lane ^ 1
+      const bool paired = valid && (c & 1u) == odd && pj == j;
+      if (two && __ballot(valid && !paired) == 0ull) {
+        const uint64_t sk = (uint64_t)s + odd;
+        const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
+                                  (uint32_t)(I.seed >> 32));
+        const uint32_t ma = odd ? W.z : W.x, mb = odd ? W.w : W.y;  // this lane's cell in this call
+        const uint32_t ra = hs_dpp_u32<0xB1>(0u, odd ? W.x : W.z);
         const uint32_t rb = hs_dpp_u32<0xB1>(0u, odd ? W.y : W.w);
-        s_x[0][k][v] = hs_exp1_fast(odd ? ra : ma, odd ? rb : mb);
-        s_x[1][k][v] = hs_exp1_fast(odd ? ma : ra, odd ? mb : rb);
+        if (valid) {
+          s_x[s & 3][k][ot] = hs_exp1_fast(odd ? ra : ma, odd ? rb : mb);
+          s_x[(s + 1) & 3][k][ot] = hs_exp1_fast(odd ? ma : ra, odd ? mb : rb);
+        }
+      } else if (valid) {
+        for (int q = 0; q < (two ? 2 : 1); ++q) {
+          const uint64_t sk = (uint64_t)s + q;
+          const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
+                                    (uint32_t)(I.seed >> 32));
+          s_x[sk & 3][k][ot] = (c & 1u) ? hs_exp1_fast(W.z, W.w) : hs_exp1_fast(W.x, W.y);
+        }
       }
     }
   };
 
   bool go = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
-  if (go) draw(step, (step & 1) == 0, tid);
+  if (go && !owner) draw(step, (step & 1) == 0);
+  __syncthreads();
   while (go) {
-    const int par = (int)(step & 1);
+    const int buf = (int)(step & 3);
+    // G == 1 has no exchange to hide the draws behind: the drawers fill steps s+1, s+2 while the
+    // owners work on step s (their buffers were last read at steps s-3, s-2)
+    if (!owner && G == 1 && (step & 1)) draw(step + 1, true);
     // 1. keys of the owned candidates, packed (key bits | position | m+1) so one 64-bit minimum
     //    picks the lowest key and, on ties, the lowest flat index (nanargmin's first index)
     unsigned long long best = ~0ull;
@@ -196,7 +224,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
       if ((cand >> k) & 1u) {
-        float key = hs_log_fast(s_x[par][k][tid]) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], beta);
+        float key = hs_log_fast(s_x[buf][k][ot]) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], beta);
         if (isnan(key)) key = INFINITY;
         const unsigned long long p =
             ((unsigned long long)key_bits(key) << 32) | (pos[k] << 2) | (uint32_t)(spin(k) + 1);
@@ -206,29 +234,31 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         }
       }
     }
-    // 2. the lane holding its wave's best leaves that cell's fields and draw for wave 0
-    const unsigned long long wbest = hs_wave_min_u64(best);
-    if (bk >= 0 && best == wbest) {
-      float se = 0.f, sec = 0.f, sep = 0.f, sepc = 0.f, seb = 0.f;
+    // 2. the lane holding its owner wave's best leaves that cell's fields and draw for wave 0
+    if (owner) {  // waves 0-7: wave-uniform, so the DPP minimum sees all lanes
+      const unsigned long long wbest = hs_wave_min_u64(best);
+      if (bk >= 0 && best == wbest) {
+        float se = 0.f, sec = 0.f, sep = 0.f, sepc = 0.f, seb = 0.f;
 #pragma unroll
-      for (int k = 0; k < MAXC; ++k) {
-        if (k == bk) {
-          se = e[k];
-          sec = ec[k];
-          sep = ep[k];
-          sepc = epc[k];
-          seb = eb[k];
+        for (int k = 0; k < MAXC; ++k) {
+          if (k == bk) {
+            se = e[k];
+            sec = ec[k];
+            sep = ep[k];
+            sepc = epc[k];
+            seb = eb[k];
+          }
         }
+        s_pc[wv] = wbest;
+        s_in[wv][0] = se;
+        s_in[wv][1] = sec;
+        s_in[wv][2] = sep;
+        s_in[wv][3] = sepc;
+        s_in[wv][4] = seb;
+        s_in[wv][5] = s_x[buf][bk][ot];
+      } else if (lane == 0 && wbest == ~0ull) {
+        s_pc[wv] = ~0ull;
       }
-      s_pc[wv] = wbest;
-      s_in[wv][0] = se;
-      s_in[wv][1] = sec;
-      s_in[wv][2] = sep;
-      s_in[wv][3] = sepc;
-      s_in[wv][4] = seb;
-      s_in[wv][5] = s_x[par][bk][tid];
-    } else if (lane == 0 && wbest == ~0ull) {
-      s_pc[wv] = ~0ull;
     }
     __syncthreads();
     // 3. wave 0: the workgroup's best; with G > 1 it writes the workgroup's record for this round
@@ -237,7 +267,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     //    the winner flips and whether another step follows.
     if (wv == 0) {
       unsigned long long src = (unsigned long long)lane, pd = 0ull;
-      unsigned long long pc = hs_wave_min_pay(lane < TT / 64 ? s_pc[lane] : ~0ull, src);
+      unsigned long long pc = hs_wave_min_pay(lane < OW / 64 ? s_pc[lane] : ~0ull, src);
       if (pc == ~0ull) {
         pc = NONE;
       } else {
@@ -304,19 +334,16 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         s_win = pc;
         s_ctl = ctl;
       }
-    } else if (step & 1) {
-      // meanwhile waves 1-15 draw the next (even) step and the one after for all 16 waves' cells
-      // (the keys of this step and wave 0's copy of the best cell's draw were read before the
-      // barrier above, so both buffers are free)
-      draw(step + 1, true, tid - 64);
-      if (wv == 1) draw(step + 1, true, tid + (TT - 128));
+    } else if (!owner && G > 1 && (step & 1)) {
+      // meanwhile the drawers fill the next two steps' buffers
+      draw(step + 1, true);
     }
     __syncthreads();
     const uint32_t ctl = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ctl);
     if (ctl & 4u) break;
     step += 1;
     go = (ctl & 2u) != 0;
-    if (!(ctl & 1u)) continue;
+    if (!(ctl & 1u) || !owner) continue;
     // 4. rank-1 field update of the winner (energies.py:310-329 + exchange stencil)
     const unsigned long long win = s_win;
     const uint32_t wpos = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)(win >> 2) & 0xFFFFFFu));
@@ -401,8 +428,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   // write back
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
-    if (k >= nval) continue;
-    const int c = base + k * stride;
+    if (!owner || k >= nval) continue;
+    const int c = (int)(pos[k] >> XB) * nx + (int)(pos[k] & XM);
     I.m[c] = (int8_t)spin(k);
     I.E[c] = e[k];
     I.Ec[c] = ec[k];
@@ -448,10 +475,9 @@ struct Layout {
 };
 
 int team_blocks(const hs_team_t &t) {
-  const int ncell = t.inst->nx * t.inst->ny;
   if (t.nblocks > 0) return t.nblocks;
-  // default: one cell per thread, at most 128 workgroups
-  int g = (ncell + TT - 1) / TT;
+  // default: one magnet per owner thread, at most 128 workgroups
+  int g = (t.inst->n_occ + OW - 1) / OW;
   return g < 1 ? 1 : (g > 128 ? 128 : g);
 }
 
@@ -468,8 +494,8 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
     L.G[i] = g;
     L.first[i] = L.nblocks;
     L.nblocks += g;
-    const int ncell = teams[i].inst->nx * teams[i].inst->ny;
-    const int c = (ncell + g * TT - 1) / (g * TT);
+    const int64_t per = (int64_t)g * OW;
+    const int c = (int)((teams[i].inst->n_occ + per - 1) / per);
     while (L.maxc < c) L.maxc *= 2;
   }
   L.sync_off = L.map_off + al256(sizeof(int32_t) * (size_t)L.nblocks);
@@ -521,7 +547,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   Layout L;
   make_layout(teams, nteams, L);
   if (ws_bytes < (int64_t)L.total) return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)L.total);
-  if (L.maxc > 16) return hs_set_error(HS_ERR_ARG, "more than 16 cells per thread: give the team more workgroups");
+  if (L.maxc > 16) return hs_set_error(HS_ERR_ARG, "more than 16 magnets per owner thread: give the team more workgroups");
   for (int i = 0; i < nteams; ++i) {
     // packed candidates hold the position y << ceil(log2 nx) | x in 24 bits
     const hs_instance_t &in = *teams[i].inst;

@@ -23,14 +23,13 @@ def _key(mm):
 
 
 def neel_candidates(mm, max_workgroups=256):
-    """Team sizes worth trying: from one cell per thread (ncell/1024) down to 16 cells per thread,
-    at most the co-resident limit."""
-    ncell = mm.nx * mm.ny
-    base = max(1, math.ceil(ncell / 1024))
+    """Team sizes worth trying: around one magnet per owner thread (n/512 workgroups) down to 16
+    magnets per owner thread, at most the co-resident limit."""
+    base = max(1, math.ceil(mm.n / 512))
     out = []
     for f in (2, 1, 0.75, 0.5, 0.25):
         g = max(1, min(max_workgroups, int(round(base * f))))
-        if math.ceil(ncell / (g * 1024)) <= 16 and g not in out:
+        if math.ceil(mm.n / (g * 512)) <= 16 and g not in out:
             out.append(g)
     return out or [max(1, min(max_workgroups, base))]
 

@@ -43,7 +43,7 @@ def _same(a, b):
             assert x == y
 
 
-@pytest.mark.parametrize("make,n", [(_pinwheel, 32), (_oop, 24)])
+@pytest.mark.parametrize("make,n", [(_pinwheel, 32), (_oop, 24), (_oop, 23)])
 @pytest.mark.parametrize("nblocks", [1, 2, 5, 0])
 def test_team_equals_per_step_launches(make, n, nblocks):
     a = make(n, 3, recompute_every=None)
