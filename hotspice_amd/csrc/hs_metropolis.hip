@@ -7,7 +7,7 @@
 //     clipped to [1e-10, 1e10] (core.py:666, 671), optional METROPOLIS_TIME (core.py:667-669),
 //     the flips with E[w] *= -1 (energies.py:124-126, 332-334), and the order-preserving list of
 //     accepted switches with per-sublattice counts (energies.py:335-343);
-//   k_metro_update (one workgroup per 32x8 output tile of a job): the sparse-source correlation
+//   k_metro_update (32x8 output tiles): the sparse-source correlation
 //     of the accepted switches with the dipolar kernel -- full window, or the (2k+1)^2
 //     REDUCED_KERNEL_SIZE window with fill/wrap boundary when a sublattice has more than
 //     SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF switches (energies.py:343-361) -- plus the
@@ -16,6 +16,10 @@
 //     whose cells lie within the window of the tile), or, where that is not shorter (full
 //     kernels, PBC, single-sample steps), by filtering the whole list through LDS. Both visit
 //     the sources in selection order, so every cell sums the same terms in the same order.
+//     The selection marks the tiles that the windows of its accepted switches reach (and the
+//     tile of each switch, which applies its flip) and appends them to a worklist; the update
+//     launch is a fixed grid that works through that list, so a step costs in proportion to the
+//     tiles it changes, not to the grid.
 // A batch runs every job's step in the same two launches; each job keeps its own stop
 // predicate, so a job alone and a job inside a batch execute identical arithmetic.
 #include <hip/hip_runtime.h>
@@ -34,6 +38,10 @@ constexpr int MU_TX = 32, MU_TY = 8, MU_THREADS = MU_TX * MU_TY;
 constexpr int MAX_ATTEMPTS = 4096;
 // instances up to this many grid cells step in the persistent one-workgroup kernel
 constexpr int PERSIST_CELLS = 80 * 80;
+// workgroups of an update launch over the worklist (8 per CU of a 256-CU device)
+constexpr int UPDATE_WGS = 2048;
+// a switch whose window spans more tiles than this lists every tile of its job instead
+constexpr int MARK_MAX_TILES = 256;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -48,6 +56,15 @@ struct MetroWS {
   uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
   int32_t *pwin;     // [nsc] Poisson: 1 = this round's candidate is accepted
   float *cmmu;       // [nsc] (m mu) of each candidate after its flip (read where accf is set)
+  uint8_t *tmark;    // [ntiles] 1 = tile reached by a switch of this step (cleared when listed)
+};
+
+// The step's (job, tile) worklist of an update launch: count[par] entries in wl. The selection
+// of step parity `par` appends; the update of that step reads them and zeroes count[par ^ 1]
+// for the next step.
+struct WorkList {
+  int2 *wl;
+  int32_t *count;
 };
 
 // Everything one job's step needs (passed by value for one instance, from an array for a batch).
@@ -63,7 +80,7 @@ struct MetroJob {
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-MetroWS carve(void *base, int nsc) {
+MetroWS carve(void *base, int nsc, int ntiles) {
   char *p = (char *)base;
   MetroWS w;
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
@@ -95,14 +112,22 @@ MetroWS carve(void *base, int nsc) {
   p += align256(n * 4);
   w.cmmu = (float *)p;
   p += align256(n * 4);
+  w.tmark = (uint8_t *)p;  // padded to whole 16-byte chunks (align256)
+  p += align256((size_t)(ntiles > 0 ? ntiles : 1));
   return w;
 }
 
-size_t ws_bytes(int nsc) {
+// Per-job scratch (the MetroWS arrays).
+size_t ws_bytes(int nsc, int ntiles) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
   return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
-         7 * align256(n * 4);
+         7 * align256(n * 4) + align256((size_t)(ntiles > 0 ? ntiles : 1));
 }
+
+// A worklist for `ntiles` tiles: the entries and the two step counters.
+size_t wl_bytes(int64_t ntiles) { return align256(8 * (size_t)(ntiles > 0 ? ntiles : 1)) + 256; }
+
+int tiles_of(const hs_instance_t &in) { return ((in.nx + MU_TX - 1) / MU_TX) * ((in.ny + MU_TY - 1) / MU_TY); }
 
 // Block-wide sum of an int (1024 threads).
 HS_INLINE int block_sum_int(int v, int *s_tmp) {
@@ -258,7 +283,41 @@ __device__ int poisson_select(const hs_instance_t &I, const hs_select_t &SEL, co
   return block_sum_int(val, s_tmp);
 }
 
-__device__ __forceinline__ bool metro_select(const MetroJob &J) {
+// Tile ranges [t0, t1] (up to three, the images k*n with wrap) of the cells within w of s along
+// one axis of n cells and tiles of T cells: exactly the tiles for which metro_update's
+// reaches(s, w, n, wrap, ...) holds with an actual cell in reach.
+HS_INLINE int tile_ranges(int s, int w, int n, bool wrap, int T, int t0[3], int t1[3]) {
+  int cnt = 0;
+  if (w < 0) return 0;
+  for (int k = wrap ? -1 : 0; k <= (wrap ? 1 : 0); ++k) {
+    const int lo = max(0, s + k * n - w), hi = min(n - 1, s + k * n + w);
+    if (lo > hi) continue;
+    t0[cnt] = lo / T;
+    t1[cnt] = hi / T;
+    ++cnt;
+  }
+  return cnt;
+}
+
+// Marks the tiles of the rectangle (sx +- wx) x (sy +- wy); returns false (marking nothing) when
+// it spans more than MARK_MAX_TILES tiles.
+HS_INLINE bool mark_rect(uint8_t *tmark, int tiles_x, int nx, int ny, int sx, int wx, int sy, int wy, bool wrap) {
+  int x0[3], x1[3], y0[3], y1[3];
+  const int nxr = tile_ranges(sx, wx, nx, wrap, MU_TX, x0, x1);
+  const int nyr = tile_ranges(sy, wy, ny, wrap, MU_TY, y0, y1);
+  int span = 0;
+  for (int a = 0; a < nyr; ++a)
+    for (int b = 0; b < nxr; ++b) span += (y1[a] - y0[a] + 1) * (x1[b] - x0[b] + 1);
+  if (span > MARK_MAX_TILES) return false;
+  for (int a = 0; a < nyr; ++a)
+    for (int ty = y0[a]; ty <= y1[a]; ++ty)
+      for (int b = 0; b < nxr; ++b)
+        for (int tx = x0[b]; tx <= x1[b]; ++tx) tmark[ty * tiles_x + tx] = 1;
+  return true;
+}
+
+__device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
+                                             int job = 0) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const hs_run_t &R = J.R;
@@ -271,6 +330,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
   __shared__ int s_site[HS_MAX_UC];
   __shared__ int s_single;
   __shared__ float s_single_mmu;
+  __shared__ int s_all, s_base;
   const int tid = threadIdx.x;
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
@@ -436,6 +496,58 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J) {
     // truncates per sublattice above the cutoff when REDUCED_KERNEL_SIZE != 0 (energies.py:343-349)
     ws.trunc[tid] = (total > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
   }
+  if (WL.wl && total > 0) {
+    // the tiles this step changes: the windows metro_update adds (dipolar: stored or truncated
+    // window, wrapped when truncated with PBC; exchange: the NN stencil, wrapped with PBC) and the
+    // tile of every switch (its flip), appended once each to the worklist
+    const int tiles_x = J.tiles_x, ntile = J.tiles_x * J.tiles_y;
+    const bool has_dip = I.dip_pf != 0.0, has_ex = I.J != 0.0 && I.nn_w > 0;
+    const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
+    if (tid == 0) s_all = 0;
+    __syncthreads();
+    for (int j = a0; j < a1; ++j) {
+      const int c = single ? s_single : (ws.accf[j] ? ws.cand[j] : -1);
+      if (c < 0) continue;
+      const int sy = c / nx, sx = c % nx, ss = hs_site(I, sy, sx);
+      ws.tmark[(sy / MU_TY) * tiles_x + sx / MU_TX] = 1;
+      bool ok = true;
+      if (has_dip && I.sub_of[ss] >= 0) {
+        const bool tr = total > 1 && SEL.reduced_k != 0 && s_site[ss] > SEL.cutoff;
+        ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, tr ? tkx : I.kwx, sy, tr ? tky : I.kwy, tr && I.pbc);
+      }
+      if (ok && has_ex) ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, I.nn_w / 2, sy, I.nn_h / 2, I.pbc != 0);
+      if (!ok) s_all = 1;
+    }
+    __syncthreads();
+    // 16 marks per load (tmark is 256-byte aligned and padded); bytes past the last tile never count
+    const bool all = s_all != 0;
+    const int nchunk = (ntile + 15) / 16;
+    auto chunk_bits = [&](int ch) {  // bit b: tile 16*ch + b is listed
+      const uint4 v = *(const uint4 *)(ws.tmark + 16 * ch);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      uint32_t bits = 0;
+      for (int q = 0; q < 16; ++q) bits |= ((w[q >> 2] >> (8 * (q & 3))) & 0xffu) != 0 ? (1u << q) : 0u;
+      if (all) bits = 0xffffu;
+      const int valid = ntile - 16 * ch;
+      return valid >= 16 ? bits : bits & ((1u << valid) - 1u);
+    };
+    int mine = 0;
+    for (int ch = tid; ch < nchunk; ch += MS_THREADS) mine += __popc(chunk_bits(ch));
+    int nlist = 0;
+    int off = block_exclusive_scan(mine, s_i, nlist);
+    if (tid == 0) s_base = atomicAdd(&WL.count[par], nlist);
+    __syncthreads();
+    off += s_base;
+    for (int ch = tid; ch < nchunk; ch += MS_THREADS) {
+      uint32_t bits = chunk_bits(ch);
+      *(uint4 *)(ws.tmark + 16 * ch) = make_uint4(0u, 0u, 0u, 0u);
+      while (bits) {
+        const int b = __ffs((int)bits) - 1;
+        bits &= bits - 1u;
+        WL.wl[off++] = make_int2(job, 16 * ch + b);
+      }
+    }
+  }
   if (tid == 0) {
     ws.info[0] = ox;
     ws.info[1] = oy;
@@ -479,7 +591,10 @@ HS_INLINE bool reaches(int s, int w, int n, bool wrap, int lo, int hi) {
 HS_INLINE int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 // Update of one TX x TY output tile by a workgroup of TX*TY threads (one cell per thread).
-template <int TX, int TY>
+// PRELOAD: load the cell's own state (m, mu, fields) at the start, with the step's counts, for
+// tiles known to be reached (the worklist); otherwise only once a source reaches the tile (the
+// persistent kernel visits every tile and keeps its registers for the selection).
+template <int TX, int TY, bool PRELOAD>
 __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int tile_y) {
   constexpr int MU_TX = TX, MU_TY = TY, MU_THREADS = TX * TY;
   const hs_instance_t &I = J.I;
@@ -489,15 +604,51 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   __shared__ float s_mmu[MU_THREADS];
   __shared__ int s_site_of[MU_THREADS];
   __shared__ int s_cnt[MU_THREADS / 64 + 1];
-  const int a = *ws.acc_n;
-  if (a == 0) return;
+  __shared__ int s_sub[HS_MAX_UC], s_trunc[HS_MAX_UC];
+  __shared__ float s_oxu[HS_MAX_UC], s_oyu[HS_MAX_UC], s_nn[HS_MAX_NN];
   const int tid = threadIdx.x;
   const int x0 = tile_x * MU_TX, y0 = tile_y * MU_TY;
   const int x = x0 + (tid % MU_TX), y = y0 + (tid / MU_TX);
   const int nx = I.nx, ny = I.ny;
   const bool inside = x < nx && y < ny;
   const int c = inside ? y * nx + x : 0;
-  int mi = 2;  // the cell's spin, loaded once some source reaches the tile (2 = not loaded)
+  // everything that does not depend on the step's sources is loaded up front, together: the
+  // step's counts, the per-site tables and the cell's own state
+  const int a = *ws.acc_n;
+  int info_ox = 0, info_oy = 0, info_single = 0;
+  if (PRELOAD) {
+    info_ox = ws.info[0];
+    info_oy = ws.info[1];
+    info_single = ws.info[2];
+  }
+  if (PRELOAD && tid < I.ux * I.uy) {
+    s_sub[tid] = I.sub_of[tid];
+    s_trunc[tid] = ws.trunc[tid];
+    s_oxu[tid] = (float)I.ox_uc[tid];
+    s_oyu[tid] = (float)I.oy_uc[tid];
+  }
+  if (PRELOAD && tid < I.nn_w * I.nn_h) s_nn[tid] = I.nn_mask[tid];
+  // per-site tables: from LDS when preloaded (short dependency chains), else straight from the
+  // descriptor (fewer live registers in the persistent kernel)
+  auto sub_of = [&](int ss) { return PRELOAD ? s_sub[ss] : I.sub_of[ss]; };
+  auto trunc_of = [&](int ss) { return PRELOAD ? s_trunc[ss] : ws.trunc[ss]; };
+  auto ox_of = [&](int ss) { return PRELOAD ? s_oxu[ss] : (float)I.ox_uc[ss]; };
+  auto oy_of = [&](int ss) { return PRELOAD ? s_oyu[ss] : (float)I.oy_uc[ss]; };
+  int m0 = 0;
+  float mu0 = 0.f, E0 = 0.f, Ec0 = 0.f, Ep0 = 0.f, Epc0 = 0.f;
+  if (PRELOAD && inside) {
+    m0 = (int)I.m[c];
+    mu0 = I.mu[c];
+    E0 = I.E[c];
+    Ec0 = I.Ec[c];
+    if (I.use_perp) {
+      Ep0 = I.Ep[c];
+      Epc0 = I.Epc[c];
+    }
+  }
+  if (a == 0) return;
+  if (PRELOAD) __syncthreads();
+  int mi = 2;  // the cell's spin, taken once some source reaches the tile (2 = not yet)
   const bool has_dip = I.dip_pf != 0.0;
   const int hy = I.nn_h / 2, hx = I.nn_w / 2;
   const bool has_ex = I.J != 0.0 && I.nn_w > 0;
@@ -512,9 +663,9 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   auto add_source = [&](int scell, float mm, int ss) {
     const int sy = scell / nx, sx = scell % nx;
     if (has_dip) {
-      const int n = I.sub_of[ss];
+      const int n = sub_of(ss);
       if (n >= 0) {
-        const bool tr = ws.trunc[ss] != 0;
+        const bool tr = trunc_of(ss) != 0;
         const bool wrap = tr && I.pbc;
         const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
         int dys[3], dxs[3];
@@ -540,11 +691,12 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         dx = dx > nx / 2 ? dx - nx : dx;
       }
       if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
-        const float wq = I.nn_mask[(dy + hy) * I.nn_w + (dx + hx)];
+        const int q = (dy + hy) * I.nn_w + (dx + hx);
+        const float wq = PRELOAD ? s_nn[q] : I.nn_mask[q];
         if (wq != 0.f) {
           const float msn = mm > 0.f ? 1.f : -1.f;  // m of the source after its flip
           if (I.in_plane) {
-            const float oxs = I.ox_uc[ss], oys = I.oy_uc[ss];
+            const float oxs = ox_of(ss), oys = oy_of(ss);
             ex += msn * wq * (oxi * oxs + oyi * oys);
             exp_ += msn * wq * (oys * oxi - oxs * oyi);
           } else {
@@ -561,8 +713,9 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   // within the largest window of the tile can reach it.
   bool lookup = false;
   int rs0 = 0, rs1 = -1, cs0 = 0, cs1 = -1;
-  if (SEL.scheme == 0 && !I.pbc && ws.info[2] == 0) {
-    const int ox = ws.info[0], oy = ws.info[1];
+  if (!PRELOAD) info_single = ws.info[2];
+  if (SEL.scheme == 0 && !I.pbc && info_single == 0) {
+    const int ox = PRELOAD ? info_ox : ws.info[0], oy = PRELOAD ? info_oy : ws.info[1];
     const int Wx = max(has_dip ? I.kwx : 0, has_ex ? hx : 0), Wy = max(has_dip ? I.kwy : 0, has_ex ? hy : 0);
     const int nrows = SEL.nsc / SEL.ncols;
     cs0 = max(0, -floor_div(-(x0 - Wx - ox - SEL.Rx + 1), 2 * SEL.Rx));
@@ -584,10 +737,12 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       bool have;
       if (lookup) {
         const int j = (rs0 + sidx / ncs) * SEL.ncols + cs0 + sidx % ncs;
-        have = ws.accf[j] != 0;
+        const int af = ws.accf[j], cj = ws.cand[j];  // loaded together (cand/cmmu are set for every j)
+        const float mj = ws.cmmu[j];
+        have = af != 0;
         if (have) {
-          sc = ws.cand[j];
-          smmu = ws.cmmu[j];  // (m mu) after the flip, as in the selection order list
+          sc = cj;
+          smmu = mj;  // (m mu) after the flip, as in the selection order list
         }
       } else {
         have = true;
@@ -597,8 +752,8 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       if (have) {
         const int sy = sc / nx, sx = sc % nx;
         ssite = hs_site(I, sy, sx);
-        if (has_dip && I.sub_of[ssite] >= 0) {
-          const bool tr = ws.trunc[ssite] != 0;
+        if (has_dip && sub_of(ssite) >= 0) {
+          const bool tr = trunc_of(ssite) != 0;
           const int wx = tr ? tkx : I.kwx, wy = tr ? tky : I.kwy;
           const bool wrap = tr && I.pbc;
           rel = reaches(sx, wx, nx, wrap, x0, x0 + MU_TX - 1) && reaches(sy, wy, ny, wrap, y0, y0 + MU_TY - 1);
@@ -627,12 +782,12 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
     }
     __syncthreads();
     if (nrel == 0) continue;
-    if (mi == 2) {  // first chunk with sources: load the cell's own spin and site
-      mi = inside ? (int)I.m[c] : 0;
+    if (mi == 2) {  // first chunk with sources: the cell's own spin and site
+      mi = PRELOAD ? m0 : (inside ? (int)I.m[c] : 0);
       if (mi != 0) {
         const int site_i = hs_site(I, y, x);
-        oxi = I.ox_uc[site_i];
-        oyi = I.oy_uc[site_i];
+        oxi = ox_of(site_i);
+        oyi = oy_of(site_i);
       }
     }
     if (!inside || mi == 0) continue;
@@ -647,9 +802,14 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
     I.m[c] = (int8_t)mi;
   }
   const float sgn = flipped ? -1.f : 1.f;
-  const float mmui = (float)mi * I.mu[c];
+  if (!PRELOAD) {
+    mu0 = I.mu[c];
+    E0 = I.E[c];
+    Ec0 = I.Ec[c];
+  }
+  const float mmui = (float)mi * mu0;
   // E += 2*prefactor*(m mu)_i*conv (energies.py:362-363); exchange: -J m_i * 2 m_s * NN
-  float e = sgn * I.E[c], ec = sgn * I.Ec[c];
+  float e = sgn * E0, ec = sgn * Ec0;
   if (touched) {
     const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
     hs_kahan_add(e, ec, de);
@@ -657,7 +817,11 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   I.E[c] = e;
   I.Ec[c] = ec;
   if (I.use_perp) {
-    float p = sgn * I.Ep[c], pc = sgn * I.Epc[c];
+    if (!PRELOAD) {
+      Ep0 = I.Ep[c];
+      Epc0 = I.Epc[c];
+    }
+    float p = sgn * Ep0, pc = sgn * Epc0;
     if (touched) {
       const float dep = (float)(2.0 * I.dip_pf) * (mmui * accp) + (float)(-2.0 * I.J) * ((float)mi * exp_);
       hs_kahan_add(p, pc, dep);
@@ -667,26 +831,56 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   }
 }
 
-__global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J) { metro_select(J); }
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J, WorkList WL, int par) {
+  metro_select(J, WL, par, 0);
+}
 
 // One selection workgroup per job of the list `which`.
 __global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob *__restrict__ jobs,
-                                                                  const int32_t *__restrict__ which) {
-  metro_select(jobs[which[blockIdx.x]]);
+                                                                  const int32_t *__restrict__ which, WorkList WL,
+                                                                  int par) {
+  const int j = which[blockIdx.x];
+  metro_select(jobs[j], WL, par, j);
 }
 
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J) {
-  metro_update<MU_TX, MU_TY>(J, blockIdx.x, blockIdx.y);
+// The update of the step's listed tiles (one tile at a time per workgroup); then the counter of
+// the next step is cleared.
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J, WorkList WL, int par) {
+  const int n = WL.count[par];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int t = WL.wl[i].y;
+    metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
 }
 
-// One workgroup per tile of every job: tile_job[b] is the job of block b, tile_first[j] its first block.
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob *__restrict__ jobs,
-                                                                  const int32_t *__restrict__ tile_job,
-                                                                  const int32_t *__restrict__ tile_first) {
-  const int jb = tile_job[blockIdx.x];
-  const MetroJob &J = jobs[jb];
-  const int t = blockIdx.x - tile_first[jb];
-  metro_update<MU_TX, MU_TY>(J, t % J.tiles_x, t / J.tiles_x);
+__global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob *__restrict__ jobs, WorkList WL,
+                                                                  int par) {
+  const int n = WL.count[par];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int2 e = WL.wl[i];
+    const MetroJob &J = jobs[e.x];
+    metro_update<MU_TX, MU_TY, true>(J, e.y % J.tiles_x, e.y / J.tiles_x);
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
+}
+
+// Clears the tile marks of every job and both worklist counters (any workspace state left by an
+// earlier call is discarded).
+__global__ void k_metro_clear_one(MetroJob J, WorkList WL) {
+  const int nt = (J.tiles_x * J.tiles_y + 15) & ~15;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) J.ws.tmark[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 2) WL.count[threadIdx.x] = 0;
+}
+
+__global__ void k_metro_clear_many(const MetroJob *__restrict__ jobs, int njobs, WorkList WL) {
+  for (int j = 0; j < njobs; ++j) {
+    const int nt = (jobs[j].tiles_x * jobs[j].tiles_y + 15) & ~15;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) jobs[j].ws.tmark[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) WL.count[threadIdx.x] = 0;
 }
 
 // Persistent Metropolis stepping of small instances: one 1024-thread workgroup per job runs the
@@ -699,7 +893,7 @@ __device__ __forceinline__ void metro_persist(const MetroJob &J) {
     const bool stepped = metro_select(J);
     __syncthreads();
     if (!stepped) break;
-    for (int t = 0; t < tx * ty; ++t) metro_update<32, 32>(J, t % tx, t / tx);
+    for (int t = 0; t < tx * ty; ++t) metro_update<32, 32, false>(J, t % tx, t / tx);
     __syncthreads();
   }
 }
@@ -750,7 +944,7 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
   J.SEL.nsc = sel->scheme != 1 ? sel->nsc : 1;
   J.R = *run;
   J.st = state;
-  J.ws = carve(ws, J.SEL.nsc);
+  J.ws = carve(ws, J.SEL.nsc, tiles_of(*inst));
   J.log_cells = log_cells;
   J.log_count = log_count;
   J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
@@ -761,9 +955,9 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
 }  // namespace
 
 extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
-  (void)inst;
-  if (!sel) return -1;
-  return (int64_t)ws_bytes(sel->scheme != 1 ? sel->nsc : 1);
+  if (!sel || !inst || inst->nx < 1 || inst->ny < 1) return -1;
+  const int nt = tiles_of(*inst);
+  return (int64_t)(ws_bytes(sel->scheme != 1 ? sel->nsc : 1, nt) + wl_bytes(nt));
 }
 
 extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
@@ -785,11 +979,17 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
     HS_LAUNCH_CHECK("k_metro_persist_one");
     return HS_OK;
   }
-  const dim3 grid(J.tiles_x, J.tiles_y);
+  const int nt = J.tiles_x * J.tiles_y;
+  char *wlp = (char *)ws + ws_bytes(J.SEL.nsc, nt);
+  const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
+  k_metro_clear_one<<<(nt + 255) / 256 < 64 ? (nt + 255) / 256 : 64, 256, 0, s>>>(J, WL);
+  HS_LAUNCH_CHECK("k_metro_clear_one");
+  const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
   for (int64_t j = 0; j < run->max_steps; ++j) {
-    k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J);
+    const int par = (int)(j & 1);
+    k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J, WL, par);
     HS_LAUNCH_CHECK("k_metro_select");
-    k_metro_update_one<<<grid, MU_THREADS, 0, s>>>(J);
+    k_metro_update_one<<<wgs, MU_THREADS, 0, s>>>(J, WL, par);
     HS_LAUNCH_CHECK("k_metro_update");
   }
   return HS_OK;
@@ -798,10 +998,10 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
 namespace {
 
 struct MetroLayout {
-  size_t jobs_off, tile_job_off, tile_first_off, small_off, large_off, total;
+  size_t jobs_off, small_off, large_off, wl_off, total;
   std::vector<size_t> ws_off;
-  std::vector<int32_t> first, small, large;
-  int32_t ntiles;
+  std::vector<int32_t> small, large;
+  int64_t ntiles;  // tiles of the two-launch (large) jobs
 };
 
 bool persistent(const hs_instance_t &in) { return in.nx * in.ny <= PERSIST_CELLS; }
@@ -812,30 +1012,26 @@ bool persistent(const hs_instance_t &in) { return in.nx * in.ny <= PERSIST_CELLS
 void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
   L.jobs_off = 0;
   L.ntiles = 0;
-  L.first.assign(n, 0);
   L.small.clear();
   L.large.clear();
   bool all_small = true;
   for (int i = 0; i < n; ++i) all_small = all_small && persistent(*teams[i].inst);
   for (int i = 0; i < n; ++i) {
-    const hs_instance_t &in = *teams[i].inst;
     if (all_small) {
       L.small.push_back(i);
       continue;
     }
     L.large.push_back(i);
-    L.first[i] = L.ntiles;
-    L.ntiles += ((in.nx + MU_TX - 1) / MU_TX) * ((in.ny + MU_TY - 1) / MU_TY);
+    L.ntiles += tiles_of(*teams[i].inst);
   }
-  L.tile_job_off = align256(sizeof(MetroJob) * (size_t)n);
-  L.tile_first_off = L.tile_job_off + align256(4 * (size_t)(L.ntiles > 0 ? L.ntiles : 1));
-  L.small_off = L.tile_first_off + align256(4 * (size_t)n);
+  L.small_off = align256(sizeof(MetroJob) * (size_t)n);
   L.large_off = L.small_off + align256(4 * (size_t)n);
-  size_t off = L.large_off + align256(4 * (size_t)n);
+  L.wl_off = L.large_off + align256(4 * (size_t)n);
+  size_t off = L.wl_off + wl_bytes(L.ntiles);
   L.ws_off.resize(n);
   for (int i = 0; i < n; ++i) {
     L.ws_off[i] = off;
-    off += ws_bytes(teams[i].sel->scheme != 1 ? teams[i].sel->nsc : 1);
+    off += ws_bytes(teams[i].sel->scheme != 1 ? teams[i].sel->nsc : 1, tiles_of(*teams[i].inst));
   }
   L.total = off;
 }
@@ -871,20 +1067,15 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   if (ws_bytes_ < (int64_t)L.total)
     return hs_set_error(HS_ERR_ARG, "workspace %lld < %lld bytes", (long long)ws_bytes_, (long long)L.total);
   char *base = (char *)ws;
-  std::vector<char> host(L.large_off + align256(4 * (size_t)nteams), 0);
+  std::vector<char> host(L.wl_off, 0);
   MetroJob *jobs = (MetroJob *)(host.data() + L.jobs_off);
-  int32_t *tile_job = (int32_t *)(host.data() + L.tile_job_off);
-  int32_t *tile_first = (int32_t *)(host.data() + L.tile_first_off);
   int32_t *small = (int32_t *)(host.data() + L.small_off);
   int64_t large_steps = 0;
   for (int i = 0; i < nteams; ++i) {
     const hs_team_t &t = teams[i];
     jobs[i] = make_job(t.inst, t.sel, &t.run, t.state, base + L.ws_off[i], t.log_cells, t.log_idx);
-    tile_first[i] = L.first[i];
     if (L.large.empty() || std::find(L.large.begin(), L.large.end(), i) == L.large.end()) continue;
     large_steps = t.run.max_steps > large_steps ? t.run.max_steps : large_steps;
-    const int nt = jobs[i].tiles_x * jobs[i].tiles_y;
-    for (int b = 0; b < nt; ++b) tile_job[L.first[i] + b] = i;
   }
   for (size_t k = 0; k < L.small.size(); ++k) small[k] = L.small[k];
   int32_t *large = (int32_t *)(host.data() + L.large_off);
@@ -893,8 +1084,6 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
   const MetroJob *dj = (const MetroJob *)(base + L.jobs_off);
-  const int32_t *dtj = (const int32_t *)(base + L.tile_job_off);
-  const int32_t *dtf = (const int32_t *)(base + L.tile_first_off);
   k_state_begin_many<<<(nteams + 255) / 256, 256, 0, s>>>(dj, nteams);
   HS_LAUNCH_CHECK("k_state_begin_many");
   if (!L.small.empty()) {  // a batch of small instances: one persistent workgroup each
@@ -902,11 +1091,20 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
     HS_LAUNCH_CHECK("k_metro_persist_many");
   }
   steps = L.ntiles > 0 ? large_steps : 0;
-  for (int64_t j = 0; j < steps; ++j) {
-    k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, (const int32_t *)(base + L.large_off));
-    HS_LAUNCH_CHECK("k_metro_select_many");
-    k_metro_update_many<<<L.ntiles, MU_THREADS, 0, s>>>(dj, dtj, dtf);
-    HS_LAUNCH_CHECK("k_metro_update_many");
+  if (steps > 0) {
+    char *wlp = base + L.wl_off;
+    const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)L.ntiles))};
+    k_metro_clear_many<<<64, 256, 0, s>>>(dj, nteams, WL);
+    HS_LAUNCH_CHECK("k_metro_clear_many");
+    const unsigned wgs = (unsigned)(L.ntiles < UPDATE_WGS ? L.ntiles : UPDATE_WGS);
+    for (int64_t j = 0; j < steps; ++j) {
+      const int par = (int)(j & 1);
+      k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, (const int32_t *)(base + L.large_off),
+                                                                            WL, par);
+      HS_LAUNCH_CHECK("k_metro_select_many");
+      k_metro_update_many<<<wgs, MU_THREADS, 0, s>>>(dj, WL, par);
+      HS_LAUNCH_CHECK("k_metro_update_many");
+    }
   }
   return HS_OK;
 }

@@ -27,8 +27,15 @@ def as_2D_array(value, shape) -> np.ndarray:
 
 
 def lower_than(x, y, rtol=1e-10):
-    """utils.py:90-92: False if x is within rtol of y or greater than y."""
-    return x < y and not np.isclose(x, y, atol=0, rtol=rtol)
+    """utils.py:90-92: False if x is within rtol of y or greater than y. Scalar form of
+    np.isclose(x, y, atol=0, rtol=rtol): |x - y| <= rtol*|y| for finite y, x == y otherwise (the
+    host loops call this once per instance per launch, where np.isclose costs microseconds)."""
+    if not x < y:
+        return False
+    x, y = float(x), float(y)
+    if math.isinf(y):
+        return True  # x < y, so x != y
+    return not abs(x - y) <= rtol * abs(y)
 
 
 def check_repetition(arr, nx: int, ny: int):

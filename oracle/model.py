@@ -6,8 +6,10 @@ leg of bench.py, as the checker. The product (hotspice_amd) never imports it.
 Everything is float64 on (ny, nx) grids, as in the reference. Randomness is injectable:
   rng="philox": the keyed Philox streams of oracle/philox.py (the streams the HIP kernels
                 draw), so trajectories can be replayed bit-for-bit;
-  rng="numpy":  NumPy generators with the same call structure as the reference (used only
-                to time the CPU path, where no replay is needed).
+  rng="numpy":  NumPy generators with the same call structure as the reference, seeded per
+                instance (a Generator for the continuous draws, a RandomState for the legacy
+                randint/choice calls the reference makes on the global np.random), for the
+                CPU timing and the statistical comparisons.
 
 Restated reference code (file:line in bvwaeyen/Hotspice):
   Energy contract ............ hotspice/energies.py:22-83
@@ -559,9 +561,9 @@ class OracleMagnets:
         n = sc_x.size
         if self.rng_mode == "numpy":
             if self.PBC:
-                ox, oy = np.random.randint(self.nx), np.random.randint(self.ny)
+                ox, oy = self._np_legacy.randint(self.nx), self._np_legacy.randint(self.ny)
             else:
-                ox, oy = np.random.randint(-Rx, Rx), np.random.randint(-Ry, Ry)
+                ox, oy = self._np_legacy.randint(-Rx, Rx), self._np_legacy.randint(-Ry, Ry)
         else:
             u = philox.sel_uniforms(self.seed, self.step, attempt)
             if self.PBC:
@@ -577,7 +579,7 @@ class OracleMagnets:
         nz = occ_sc.nonzero()
         cnt = nz[0].size
         if self.rng_mode == "numpy":
-            pick = np.random.choice(cnt, n)
+            pick = self._np_legacy.choice(cnt, n)
         else:
             cu = philox.choice_uniforms(self.seed, self.step, attempt, n)
             pick = np.floor(cu * cnt).astype(np.int64)
@@ -602,7 +604,7 @@ class OracleMagnets:
     def _select_single(self, attempt):
         """core.py:489-494"""
         if self.rng_mode == "numpy":
-            i = np.random.randint(self.n)
+            i = self._np_legacy.randint(self.n)
         else:
             i = philox.randint(philox.sel_uniforms(self.seed, self.step, attempt)[2], self.n)
         return self._nonzero_array[:, i].reshape(2, -1)

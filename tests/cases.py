@@ -54,6 +54,13 @@ SYNTH = {
                                    params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=3,
                                                SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
                                    update_kw=dict(r=6e-6), nsteps=60, seed=33),
+    # two-launch step with PBC: truncated windows wrap, in-plane exchange stencil wraps
+    "metro_ipsq_pbc_large": dict(lattice="IP_Square", a=1e-6, n=96, T=300, moment=2.37e-16,
+                                 energies=[("dipolar", dict(cutoff_range=4e-6)), ("exchange", dict(J_kT=0.1)),
+                                           ("zeeman", dict(magnitude=1e-3, angle=0.4))],
+                                 params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
+                                             SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
+                                 kw=dict(PBC=True), update_kw=dict(r=10e-6), nsteps=60, seed=34),
     "metro_ipsq_window": dict(lattice="IP_Square", a=1e-6, n=48, T=400, moment=2.37e-16,
                               energies=[("dipolar", dict(cutoff_range=3e-6)), ("zeeman", dict(magnitude=1e-3, angle=0.7))],
                               params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,
