@@ -120,7 +120,11 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
 
   // register-resident state of the magnets of list entries e_k = base + k*stride (k < nval).
   // Owners keep pos = y << XB | x and the fields; drawers keep the flat cell index in pos.
-  const int base = rank * OW + ot, stride = G * OW;
+  // Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD b % 8); when the team
+  // starts at a multiple of 8, XCD x holds ranks x, x+8, ..., and those take consecutive bands
+  // of the list, so each XCD's L2 serves the kernel rows of one contiguous band of the grid.
+  const int band = (G % 8 == 0 && sT.first % 8 == 0) ? (rank % 8) * (G / 8) + rank / 8 : rank;
+  const int base = band * OW + ot, stride = G * OW;
   int nval = 0;
   uint32_t pos[MAXC];
   float e[MAXC], ec[MAXC], ep[MAXC], epc[MAXC], eb[MAXC], mu[MAXC];
