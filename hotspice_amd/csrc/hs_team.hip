@@ -65,23 +65,31 @@ HS_INLINE uint32_t key_bits(float k) {
 
 // FP32 log-rate of a cell: -b*beta (one channel) or log(exp(-b1 beta) + exp(-b2 beta)); the
 // same expression as the per-step launches use, so both paths rank cells identically.
+// mpf = (float)m_perp_factor, barrier = 0 'simple' / 1 'parabolic'.
 template <bool PERP>
-HS_INLINE float team_log_rate(const hs_instance_t &I, float e, float ep, float eb, float beta) {
+HS_INLINE float team_log_rate(float mpf, int barrier, float e, float ep, float eb, float beta) {
   const float dE = -2.f * e, El = e;
   if (PERP) {
-    const float epf = (float)I.m_perp_factor * ep;
+    const float epf = mpf * ep;
     const float b1 = fmaxf(dE, (eb + epf) - El), b2 = fmaxf(dE, (eb - epf) - El);
     const float lo = fminf(b1, b2), hi = fmaxf(b1, b2);
     return -lo * beta + hs_log_fast(1.f + __builtin_amdgcn_exp2f(-(hi - lo) * beta * 1.44269504088896f));
   }
   float b;
-  if (I.barrier == 0) {
+  if (barrier == 0) {
     b = fmaxf(dE, eb - El);
   } else {
     const float q = dE * (0.25f / eb) + 1.f;
     b = dE > 4.f * eb ? dE : fminf(eb * (q * q), dE + 4.f * eb);
   }
   return -b * beta;
+}
+
+// x % d for x < 2^24 and 1 <= d <= 64, with M = ceil(2^32 / d): q = floor(x * M / 2^32) is exact
+// because x * (M d - 2^32) < 2^24 * 64 < 2^32.
+HS_INLINE int mod_small(uint32_t x, uint32_t d, uint64_t M) {
+  const uint32_t q = (uint32_t)(((uint64_t)x * M) >> 32);
+  return (int)(x - q * d);
 }
 
 template <int MAXC, bool PERP>
@@ -117,6 +125,17 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   const int hy = I.nn_h / 2, hx = I.nn_w / 2;
   const double nocc = (double)I.n_occ;
   const float beta = I.beta[0];  // uniform over the grid (the host checks)
+  // Loop invariants in registers: the descriptor sits in LDS, and the loop's own LDS stores would
+  // otherwise turn every use into a fresh LDS read on the step's critical path.
+  const float mpf = (float)I.m_perp_factor;
+  const double mpf64 = I.m_perp_factor, dip_pf = I.dip_pf, Jx = I.J;
+  const int barrier = I.barrier, kwx = I.kwx, kwy = I.kwy, ny = I.ny, pbc = I.pbc, in_plane = I.in_plane;
+  const int nn_h = I.nn_h, nn_w = I.nn_w;
+  const float *const K32 = I.K32, *const Kpo32 = I.Kpo32, *const mug = I.mu;
+  const uint32_t seed_lo = (uint32_t)I.seed, seed_hi = (uint32_t)(I.seed >> 32);
+  const uint32_t ux = (uint32_t)I.ux, uy = (uint32_t)I.uy;
+  const uint64_t Mux = ((1ull << 32) + ux - 1) / ux, Muy = ((1ull << 32) + uy - 1) / uy;
+  auto site_of = [&](int y, int x) { return mod_small((uint32_t)y, uy, Muy) * (int)ux + mod_small((uint32_t)x, ux, Mux); };
 
   // register-resident state of the magnets of list entries e_k = base + k*stride (k < nval).
   // Owners keep pos = y << XB | x and the fields; drawers keep the flat cell index in pos.
@@ -193,8 +212,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       const bool paired = valid && (c & 1u) == odd && pj == j;
       if (two && __ballot(valid && !paired) == 0ull) {
         const uint64_t sk = (uint64_t)s + odd;
-        const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
-                                  (uint32_t)(I.seed >> 32));
+        const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), seed_lo, seed_hi);
         const uint32_t ma = odd ? W.z : W.x, mb = odd ? W.w : W.y;  // this lane's cell in this call
         const uint32_t ra = hs_dpp_u32<0xB1>(0u, odd ? W.x : W.z);
         const uint32_t rb = hs_dpp_u32<0xB1>(0u, odd ? W.y : W.w);
@@ -205,8 +223,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       } else if (valid) {
         for (int q = 0; q < (two ? 2 : 1); ++q) {
           const uint64_t sk = (uint64_t)s + q;
-          const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), (uint32_t)I.seed,
-                                    (uint32_t)(I.seed >> 32));
+          const uint4 W = hs_philox(make_uint4(j, (uint32_t)sk, (uint32_t)(sk >> 32), 1u), seed_lo, seed_hi);
           s_x[sk & 3][k][ot] = (c & 1u) ? hs_exp1_fast(W.z, W.w) : hs_exp1_fast(W.x, W.y);
         }
       }
@@ -228,7 +245,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
 #pragma unroll
     for (int k = 0; k < MAXC; ++k) {
       if ((cand >> k) & 1u) {
-        float key = hs_log_fast(s_x[buf][k][ot]) - team_log_rate<PERP>(I, e[k], ep[k], eb[k], beta);
+        float key = hs_log_fast(s_x[buf][k][ot]) - team_log_rate<PERP>(mpf, barrier, e[k], ep[k], eb[k], beta);
         if (isnan(key)) key = INFINITY;
         const unsigned long long p =
             ((unsigned long long)key_bits(key) << 32) | (pos[k] << 2) | (uint32_t)(spin(k) + 1);
@@ -278,7 +295,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
         const float *in = s_in[src];
         const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
-                                     (double)in[4], (double)beta, I.m_perp_factor, I.barrier, PERP, in[5], R.attempt_freq);
+                                     (double)in[4], (double)beta, mpf64, barrier, PERP, in[5], R.attempt_freq);
         pd = (unsigned long long)__double_as_longlong(dt);
       }
       bool bad = false;
@@ -353,14 +370,14 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     const uint32_t wpos = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)(win >> 2) & 0xFFFFFFu));
     const int mw = __builtin_amdgcn_readfirstlane((int)(win & 3ull)) - 1;
     const int yw = (int)(wpos >> XB), xw = (int)(wpos & XM);
-    const int w = yw * nx + xw, sitew = hs_site(I, yw, xw);
-    const int nw = I.dip_pf != 0.0 ? I.sub_of[sitew] : -1;
+    const int w = yw * nx + xw, sitew = site_of(yw, xw);
+    const int nw = dip_pf != 0.0 ? I.sub_of[sitew] : -1;
     const double mnew = -(double)mw;
-    const float A = (float)(2.0 * I.dip_pf * (mnew * (double)I.mu[w]));
-    const float Ax = (float)(-2.0 * I.J * mnew);
+    const float A = (float)(2.0 * dip_pf * (mnew * (double)mug[w]));
+    const float Ax = (float)(-2.0 * Jx * mnew);
     const float oxw = (float)I.ox_uc[sitew], oyw = (float)I.oy_uc[sitew];
-    const float *Kw = nw >= 0 ? I.K32 + (size_t)nw * KS : nullptr;
-    const float *Kpw = (PERP && nw >= 0) ? I.Kpo32 + (size_t)nw * KS : nullptr;
+    const float *Kw = nw >= 0 ? K32 + (size_t)nw * KS : nullptr;
+    const float *Kpw = (PERP && nw >= 0) ? Kpo32 + (size_t)nw * KS : nullptr;
     constexpr int CH = MAXC < 4 ? MAXC : 4;  // cells whose kernel entries are loaded together
 #pragma unroll
     for (int k0 = 0; k0 < MAXC; k0 += CH) {
@@ -372,8 +389,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         kpv[j] = 0.f;
         if (k < nval && Kw) {
           const int dy = (int)(pos[k] >> XB) - yw, dx = (int)(pos[k] & XM) - xw;
-          if (dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx) {
-            const int o = (I.kwy + dy) * KW + (I.kwx + dx);
+          if (dy >= -kwy && dy <= kwy && dx >= -kwx && dx <= kwx) {
+            const int o = (kwy + dy) * KW + (kwx + dx);
             kv[j] = Kw[o];
             if (PERP) kpv[j] = Kpw[o];
           }
@@ -402,19 +419,19 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         float dep = PERP ? A * (mmu * kpv[j]) : 0.f;
         if (Ax != 0.f) {
           int qy = dy, qx = dx;
-          if (I.pbc) {
-            qy = hs_pmod(dy, I.ny);
-            qy = qy > I.ny / 2 ? qy - I.ny : qy;
-            qx = hs_pmod(dx, I.nx);
-            qx = qx > I.nx / 2 ? qx - I.nx : qx;
+          if (pbc) {
+            qy = hs_pmod(dy, ny);
+            qy = qy > ny / 2 ? qy - ny : qy;
+            qx = hs_pmod(dx, nx);
+            qx = qx > nx / 2 ? qx - nx : qx;
           }
           qy += hy;
           qx += hx;
-          if (qy >= 0 && qy < I.nn_h && qx >= 0 && qx < I.nn_w) {
-            const float wq = I.nn_mask[qy * I.nn_w + qx];
+          if (qy >= 0 && qy < nn_h && qx >= 0 && qx < nn_w) {
+            const float wq = I.nn_mask[qy * nn_w + qx];
             if (wq != 0.f) {
-              if (I.in_plane) {
-                const int s = hs_site(I, y, x);
+              if (in_plane) {
+                const int s = site_of(y, x);
                 const float oxi = (float)I.ox_uc[s], oyi = (float)I.oy_uc[s];
                 de += Ax * (float)mk * wq * (oxi * oxw + oyi * oyw);
                 if (PERP) dep += Ax * (float)mk * wq * (oyw * oxi - oxw * oyi);
