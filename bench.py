@@ -40,7 +40,7 @@ PERIOD = 1024
 # HBM bytes per launch of k_neel_team on the 256x256 pinwheel workload (1024 attempts), from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy of known size; the
 # commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
-NEEL_TEAM_TRAFFIC_PER_LAUNCH = 66.6e6
+NEEL_TEAM_TRAFFIC_PER_LAUNCH = 96.3e6
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 

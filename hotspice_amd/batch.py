@@ -71,11 +71,12 @@ def _pending(m, start, t_max, mc, st=None):
 
 def neel_team_size(m, nblocks=0):
     """Workgroups of an instance's Neel team: `nblocks` (or the instance's own setting), else the
-    library's default of one magnet per owner thread (512 per workgroup), at most 128."""
+    library's default of one magnet per owner thread (512 per workgroup), at most one per CU."""
     nblocks = nblocks or (m.neel_blocks if isinstance(m.neel_blocks, int) else 0)
     if nblocks:
         return int(nblocks)
-    return max(1, min(128, -(-m.n // 512)))
+    cus = m._dev.torch.cuda.get_device_properties(m._dev.device).multi_processor_count
+    return max(1, min(cus, -(-m.n // 512)))
 
 
 def launch_neel(items, attempt_freq=1e10, begin=False):

@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   // own call for each step.
   auto draw = [&](int64_t s, bool two) {
     const uint32_t odd = (uint32_t)lane & 1u;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < MAXC; ++k) {
       const bool valid = k < nval;
       const uint32_t c = valid ? pos[k] : 0u, j = c >> 1;
@@ -495,14 +495,25 @@ struct Layout {
   int nblocks, maxc;
 };
 
-int team_blocks(const hs_team_t &t) {
+// Default team size: one magnet per owner thread, at most one workgroup per CU of the device
+// (more magnets per thread than 4 spill the register-resident state).
+int team_blocks(const hs_team_t &t, int cap) {
   if (t.nblocks > 0) return t.nblocks;
-  // default: one magnet per owner thread, at most 128 workgroups
   int g = (t.inst->n_occ + OW - 1) / OW;
-  return g < 1 ? 1 : (g > 128 ? 128 : g);
+  return g < 1 ? 1 : (g > cap ? cap : g);
+}
+
+// CUs of the current device (the cap of a default team), 128 if the query fails.
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    return 128;
+  return cus;
 }
 
 int make_layout(const hs_team_t *teams, int n, Layout &L) {
+  const int cap = device_cus();
   L.teams_off = 0;
   L.inst_off = al256(sizeof(TeamDev) * n);
   L.map_off = L.inst_off + al256(sizeof(hs_instance_t) * n);
@@ -511,7 +522,7 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
   L.G.resize(n);
   L.first.resize(n);
   for (int i = 0; i < n; ++i) {
-    const int g = team_blocks(teams[i]);
+    const int g = team_blocks(teams[i], cap);
     L.G[i] = g;
     L.first[i] = L.nblocks;
     L.nblocks += g;
