@@ -394,8 +394,9 @@ __global__ void __launch_bounds__(256) k_recompute_dip(hs_instance_t I, RecPlan 
   rec_dip(I, plan, Kbase, part, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Several recomputes in one launch: block b works for job block_job[b] (its blocks in the order
-// tile, then sublattice p, then source group g).
+// Several recomputes in two launches, one for the small grids and one for the tiled ones: block b
+// works for job block_job[b] (a tiled job's blocks in the order tile, then sublattice p, then
+// source group g).
 __device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *Kbase, double *part, int c);
 
 struct RecJob {
@@ -406,16 +407,21 @@ struct RecJob {
   int32_t first, small;
 };
 
+// Tiled jobs: blocks offset + blockIdx.x of the block map (the small jobs' blocks come first and
+// run in k_recompute_small_many, whose LDS footprint differs).
 __global__ void __launch_bounds__(256) k_recompute_dip_many(const RecJob *__restrict__ jobs,
-                                                            const int32_t *__restrict__ block_job) {
-  const RecJob &J = jobs[block_job[blockIdx.x]];
-  const int lb = blockIdx.x - J.first;
-  if (J.small) {
-    rec_small(J.I, J.K, J.part, lb * 256 + threadIdx.x);
-    return;
-  }
+                                                            const int32_t *__restrict__ block_job, int offset) {
+  const int b = offset + (int)blockIdx.x;
+  const RecJob &J = jobs[block_job[b]];
+  const int lb = b - J.first;
   const int nt = J.plan.tA * J.plan.tB;
   rec_dip(J.I, J.plan, J.K, J.part, lb % nt, (lb / nt) % J.I.n_sub, lb / (nt * J.I.n_sub));
+}
+
+__global__ void __launch_bounds__(256) k_recompute_small_many(const RecJob *__restrict__ jobs,
+                                                              const int32_t *__restrict__ block_job) {
+  const RecJob &J = jobs[block_job[blockIdx.x]];
+  rec_small(J.I, J.K, J.part, ((int)blockIdx.x - J.first) * 256 + threadIdx.x);
 }
 
 __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const double *partE, const double *partP,
@@ -466,12 +472,18 @@ __global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, 
   rec_finish(I, F, partE, partP, which, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Small grids: one thread per target cell sums K[source - target] * (m mu)[source] over the
-// stored window directly (the tiled kernel's 32x64 tiles would be mostly padding there). Only
-// occupied targets on a kernel's sublattice are computed, as the finish pass reads no others.
+// Small grids (at most SMALL_REC_CELLS cells): one thread per target cell sums
+// K[source - target] * (m mu)[source] over the stored window directly (the tiled kernel's 32x64
+// tiles would be mostly padding there). The workgroup (256 threads) first stages (m mu) of the
+// whole grid in LDS, so the inner loop reads only the kernel entry (coalesced over a row of
+// targets). Only occupied targets on a kernel's sublattice are computed, as the finish pass reads
+// no others.
 __device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *Kbase, double *part, int c) {
-  const int nx = I.nx, ny = I.ny;
-  if (c >= nx * ny) return;
+  __shared__ double s_v[4096];
+  const int nx = I.nx, ny = I.ny, ncell = nx * ny;
+  for (int s = threadIdx.x; s < ncell; s += blockDim.x) s_v[s] = (double)I.m[s] * I.mu64[s];
+  __syncthreads();
+  if (c >= ncell) return;
   const int y = c / nx, x = c % nx, n = I.sub_of[hs_site(I, y, x)];
   double acc = 0.0;
   if (n >= 0 && I.m[c] != 0) {
@@ -481,9 +493,9 @@ __device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *
     const int x0 = max(0, x - I.kwx), x1 = min(nx - 1, x + I.kwx);
     for (int ys = y0; ys <= y1; ++ys) {
       const double *krow = Kn + (size_t)(ys - y + I.kwy) * KW + (I.kwx - x);
+      const double *vrow = s_v + (size_t)ys * nx;
       for (int xs = x0; xs <= x1; ++xs) {
-        const int s = ys * nx + xs;
-        const double v = (double)I.m[s] * I.mu64[s];
+        const double v = vrow[xs];
         if (v != 0.0) acc = fma(krow[xs], v, acc);
       }
     }
@@ -650,7 +662,15 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
   FinJob *fj = (FinJob *)&host[L.fin_off];
   int32_t *dmap = (int32_t *)&host[L.dmap_off];
   int32_t *fmap = (int32_t *)&host[L.fmap_off];
-  int nd = 0, db = 0, fb = 0;
+  // block numbering: every small job's blocks first, then the tiled jobs' blocks
+  int dbs = 0;
+  for (int i = 0; i < n; ++i) {
+    const hs_instance_t &I = *insts[i];
+    if (I.dip_pf == 0.0 || !small_rec(&I)) continue;
+    const int per = (I.nx * I.ny + 255) / 256;
+    dbs += ((which[i] & 1) ? per : 0) + (((which[i] & 2) && I.use_perp) ? per : 0);
+  }
+  int nd = 0, db_small = 0, db = dbs, fb = 0;
   for (int i = 0; i < n; ++i) {
     const hs_instance_t &I = *insts[i];
     const RecPlan pl = make_plan(&I);
@@ -667,10 +687,11 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
         J.plan = pl;
         J.K = w == 1 ? I.K64 : I.Kps64;
         J.part = w == 1 ? partE : partP;
-        J.first = db;
         J.small = small_rec(&I) ? 1 : 0;
-        for (int b = 0; b < per; ++b) dmap[db + b] = nd;
-        db += per;
+        int &next = J.small ? db_small : db;
+        J.first = next;
+        for (int b = 0; b < per; ++b) dmap[next + b] = nd;
+        next += per;
         ++nd;
       }
     }
@@ -689,8 +710,13 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
   char *base = (char *)ws;
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), L.total, hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
-  if (db > 0) {
-    k_recompute_dip_many<<<db, 256, 0, s>>>((const RecJob *)(base + L.dip_off), (const int32_t *)(base + L.dmap_off));
+  if (dbs > 0) {
+    k_recompute_small_many<<<dbs, 256, 0, s>>>((const RecJob *)(base + L.dip_off), (const int32_t *)(base + L.dmap_off));
+    HS_LAUNCH_CHECK("k_recompute_small_many");
+  }
+  if (db > dbs) {
+    k_recompute_dip_many<<<db - dbs, 256, 0, s>>>((const RecJob *)(base + L.dip_off),
+                                                  (const int32_t *)(base + L.dmap_off), dbs);
     HS_LAUNCH_CHECK("k_recompute_dip_many");
   }
   k_recompute_finish_many<<<fb, 256, 0, s>>>((const FinJob *)(base + L.fin_off), (const int32_t *)(base + L.fmap_off));

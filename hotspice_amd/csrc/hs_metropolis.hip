@@ -42,6 +42,8 @@ constexpr int PERSIST_CELLS = 80 * 80;
 constexpr int UPDATE_WGS = 2048;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
+// samples per thread whose loads the selection kernels issue together (and keep in registers)
+constexpr int SELECT_BATCH = 2;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -316,6 +318,11 @@ HS_INLINE bool mark_rect(uint8_t *tmark, int tiles_x, int nx, int ny, int sx, in
   return true;
 }
 
+// One Metropolis step's selection for one job (one workgroup). Each thread takes samples
+// [j0, j1) and works on them in batches of QB: within a batch the independent loads of all its
+// samples are issued together, and when a thread has at most QB samples their candidate cells,
+// acceptance flags and (m mu) stay in registers from phase to phase instead of being re-read.
+template <int QB>
 __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
                                              int job = 0) {
   const hs_instance_t &I = J.I;
@@ -353,6 +360,17 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   const int q = (nsc + MS_THREADS - 1) / MS_THREADS;
   const int j0 = min(nsc, tid * q), j1 = min(nsc, j0 + q);
   const int nx = I.nx, ny = I.ny;
+  // all of this thread's grid samples fit one batch: they stay in registers (the Poisson scheme
+  // writes its candidates to the workspace only)
+  const bool regs = q <= QB && SEL.scheme == 0;
+  int candr[QB], accr[QB];
+  float mmur[QB];
+#pragma unroll
+  for (int u = 0; u < QB; ++u) {
+    candr[u] = -1;
+    accr[u] = 0;
+    mmur[u] = 0.f;
+  }
   int nvalid = 0;
   bool single = false;
   int attempt = 0;
@@ -382,25 +400,54 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
       const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
       const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
       int inb = 0, val = 0;
-      for (int j = j0; j < j1; ++j) {
-        const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u | ((uint32_t)attempt << 8), (uint64_t)j), 0) * cnt);
-        const int pc = SEL.pat_cells[p0 + pick];
-        const int sy = 2 * (j / SEL.ncols), sx = 2 * (j % SEL.ncols);
-        int x = sx * SEL.Rx + (pc & 0xffff), y = sy * SEL.Ry + (pc >> 16);
-        bool ok;
-        if (I.pbc) {
-          ok = (x < max(SEL.Rx, nx - SEL.Rx)) && (y < max(SEL.Ry, ny - SEL.Ry));  // core.py:545
-          x = (x + ox) % nx;
-          y = (y + oy) % ny;
-        } else {
-          x += ox;
-          y += oy;
-          ok = x >= 0 && x < nx && y >= 0 && y < ny;  // core.py:552
+      for (int jb = j0; jb < j1; jb += QB) {
+        int pcs[QB], cl[QB], fl[QB];
+        bool oks[QB];
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {  // the picks, and their pattern loads issued together
+          const int j = jb + u;
+          pcs[u] = 0;
+          if (j < j1) {
+            const int pick =
+                (int)floor(hs_u53(hs_bits53(I.seed, step, 3u | ((uint32_t)attempt << 8), (uint64_t)j), 0) * cnt);
+            pcs[u] = SEL.pat_cells[p0 + pick];
+          }
         }
-        const bool v = ok && (I.flags[y * nx + x] & HS_FLAG_OCC);
-        ws.cand[j] = v ? y * nx + x : -1;
-        inb += ok;
-        val += v;
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {  // the sample cells, and their flag loads issued together
+          const int j = jb + u;
+          oks[u] = false;
+          cl[u] = -1;
+          fl[u] = 0;
+          if (j < j1) {
+            const int sy = 2 * (j / SEL.ncols), sx = 2 * (j % SEL.ncols);
+            int x = sx * SEL.Rx + (pcs[u] & 0xffff), y = sy * SEL.Ry + (pcs[u] >> 16);
+            bool ok;
+            if (I.pbc) {
+              ok = (x < max(SEL.Rx, nx - SEL.Rx)) && (y < max(SEL.Ry, ny - SEL.Ry));  // core.py:545
+              x = (x + ox) % nx;
+              y = (y + oy) % ny;
+            } else {
+              x += ox;
+              y += oy;
+              ok = x >= 0 && x < nx && y >= 0 && y < ny;  // core.py:552
+            }
+            oks[u] = ok;
+            cl[u] = y * nx + x;
+            if (ok) fl[u] = I.flags[cl[u]];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+          const int j = jb + u;
+          if (j >= j1) continue;
+          const bool v = oks[u] && (fl[u] & HS_FLAG_OCC);
+          const int c = v ? cl[u] : -1;
+          ws.cand[j] = c;
+          candr[u] = c;  // (one batch when regs)
+          inb += oks[u];
+          val += v;
+        }
       }
       // one block reduction of both counts (in-range samples in the high word, valid ones in the low)
       const long long both = block_sum_i64(((long long)inb << 32) + val, s_l);
@@ -437,58 +484,92 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   int my_acc = 0;
   double tsum = 0.0;
   const int a0 = single ? (tid == 0 ? 0 : 1) : j0, a1 = single ? 1 : j1;
-  for (int j = a0; j < a1; ++j) {
-    const int c = single ? s_single : ws.cand[j];
-    bool acc = false;
-    float mmu = 0.f;
-    if (c >= 0) {
-      const double e = (double)I.E[c] - (double)I.Ec[c];
-      const double beta = (double)I.beta[c];
-      mmu = (float)(-(int)I.m[c]) * I.mu[c];  // (m mu) after a flip
-      // exp(-switch_energy*beta) with switch_energy = -2E, relative error ~3e-7 (FP64 exponent
-      // split, FP32 mantissa), clipped to [1e-10, 1e10]
-      double p = hs_scaled_exp(1.f, 2.0 * e * beta, 1.f, 1.0);
-      p = fmin(fmax(p, 1e-10), 1e10);
-      if (SEL.metropolis_time) {
-        const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
-        tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * beta);
+  for (int jb = a0; jb < a1; jb += QB) {
+    int cs[QB], mv[QB];
+    float ev[QB], ecv[QB], btv[QB], muv[QB];
+#pragma unroll
+    for (int u = 0; u < QB; ++u) {  // the samples' state, loaded together
+      const int j = jb + u;
+      cs[u] = j < a1 ? (single ? s_single : (regs ? candr[u] : ws.cand[j])) : -1;
+      mv[u] = 0;
+      ev[u] = ecv[u] = btv[u] = muv[u] = 0.f;
+      if (cs[u] >= 0) {
+        ev[u] = I.E[cs[u]];
+        ecv[u] = I.Ec[cs[u]];
+        btv[u] = I.beta[cs[u]];
+        mv[u] = (int)I.m[cs[u]];
+        muv[u] = I.mu[cs[u]];
       }
-      const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
-      acc = ua < p;
     }
-    if (single) {
-      if (tid == 0) {
-        s_single = acc ? c : -1;
-        s_single_mmu = mmu;
+#pragma unroll
+    for (int u = 0; u < QB; ++u) {
+      const int j = jb + u;
+      if (j >= a1) continue;
+      const int c = cs[u];
+      bool acc = false;
+      float mmu = 0.f;
+      if (c >= 0) {
+        const double e = (double)ev[u] - (double)ecv[u];
+        const double beta = (double)btv[u];
+        mmu = (float)(-mv[u]) * muv[u];  // (m mu) after a flip
+        // exp(-switch_energy*beta) with switch_energy = -2E, relative error ~3e-7 (FP64 exponent
+        // split, FP32 mantissa), clipped to [1e-10, 1e10]
+        double p = hs_scaled_exp(1.f, 2.0 * e * beta, 1.f, 1.0);
+        p = fmin(fmax(p, 1e-10), 1e10);
+        if (SEL.metropolis_time) {
+          const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
+          tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * beta);
+        }
+        const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
+        acc = ua < p;
       }
-    } else {
-      ws.accf[j] = acc ? 1 : 0;
-      ws.cmmu[j] = mmu;
+      if (single) {
+        if (tid == 0) {
+          s_single = acc ? c : -1;
+          s_single_mmu = mmu;
+        }
+      } else {
+        ws.accf[j] = acc ? 1 : 0;
+        ws.cmmu[j] = mmu;
+        accr[u] = acc ? 1 : 0;
+        mmur[u] = mmu;
+      }
+      my_acc += acc;
     }
-    my_acc += acc;
   }
+  // this thread's accepted cell of sample slot u of batch jb (-1 if none), and its (m mu)
+  auto accepted = [&](int jb, int u, float &mmu) {
+    const int j = jb + u;
+    if (single) {
+      mmu = s_single_mmu;
+      return s_single;
+    }
+    if (regs) {
+      mmu = mmur[u];
+      return accr[u] ? candr[u] : -1;
+    }
+    mmu = ws.cmmu[j];
+    return ws.accf[j] ? ws.cand[j] : -1;
+  };
   if (SEL.metropolis_time) tsum = block_sum_double(tsum, s_d);
   int total = 0;
   int pos = block_exclusive_scan(my_acc, s_i, total);
   __syncthreads();
   // the accepted switches in selection order; the flips themselves (m *= -1, E *= -1,
   // energies.py:124-126, 332-334) are applied by the update pass, by the tile holding the cell
-  for (int j = a0; j < a1; ++j) {
-    int c;
-    float mmu;
-    if (single) {
-      c = s_single;
-      mmu = s_single_mmu;
-    } else {
-      c = ws.accf[j] ? ws.cand[j] : -1;
-      mmu = ws.cmmu[j];
+  for (int jb = a0; jb < a1; jb += QB) {
+#pragma unroll
+    for (int u = 0; u < QB; ++u) {
+      if (jb + u >= a1) continue;
+      float mmu;
+      const int c = accepted(jb, u, mmu);
+      if (c < 0) continue;
+      ws.acc_cell[pos] = c;
+      ws.acc_mmu[pos] = mmu;
+      atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
+      if (R.log) log_cells[(size_t)S.steps_done * (size_t)(nsc > 0 ? nsc : 1) + pos] = c;
+      ++pos;
     }
-    if (c < 0) continue;
-    ws.acc_cell[pos] = c;
-    ws.acc_mmu[pos] = mmu;
-    atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
-    if (R.log) log_cells[(size_t)S.steps_done * (size_t)(nsc > 0 ? nsc : 1) + pos] = c;
-    ++pos;
   }
   __syncthreads();
   if (tid < I.ux * I.uy) {
@@ -505,18 +586,23 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
     const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
     if (tid == 0) s_all = 0;
     __syncthreads();
-    for (int j = a0; j < a1; ++j) {
-      const int c = single ? s_single : (ws.accf[j] ? ws.cand[j] : -1);
-      if (c < 0) continue;
-      const int sy = c / nx, sx = c % nx, ss = hs_site(I, sy, sx);
-      ws.tmark[(sy / MU_TY) * tiles_x + sx / MU_TX] = 1;
-      bool ok = true;
-      if (has_dip && I.sub_of[ss] >= 0) {
-        const bool tr = total > 1 && SEL.reduced_k != 0 && s_site[ss] > SEL.cutoff;
-        ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, tr ? tkx : I.kwx, sy, tr ? tky : I.kwy, tr && I.pbc);
+    for (int jb = a0; jb < a1; jb += QB) {
+#pragma unroll
+      for (int u = 0; u < QB; ++u) {
+        if (jb + u >= a1) continue;
+        float mmu;
+        const int c = accepted(jb, u, mmu);
+        if (c < 0) continue;
+        const int sy = c / nx, sx = c % nx, ss = hs_site(I, sy, sx);
+        ws.tmark[(sy / MU_TY) * tiles_x + sx / MU_TX] = 1;
+        bool ok = true;
+        if (has_dip && I.sub_of[ss] >= 0) {
+          const bool tr = total > 1 && SEL.reduced_k != 0 && s_site[ss] > SEL.cutoff;
+          ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, tr ? tkx : I.kwx, sy, tr ? tky : I.kwy, tr && I.pbc);
+        }
+        if (ok && has_ex) ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, I.nn_w / 2, sy, I.nn_h / 2, I.pbc != 0);
+        if (!ok) s_all = 1;
       }
-      if (ok && has_ex) ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, I.nn_w / 2, sy, I.nn_h / 2, I.pbc != 0);
-      if (!ok) s_all = 1;
     }
     __syncthreads();
     // 16 marks per load (tmark is 256-byte aligned and padded); bytes past the last tile never count
@@ -832,7 +918,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
 }
 
 __global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J, WorkList WL, int par) {
-  metro_select(J, WL, par, 0);
+  metro_select<SELECT_BATCH>(J, WL, par, 0);
 }
 
 // One selection workgroup per job of the list `which`.
@@ -840,7 +926,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob
                                                                   const int32_t *__restrict__ which, WorkList WL,
                                                                   int par) {
   const int j = which[blockIdx.x];
-  metro_select(jobs[j], WL, par, j);
+  metro_select<SELECT_BATCH>(jobs[j], WL, par, j);
 }
 
 // The update of the step's listed tiles (one tile at a time per workgroup); then the counter of
@@ -890,7 +976,7 @@ __global__ void k_metro_clear_many(const MetroJob *__restrict__ jobs, int njobs,
 __device__ __forceinline__ void metro_persist(const MetroJob &J) {
   const int tx = (J.I.nx + 31) / 32, ty = (J.I.ny + 31) / 32;
   for (int64_t s = 0; s < J.R.max_steps; ++s) {
-    const bool stepped = metro_select(J);
+    const bool stepped = metro_select<1>(J);
     __syncthreads();
     if (!stepped) break;
     for (int t = 0; t < tx * ty; ++t) metro_update<32, 32, false>(J, t % tx, t / tx);
