@@ -42,6 +42,9 @@ constexpr int PERSIST_CELLS = 80 * 80;
 constexpr int UPDATE_WGS = 2048;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
+// the selection marks tiles in an LDS bitmap of this many 32-bit words (jobs with more tiles list
+// every tile)
+constexpr int MARK_WORDS = 4096;
 // samples per thread whose loads the selection kernels issue together (and keep in registers)
 constexpr int SELECT_BATCH = 2;
 
@@ -58,7 +61,6 @@ struct MetroWS {
   uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
   int32_t *pwin;     // [nsc] Poisson: 1 = this round's candidate is accepted
   float *cmmu;       // [nsc] (m mu) of each candidate after its flip (read where accf is set)
-  uint8_t *tmark;    // [ntiles] 1 = tile reached by a switch of this step (cleared when listed)
 };
 
 // The step's (job, tile) worklist of an update launch: count[par] entries in wl. The selection
@@ -82,7 +84,7 @@ struct MetroJob {
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-MetroWS carve(void *base, int nsc, int ntiles) {
+MetroWS carve(void *base, int nsc) {
   char *p = (char *)base;
   MetroWS w;
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
@@ -114,16 +116,14 @@ MetroWS carve(void *base, int nsc, int ntiles) {
   p += align256(n * 4);
   w.cmmu = (float *)p;
   p += align256(n * 4);
-  w.tmark = (uint8_t *)p;  // padded to whole 16-byte chunks (align256)
-  p += align256((size_t)(ntiles > 0 ? ntiles : 1));
   return w;
 }
 
 // Per-job scratch (the MetroWS arrays).
-size_t ws_bytes(int nsc, int ntiles) {
+size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
   return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
-         7 * align256(n * 4) + align256((size_t)(ntiles > 0 ? ntiles : 1));
+         7 * align256(n * 4);
 }
 
 // A worklist for `ntiles` tiles: the entries and the two step counters.
@@ -301,9 +301,9 @@ HS_INLINE int tile_ranges(int s, int w, int n, bool wrap, int T, int t0[3], int 
   return cnt;
 }
 
-// Marks the tiles of the rectangle (sx +- wx) x (sy +- wy); returns false (marking nothing) when
-// it spans more than MARK_MAX_TILES tiles.
-HS_INLINE bool mark_rect(uint8_t *tmark, int tiles_x, int nx, int ny, int sx, int wx, int sy, int wy, bool wrap) {
+// Marks the tiles of the rectangle (sx +- wx) x (sy +- wy) in the bitmap; returns false (marking
+// nothing) when it spans more than MARK_MAX_TILES tiles.
+HS_INLINE bool mark_rect(uint32_t *bits, int tiles_x, int nx, int ny, int sx, int wx, int sy, int wy, bool wrap) {
   int x0[3], x1[3], y0[3], y1[3];
   const int nxr = tile_ranges(sx, wx, nx, wrap, MU_TX, x0, x1);
   const int nyr = tile_ranges(sy, wy, ny, wrap, MU_TY, y0, y1);
@@ -314,7 +314,10 @@ HS_INLINE bool mark_rect(uint8_t *tmark, int tiles_x, int nx, int ny, int sx, in
   for (int a = 0; a < nyr; ++a)
     for (int ty = y0[a]; ty <= y1[a]; ++ty)
       for (int b = 0; b < nxr; ++b)
-        for (int tx = x0[b]; tx <= x1[b]; ++tx) tmark[ty * tiles_x + tx] = 1;
+        for (int tx = x0[b]; tx <= x1[b]; ++tx) {
+          const int t = ty * tiles_x + tx;
+          atomicOr(&bits[t >> 5], 1u << (t & 31));
+        }
   return true;
 }
 
@@ -338,6 +341,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   __shared__ int s_single;
   __shared__ float s_single_mmu;
   __shared__ int s_all, s_base;
+  __shared__ uint32_t s_bits[MARK_WORDS];
   const int tid = threadIdx.x;
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
@@ -584,53 +588,52 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
     const int tiles_x = J.tiles_x, ntile = J.tiles_x * J.tiles_y;
     const bool has_dip = I.dip_pf != 0.0, has_ex = I.J != 0.0 && I.nn_w > 0;
     const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
-    if (tid == 0) s_all = 0;
+    const int nword = (ntile + 31) / 32;
+    if (tid == 0) s_all = nword > MARK_WORDS ? 1 : 0;
+    for (int i = tid; i < min(nword, MARK_WORDS); i += MS_THREADS) s_bits[i] = 0u;
     __syncthreads();
-    for (int jb = a0; jb < a1; jb += QB) {
+    if (s_all == 0) {
+      for (int jb = a0; jb < a1; jb += QB) {
 #pragma unroll
-      for (int u = 0; u < QB; ++u) {
-        if (jb + u >= a1) continue;
-        float mmu;
-        const int c = accepted(jb, u, mmu);
-        if (c < 0) continue;
-        const int sy = c / nx, sx = c % nx, ss = hs_site(I, sy, sx);
-        ws.tmark[(sy / MU_TY) * tiles_x + sx / MU_TX] = 1;
-        bool ok = true;
-        if (has_dip && I.sub_of[ss] >= 0) {
-          const bool tr = total > 1 && SEL.reduced_k != 0 && s_site[ss] > SEL.cutoff;
-          ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, tr ? tkx : I.kwx, sy, tr ? tky : I.kwy, tr && I.pbc);
+        for (int u = 0; u < QB; ++u) {
+          if (jb + u >= a1) continue;
+          float mmu;
+          const int c = accepted(jb, u, mmu);
+          if (c < 0) continue;
+          const int sy = c / nx, sx = c % nx, ss = hs_site(I, sy, sx);
+          const int t0 = (sy / MU_TY) * tiles_x + sx / MU_TX;
+          atomicOr(&s_bits[t0 >> 5], 1u << (t0 & 31));
+          bool ok = true;
+          if (has_dip && I.sub_of[ss] >= 0) {
+            const bool tr = total > 1 && SEL.reduced_k != 0 && s_site[ss] > SEL.cutoff;
+            ok = mark_rect(s_bits, tiles_x, nx, ny, sx, tr ? tkx : I.kwx, sy, tr ? tky : I.kwy, tr && I.pbc);
+          }
+          if (ok && has_ex) ok = mark_rect(s_bits, tiles_x, nx, ny, sx, I.nn_w / 2, sy, I.nn_h / 2, I.pbc != 0);
+          if (!ok) s_all = 1;
         }
-        if (ok && has_ex) ok = mark_rect(ws.tmark, tiles_x, nx, ny, sx, I.nn_w / 2, sy, I.nn_h / 2, I.pbc != 0);
-        if (!ok) s_all = 1;
       }
+      __syncthreads();
     }
-    __syncthreads();
-    // 16 marks per load (tmark is 256-byte aligned and padded); bytes past the last tile never count
+    // list the marked tiles (every tile when a window was too wide or the bitmap too small)
     const bool all = s_all != 0;
-    const int nchunk = (ntile + 15) / 16;
-    auto chunk_bits = [&](int ch) {  // bit b: tile 16*ch + b is listed
-      const uint4 v = *(const uint4 *)(ws.tmark + 16 * ch);
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-      uint32_t bits = 0;
-      for (int q = 0; q < 16; ++q) bits |= ((w[q >> 2] >> (8 * (q & 3))) & 0xffu) != 0 ? (1u << q) : 0u;
-      if (all) bits = 0xffffu;
-      const int valid = ntile - 16 * ch;
-      return valid >= 16 ? bits : bits & ((1u << valid) - 1u);
+    auto word_bits = [&](int w) {  // bit b: tile 32*w + b is listed
+      const uint32_t v = all ? 0xffffffffu : s_bits[w];
+      const int valid = ntile - 32 * w;
+      return valid >= 32 ? v : v & ((1u << valid) - 1u);
     };
     int mine = 0;
-    for (int ch = tid; ch < nchunk; ch += MS_THREADS) mine += __popc(chunk_bits(ch));
+    for (int w = tid; w < nword; w += MS_THREADS) mine += __popc(word_bits(w));
     int nlist = 0;
     int off = block_exclusive_scan(mine, s_i, nlist);
     if (tid == 0) s_base = atomicAdd(&WL.count[par], nlist);
     __syncthreads();
     off += s_base;
-    for (int ch = tid; ch < nchunk; ch += MS_THREADS) {
-      uint32_t bits = chunk_bits(ch);
-      *(uint4 *)(ws.tmark + 16 * ch) = make_uint4(0u, 0u, 0u, 0u);
+    for (int w = tid; w < nword; w += MS_THREADS) {
+      uint32_t bits = word_bits(w);
       while (bits) {
         const int b = __ffs((int)bits) - 1;
         bits &= bits - 1u;
-        WL.wl[off++] = make_int2(job, 16 * ch + b);
+        WL.wl[off++] = make_int2(job, 32 * w + b);
       }
     }
   }
@@ -953,20 +956,9 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob
   if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
 }
 
-// Clears the tile marks of every job and both worklist counters (any workspace state left by an
-// earlier call is discarded).
-__global__ void k_metro_clear_one(MetroJob J, WorkList WL) {
-  const int nt = (J.tiles_x * J.tiles_y + 15) & ~15;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) J.ws.tmark[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < 2) WL.count[threadIdx.x] = 0;
-}
-
-__global__ void k_metro_clear_many(const MetroJob *__restrict__ jobs, int njobs, WorkList WL) {
-  for (int j = 0; j < njobs; ++j) {
-    const int nt = (jobs[j].tiles_x * jobs[j].tiles_y + 15) & ~15;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) jobs[j].ws.tmark[i] = 0;
-  }
-  if (blockIdx.x == 0 && threadIdx.x < 2) WL.count[threadIdx.x] = 0;
+// Clears both worklist counters (a count left by an earlier call is discarded).
+__global__ void k_metro_clear(WorkList WL) {
+  if (threadIdx.x < 2) WL.count[threadIdx.x] = 0;
 }
 
 // Persistent Metropolis stepping of small instances: one 1024-thread workgroup per job runs the
@@ -1030,7 +1022,7 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
   J.SEL.nsc = sel->scheme != 1 ? sel->nsc : 1;
   J.R = *run;
   J.st = state;
-  J.ws = carve(ws, J.SEL.nsc, tiles_of(*inst));
+  J.ws = carve(ws, J.SEL.nsc);
   J.log_cells = log_cells;
   J.log_count = log_count;
   J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
@@ -1043,7 +1035,7 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
 extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
   if (!sel || !inst || inst->nx < 1 || inst->ny < 1) return -1;
   const int nt = tiles_of(*inst);
-  return (int64_t)(ws_bytes(sel->scheme != 1 ? sel->nsc : 1, nt) + wl_bytes(nt));
+  return (int64_t)(ws_bytes(sel->scheme != 1 ? sel->nsc : 1) + wl_bytes(nt));
 }
 
 extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
@@ -1066,10 +1058,10 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
     return HS_OK;
   }
   const int nt = J.tiles_x * J.tiles_y;
-  char *wlp = (char *)ws + ws_bytes(J.SEL.nsc, nt);
+  char *wlp = (char *)ws + ws_bytes(J.SEL.nsc);
   const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
-  k_metro_clear_one<<<(nt + 255) / 256 < 64 ? (nt + 255) / 256 : 64, 256, 0, s>>>(J, WL);
-  HS_LAUNCH_CHECK("k_metro_clear_one");
+  k_metro_clear<<<1, 64, 0, s>>>(WL);
+  HS_LAUNCH_CHECK("k_metro_clear");
   const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
   for (int64_t j = 0; j < run->max_steps; ++j) {
     const int par = (int)(j & 1);
@@ -1117,7 +1109,7 @@ void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
   L.ws_off.resize(n);
   for (int i = 0; i < n; ++i) {
     L.ws_off[i] = off;
-    off += ws_bytes(teams[i].sel->scheme != 1 ? teams[i].sel->nsc : 1, tiles_of(*teams[i].inst));
+    off += ws_bytes(teams[i].sel->scheme != 1 ? teams[i].sel->nsc : 1);
   }
   L.total = off;
 }
@@ -1180,8 +1172,8 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   if (steps > 0) {
     char *wlp = base + L.wl_off;
     const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)L.ntiles))};
-    k_metro_clear_many<<<64, 256, 0, s>>>(dj, nteams, WL);
-    HS_LAUNCH_CHECK("k_metro_clear_many");
+    k_metro_clear<<<1, 64, 0, s>>>(WL);
+    HS_LAUNCH_CHECK("k_metro_clear");
     const unsigned wgs = (unsigned)(L.ntiles < UPDATE_WGS ? L.ntiles : UPDATE_WGS);
     for (int64_t j = 0; j < steps; ++j) {
       const int par = (int)(j & 1);
