@@ -19,8 +19,9 @@ Other workloads (one JSON line each, same keys):
 
 Prints ONE JSON line. `roofline` is for the dominant kernel: algorithmic bytes per launch /
 average launch duration, measured with HIP events recorded on the launch stream around every
-launch in the timed region. `cpu_baseline` (plain workloads) times the FP64 NumPy restatement of
-the reference on one host core, on a bounded sample of the same workload.
+launch in the timed region. `cpu_baseline` times the FP64 NumPy restatement of the reference on
+a bounded sample of the same workload: on one host core for the plain workloads, and on P
+independent processes (one per physical core, at most 16) for the batched one.
 
     python bench.py [--workload W] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
 """
@@ -263,6 +264,49 @@ def bench_metro(args, large):
 
 # ---- batched ----------------------------------------------------------------------------------
 
+def _cpu_batched_worker(job):
+    """One CPU worker of the batched baseline: builds its share of the 64 IP_Square instances in the
+    FP64 NumPy restatement and steps them round-robin (one Metropolis step each, Q = 0.05) for
+    `seconds`. Returns (attempts, seconds)."""
+    specs, seconds = job
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import model
+    from hotspice_amd.workloads import ipsq_batch_inputs
+    mags = []
+    for sp in specs:
+        p = ipsq_batch_inputs(sp)
+        mags.append(model.make_magnets(
+            "IP_Square", p["a"], n=p["n"], T=p["T"], E_B=p["E_B"], moment=p["moment"], pattern=p["pattern"],
+            energies=[model.Dipolar(cutoff_range=p["cutoff"]), model.Zeeman(*p["field"])],
+            params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=p["reduced_k"],
+                        SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0), seed=p["seed"], rng="numpy"))
+    for m in mags:  # warm-up (calc_r cache, first convolutions)
+        m.update()
+    a0 = sum(m.attempted_switches for m in mags)
+    t0 = time.perf_counter()
+    i = 0
+    while time.perf_counter() - t0 < seconds:
+        mags[i % len(mags)].update()
+        i += 1
+    return sum(m.attempted_switches for m in mags) - a0, time.perf_counter() - t0
+
+
+def cpu_baseline_batched(specs, seconds):
+    """The batched workload on the host the way the reference parallelises it (ParallelJobs.py:64):
+    P = physical cores (at most 16, the CPU share of one GPU) independent processes, each stepping
+    its share of the instances with the FP64 NumPy restatement; aggregate attempts/s."""
+    import multiprocessing as mp
+    import psutil
+    P = max(1, min(16, psutil.cpu_count(logical=False) or 1, len(specs)))
+    with mp.get_context("spawn").Pool(P) as pool:
+        res = pool.map(_cpu_batched_worker, [(specs[i::P], seconds) for i in range(P)])
+    rate = sum(a / dt for a, dt in res)
+    n = sum(a for a, _ in res)
+    return dict(value=rate, unit="switch attempts/s", cores=P, kind="port",
+                sample=f"{n} Metropolis attempts of the 64 instances (round-robin, one step at a time) in "
+                       f"{seconds:.0f} s on {P} processes x 1 thread, FP64 NumPy ({cores_note()})")
+
+
 def bench_batched(args):
     import numpy as np
     import torch
@@ -309,6 +353,7 @@ def bench_batched(args):
                       "accepted": accepted, "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
                       "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
     line["roofline"] = roofline("k_metro_select_many + k_metro_update_many (per batched call)", launches, bytes_total)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batched(specs, args.cpu_seconds)
     return line
 
 

@@ -105,6 +105,18 @@ def ipsq_batch_specs(count=64, seed=3):
     return out
 
 
+def ipsq_batch_inputs(spec, a=1e-6):
+    """The arrays and parameters of ipsq_batch_instance(spec) (same generator draws), for building
+    the same instance elsewhere (the CPU baseline)."""
+    n = spec["n"]
+    rng = np.random.default_rng(spec["seed"])
+    kT = kB * spec["T"]
+    E_B = 20 * kT * rng.normal(1, 0.05, size=(n, n))
+    pattern = np.where(rng.random((n, n)) < 0.5, -1.0, 1.0)
+    return dict(a=a, n=n, T=spec["T"], E_B=E_B, moment=2.37e-16, pattern=pattern, cutoff=edge_cutoff(10 * a / 2),
+                field=(spec["B"], spec["theta"]), reduced_k=10, seed=spec["seed"])
+
+
 def ipsq_batch_instance(spec, a=1e-6):
     import hotspice_amd as hs
     n = spec["n"]
