@@ -60,6 +60,7 @@ class DeviceState:
     # set to a list to collect (start event, end event, state pointers) around every team launch
     launch_events = None
     many_ws = None
+    obs_scratch = None
 
     def stream(self):
         return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
@@ -155,6 +156,24 @@ class DeviceState:
             self.many_ws = self.torch.empty(max(need, 256), dtype=self.torch.uint8, device=self.device)
         _lib.check(self.lib.hs_recompute_fields_many(insts, wh, ptrs, sizes, n, self.ptr(self.many_ws),
                                                      self.many_ws.numel(), self.stream()))
+
+    def bins(self, x_start, y_start, weights=None):
+        """A readout's bins and weights, uploaded once (see Bins)."""
+        return Bins(self, x_start, y_start, weights)
+
+    def binned_sums(self, bins):
+        """hs_binned_sums over `bins` (from DeviceState.bins): the sums of m * w per bin, as a host
+        array of shape (nw, nby, nbx)."""
+        need = int(self.lib.hs_binned_sums_scratch_bytes(C.byref(self.inst), bins.nbx, bins.nw))
+        if need < 0:
+            raise _lib.HotspiceError("bad readout bins")
+        if self.obs_scratch is None or self.obs_scratch.numel() < need:
+            self.obs_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
+        out = self.torch.empty(bins.nw * bins.nby * bins.nbx, dtype=self.torch.float64, device=self.device)
+        _lib.check(self.lib.hs_binned_sums(C.byref(self.inst), bins.nbx, bins.nby, self.ptr(bins.xs), self.ptr(bins.ys),
+                                           self.ptr(bins.w), bins.nw, self.ptr(self.obs_scratch),
+                                           self.obs_scratch.numel(), self.ptr(out), self.stream()))
+        return out.cpu().numpy().reshape(bins.nw, bins.nby, bins.nbx)
 
     def components(self):
         out = self.torch.empty((6, self.ny, self.nx), dtype=self.torch.float64, device=self.device)
@@ -253,3 +272,19 @@ def kernel_window(rrx, rry, nx, ny, cutoff, pbc):
     kwx = int(np.max(np.abs(np.nonzero(cx)[1] - (nx - 1)))) if cx.any() else 0
     kwy = int(np.max(np.abs(np.nonzero(cy)[1] - (ny - 1)))) if cy.any() else 0
     return min(kwx, nx - 1), min(kwy, ny - 1)
+
+
+class Bins:
+    """Bins and weights of a device readout: column bounds x_start (nbx+1 ascending entries from 0
+    to nx), row bounds y_start (nby+1, 0 to ny) and optional FP64 weights (nw, ny, nx)."""
+
+    def __init__(self, dev, x_start, y_start, weights=None):
+        t = dev.torch
+        self.nbx, self.nby = len(x_start) - 1, len(y_start) - 1
+        self.xs = t.tensor(np.asarray(x_start, dtype=np.int32), device=dev.device)
+        self.ys = t.tensor(np.asarray(y_start, dtype=np.int32), device=dev.device)
+        if weights is None:
+            self.w, self.nw = None, 1
+        else:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1, dev.ny, dev.nx))
+            self.w, self.nw = t.tensor(w, device=dev.device), w.shape[0]

@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["hs_core.hip", "hs_neel.hip", "hs_metropolis.hip", "hs_team.hip"]
+SOURCES = ["hs_core.hip", "hs_neel.hip", "hs_metropolis.hip", "hs_team.hip", "hs_observe.hip"]
 OUT = os.path.join(HERE, "libhotspice_hip.so")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # no packed FP32 instructions (v_pk_*_f32) in any kernel

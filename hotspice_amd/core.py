@@ -224,6 +224,7 @@ class Magnets(ABC):
         self.orientation = np.zeros(self.ixx.shape + (2,))
         self.orientation[:, :, 0] = np.cos(self.angles) * self.occupation
         self.orientation[:, :, 1] = np.sin(self.angles) * self.occupation
+        self._obs_all = None  # device readout weights follow the orientation
 
     @property
     def PBC(self):
@@ -940,18 +941,34 @@ class Magnets(ABC):
             yield
         return self.t - t_0, self.MCsteps - MCsteps_0
 
-    # ---- observables (host, on the downloaded state) ----------------------------------------------
+    # ---- observables (summed on the device: hs_binned_sums over one bin) ---------------------------
+    def _m_sums(self):
+        """(sum of m*ox, sum of m*oy) for in-plane lattices, (sum of m,) otherwise, over the grid."""
+        self._sync_m()
+        if getattr(self, "_obs_all", None) is None:
+            w = np.stack([self.orientation[:, :, 0], self.orientation[:, :, 1]]) if self.in_plane else None
+            self._obs_all = self._dev.bins([0, self.nx], [0, self.ny], w)
+        return self._dev.binned_sums(self._obs_all).reshape(-1)
+
     @property
     def m_avg_x(self) -> float:
-        return float(np.mean(np.multiply(self.m, self.orientation[:, :, 0]))) * self.nx * self.ny / self.n if self.in_plane else 0
+        """core.py:413-415: mean(m*ox) * nx*ny / n."""
+        return float(self._m_sums()[0] / (self.nx * self.ny)) * self.nx * self.ny / self.n if self.in_plane else 0
 
     @property
     def m_avg_y(self) -> float:
-        return float(np.mean(np.multiply(self.m, self.orientation[:, :, 1]))) * self.nx * self.ny / self.n if self.in_plane else 0
+        """core.py:416-418"""
+        return float(self._m_sums()[1] / (self.nx * self.ny)) * self.nx * self.ny / self.n if self.in_plane else 0
 
     @property
     def m_avg(self) -> float:
-        return (self.m_avg_x ** 2 + self.m_avg_y ** 2) ** (1 / 2) if self.in_plane else float(np.mean(self.m))
+        """core.py:419-421"""
+        if not self.in_plane:
+            return float(self._m_sums()[0] / (self.nx * self.ny))
+        sx, sy = self._m_sums()[:2]
+        mx = float(sx / (self.nx * self.ny)) * self.nx * self.ny / self.n
+        my = float(sy / (self.nx * self.ny)) * self.nx * self.ny / self.n
+        return (mx ** 2 + my ** 2) ** (1 / 2)
 
     @property
     def m_avg_angle(self) -> float:

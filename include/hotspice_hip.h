@@ -226,6 +226,21 @@ int hs_recompute_fields_many(const hs_instance_t *const *insts, const int32_t *w
 int64_t hs_recompute_many_ws_bytes(const hs_instance_t *const *insts, const int32_t *which, int32_t n);
 
 /*
+ * Readout: out[k*nby*nbx + by*nbx + bx] = sum of m[y][x] * w[k][y][x] over the cells of bin
+ * (bx, by), i.e. columns x_start[bx] <= x < x_start[bx+1] and rows y_start[by] <= y < y_start[by+1]
+ * (device arrays of nbx+1 and nby+1 ascending entries from 0 to nx / ny). w: device [nw][ny][nx]
+ * FP64 weights, or NULL with nw = 1 for weight 1. Each sum is taken in a fixed order, so it is the
+ * same on every call. Replaces the NumPy sums behind Magnets.m_avg_x / m_avg_y / m_avg
+ * (core.py:413-424), RegionalOutputReader.read_state (io.py:335-380) and
+ * OOPSquareChessOutputReader.read_state (io.py:404-445). scratch: device, of
+ * hs_binned_sums_scratch_bytes bytes. Asynchronous.
+ */
+int hs_binned_sums(const hs_instance_t *inst, int32_t nbx, int32_t nby, const int32_t *x_start,
+                   const int32_t *y_start, const double *w, int32_t nw, double *scratch, int64_t scratch_bytes,
+                   double *out, void *stream);
+int64_t hs_binned_sums_scratch_bytes(const hs_instance_t *inst, int32_t nbx, int32_t nw);
+
+/*
  * Per-energy FP64 fields for the host API (Energy.E / Energy.E_perp): out[0..3] planes =
  * Zeeman E, Zeeman E_perp, exchange E, exchange E_perp (closed forms, energies.py:106-118,
  * 771-781) and out[4], out[5] = combined E and E_perp (value - compensation). Dipolar fields
