@@ -15,6 +15,7 @@ from .core import History, Magnets, Scheme, SimParams, kB
 from .energies import DipolarEnergy, Energy, ExchangeEnergy, ZeemanEnergy
 from . import energies
 from . import ASI
+from . import io
 
-__all__ = ['ASI', 'energies', 'utils', 'kB', 'SimParams', 'Magnets', 'Scheme', 'History', 'Energy',
+__all__ = ['ASI', 'energies', 'io', 'utils', 'kB', 'SimParams', 'Magnets', 'Scheme', 'History', 'Energy',
            'ZeemanEnergy', 'DipolarEnergy', 'ExchangeEnergy']
