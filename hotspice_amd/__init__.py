@@ -1,8 +1,8 @@
 """hotspice_amd: the Monte Carlo stepping path of Hotspice on AMD MI355X (gfx950).
 
 Drop-in for the stepping API of bvwaeyen/Hotspice: ``Magnets.update()`` / ``progress()``
-under ``SimParams.UPDATE_SCHEME`` in {Neel, Metropolis}, with ``DipolarEnergy``,
-``ZeemanEnergy`` and ``ExchangeEnergy``. All stepping runs in hand-written HIP kernels
+under ``SimParams.UPDATE_SCHEME`` in {Neel, Metropolis}, with ``DipolarEnergy`` (or
+``energies.DiMonopolarEnergy``), ``ZeemanEnergy`` and ``ExchangeEnergy``. All stepping runs in hand-written HIP kernels
 (libhotspice_hip.so, C-ABI in include/hotspice_hip.h); PyTorch provides device memory.
 
     import hotspice_amd as hotspice

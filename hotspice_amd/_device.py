@@ -41,6 +41,7 @@ class DeviceState:
         self.beta = torch.zeros(n, dtype=torch.float32, device=dev)
         self.mu = torch.zeros(n, dtype=torch.float32, device=dev)
         self.mu64 = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.pw64 = None  # FP64 pair weights when they are not the moments (DiMonopolarEnergy)
         self.occ_list = torch.zeros(max(1, n), dtype=torch.int32, device=dev)
         self.state = torch.zeros(2 * _lib.STATE_BYTES, dtype=torch.uint8, device=dev)
         self.partials = torch.zeros(2 * 1024 * 32, dtype=torch.uint8, device=dev)
@@ -86,7 +87,7 @@ class DeviceState:
         self.state[: _lib.STATE_BYTES].copy_(self.torch.from_numpy(buf.copy()))
 
     # ---- kernels -------------------------------------------------------------------------
-    def build_kernels(self, geom: _lib.KernelGeom, rrx, rry, in_plane):
+    def build_kernels(self, geom: _lib.KernelGeom, rrx, rry, in_plane, mono=None):
         torch = self.torch
         dev = self.device
         kwx, kwy, ns = geom.kwx, geom.kwy, geom.n_sub
@@ -100,8 +101,16 @@ class DeviceState:
         Kps64 = torch.empty(shape, dtype=torch.float64, device=dev) if in_plane else None
         Kpo64 = torch.empty(shape, dtype=torch.float64, device=dev) if in_plane else None
         Kpo32 = torch.empty(shape, dtype=torch.float32, device=dev) if in_plane else None
-        _lib.check(self.lib.hs_build_dipolar_kernels(C.byref(geom), self.ptr(K64), self.ptr(Kps64), self.ptr(Kpo64),
-                                                     self.ptr(K32), self.ptr(Kpo32), self.stream()))
+        if mono is not None:  # DiMonopolarEnergy (rrx / rry: absolute positions; geometry table on the device)
+            raw = np.frombuffer(C.string_at(C.addressof(mono), C.sizeof(mono)), dtype=np.uint8).copy()
+            t_mono = torch.from_numpy(raw).to(dev)
+            _lib.check(self.lib.hs_build_dimonopolar_kernels(C.byref(geom), C.c_void_p(t_mono.data_ptr()), self.ptr(K64),
+                                                             self.ptr(Kps64), self.ptr(Kpo64), self.ptr(K32),
+                                                             self.ptr(Kpo32), self.stream()))
+        else:
+            _lib.check(self.lib.hs_build_dipolar_kernels(C.byref(geom), self.ptr(K64), self.ptr(Kps64),
+                                                         self.ptr(Kpo64), self.ptr(K32), self.ptr(Kpo32),
+                                                         self.stream()))
         torch.cuda.current_stream(dev).synchronize()
         self.kern = dict(K64=K64, K32=K32, Kps64=Kps64, Kpo64=Kpo64, Kpo32=Kpo32, kwx=kwx, kwy=kwy, n_sub=ns,
                          _rr=(t_rrx, t_rry))

@@ -18,6 +18,7 @@ HS_FLAG_NEEL = 2
 TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME = 1, 2, 3, 4, 5
 
 EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs_philox_uniform53", "hs_build_dipolar_kernels",
+           "hs_build_dimonopolar_kernels",
            "hs_recompute_fields", "hs_recompute_scratch_bytes", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
            "hs_metropolis_run", "hs_metropolis_ws_bytes", "hs_neel_run_teams", "hs_neel_teams_ws_bytes",
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
@@ -29,14 +30,14 @@ class Instance(C.Structure):
     _fields_ = [
         ("m", C.c_void_p), ("flags", C.c_void_p), ("E", C.c_void_p), ("Ec", C.c_void_p),
         ("Ep", C.c_void_p), ("Epc", C.c_void_p), ("EB", C.c_void_p), ("beta", C.c_void_p), ("mu", C.c_void_p),
-        ("mu64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
+        ("mu64", C.c_void_p), ("pw64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
         ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p),
         ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
         ("J", C.c_double), ("m_perp_factor", C.c_double),
         ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),
         ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
         ("use_perp", C.c_int32), ("barrier", C.c_int32), ("nn_w", C.c_int32), ("nn_h", C.c_int32),
-        ("n_occ", C.c_int32), ("has_zeeman", C.c_int32), ("uniform_beta", C.c_int32),
+        ("n_occ", C.c_int32), ("has_zeeman", C.c_int32), ("uniform_beta", C.c_int32), ("pair_src_unit", C.c_int32),
         ("sub_of", C.c_int32 * HS_MAX_UC), ("ox_uc", C.c_double * HS_MAX_UC), ("oy_uc", C.c_double * HS_MAX_UC),
         ("nn_mask", C.c_float * HS_MAX_NN),
     ]
@@ -52,6 +53,12 @@ class KernelGeom(C.Structure):
         ("shift_x", C.c_int32 * HS_MAX_UC), ("shift_y", C.c_int32 * HS_MAX_UC),
         ("uc_ox", C.c_double * HS_MAX_UC), ("uc_oy", C.c_double * HS_MAX_UC), ("uc_o", C.c_double * HS_MAX_UC),
     ]
+
+
+class MonopoleGeom(C.Structure):
+    """hs_monopole_geom_t"""
+    _fields_ = [("ca", C.c_double * HS_MAX_UC), ("sa", C.c_double * HS_MAX_UC), ("cp", C.c_double * HS_MAX_UC),
+                ("sp", C.c_double * HS_MAX_UC), ("h", C.c_double * HS_MAX_UC), ("h_small", C.c_double)]
 
 
 class Run(C.Structure):
@@ -108,6 +115,7 @@ def load():
         "hs_device_info": ([C.POINTER(i32), C.c_char_p, i32], C.c_int),
         "hs_philox_uniform53": ([u64, u64, C.c_uint32, i64, i64, i32, vp, vp], C.c_int),
         "hs_build_dipolar_kernels": ([C.POINTER(KernelGeom), vp, vp, vp, vp, vp, vp], C.c_int),
+        "hs_build_dimonopolar_kernels": ([C.POINTER(KernelGeom), vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "hs_recompute_fields": ([C.POINTER(Instance), i32, vp, i64, vp], C.c_int),
         "hs_recompute_scratch_bytes": ([C.POINTER(Instance)], C.c_int64),
         "hs_field_components": ([C.POINTER(Instance), vp, vp], C.c_int),
@@ -148,15 +156,16 @@ def exported_symbols():
 
 
 def abi_layout_python():
-    """The same 18 numbers as hs_abi_layout, from the ctypes mirrors."""
+    """The same 20 numbers as hs_abi_layout, from the ctypes mirrors."""
     return [C.sizeof(Instance), Instance.seed.offset, Instance.nx.offset, Instance.sub_of.offset,
             Instance.nn_mask.offset, C.sizeof(KernelGeom), KernelGeom.nx.offset, KernelGeom.uc_o.offset,
             C.sizeof(Run), Run.max_steps.offset, Run.begin.offset, C.sizeof(State), State.done.offset,
-            C.sizeof(Select), Select.metropolis_time.offset, C.sizeof(Team), Team.run.offset, Team.nblocks.offset]
+            C.sizeof(Select), Select.metropolis_time.offset, C.sizeof(Team), Team.run.offset, Team.nblocks.offset,
+            Instance.pair_src_unit.offset, C.sizeof(MonopoleGeom)]
 
 
 def abi_layout_library():
     lib = load()
-    out = (C.c_int64 * 18)()
-    n = lib.hs_abi_layout(out, 18)
+    out = (C.c_int64 * 20)()
+    n = lib.hs_abi_layout(out, 20)
     return list(out)[:n]

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._device import DeviceState, kernel_window
-from .energies import DipolarEnergy, Energy, ExchangeEnergy, ZeemanEnergy
+from .energies import DiMonopolarEnergy, DipolarEnergy, Energy, ExchangeEnergy, ZeemanEnergy
 from .utils import as_2D_array, check_repetition, full_obj_name, lower_than
 
 kB = 1.380649e-23
@@ -381,8 +381,8 @@ class Magnets(ABC):
         """core.py:349-351"""
         self._moment = np.array(as_2D_array(value, self.shape), dtype=float)
         self._momentSq = self._moment * self._moment
-        self._dev.upload(self._dev.mu, self._moment.astype(np.float32))
         self._dev.upload(self._dev.mu64, self._moment)
+        self._sync_pair_weights()
 
     @property
     def USE_PERP_ENERGY(self):
@@ -462,7 +462,12 @@ class Magnets(ABC):
         """core.py:195-209"""
         if not isinstance(energy, (ZeemanEnergy, DipolarEnergy, ExchangeEnergy)):
             raise NotImplementedError(f"{type(energy).__name__} has no device implementation; supported energies are "
-                                      "ZeemanEnergy, DipolarEnergy and ExchangeEnergy.")
+                                      "ZeemanEnergy, DipolarEnergy, DiMonopolarEnergy and ExchangeEnergy.")
+        if isinstance(energy, DipolarEnergy):
+            other = self._pair_energy()
+            if other is not None and type(other) is not type(energy):
+                raise NotImplementedError(f"{type(energy).__name__} and {type(other).__name__} cannot be combined on the "
+                                          f"device (one pair-interaction kernel set): remove the {type(other).__name__} first.")
         for i, e in enumerate(self._energies):
             if type(e) is type(energy):
                 if exist_ok:
@@ -471,12 +476,14 @@ class Magnets(ABC):
                     warnings.warn(f"An instance of {type(energy).__name__} was already included in the simulation, and has now been overwritten.", stacklevel=2)
                 energy.initialize(self)
                 self._energies[i] = energy
+                self._sync_pair_weights()
                 self._device_params_changed()
                 if _recompute:
                     self._device_recompute()
                 return
         energy.initialize(self)
         self._energies.append(energy)
+        self._sync_pair_weights()
         self._device_params_changed()
         if _recompute:
             self._device_recompute()
@@ -490,6 +497,7 @@ class Magnets(ABC):
                 self._energies.pop(i)
                 if isinstance(e, DipolarEnergy):
                     self._dev.drop_kernels()
+                    self._sync_pair_weights()
                 self._device_params_changed()
                 self._device_recompute()
                 return
@@ -558,9 +566,34 @@ class Magnets(ABC):
         return total - zee - exc
 
     # ---- device instance -------------------------------------------------------------------------
+    def _pair_energy(self):
+        """The DipolarEnergy or DiMonopolarEnergy of this instance (the device's pair kernels), or None."""
+        for e in getattr(self, "_energies", ()):
+            if isinstance(e, DipolarEnergy):
+                return e
+        return None
+
+    def _sync_pair_weights(self):
+        """Uploads each magnet's pair weight: its moment for DipolarEnergy, its squared charge
+        (moment/d)^2 for DiMonopolarEnergy (energies.py:413-415, fixed when that energy is
+        initialised)."""
+        d = self._dev
+        pair = self._pair_energy()
+        if isinstance(pair, DiMonopolarEnergy) and hasattr(pair, "charge_sq"):
+            q2 = np.ascontiguousarray(pair.charge_sq, dtype=np.float64)
+            if d.pw64 is None:
+                d.pw64 = d.torch.empty(self.nx * self.ny, dtype=d.torch.float64, device=d.device)
+            d.upload(d.pw64, q2)
+            d.upload(d.mu, q2.astype(np.float32))
+        else:
+            d.pw64 = None
+            d.upload(d.mu, self._moment.astype(np.float32))
+
     def _device_build_kernels(self, dip: DipolarEnergy):
         """Host part of DipolarEnergy._initialize (energies.py:183-214): offsets of the tiled
         unit-cell positions around each sublattice's centre magnet; the device builds the kernels."""
+        if isinstance(dip, DiMonopolarEnergy):
+            return self._device_build_monopole_kernels(dip)
         ux, uy, nx, ny = self.unitcell.x, self.unitcell.y, self.nx, self.ny
         ncx = 2 * math.ceil(nx / ux) + 1
         ncy = 2 * math.ceil(ny / uy) + 1
@@ -606,6 +639,50 @@ class Magnets(ABC):
         self._dev.build_kernels(g, rrx, rry, self.in_plane)
         self._kernel_sites = sites
 
+    def _device_build_monopole_kernels(self, dim: DiMonopolarEnergy):
+        """Host part of DiMonopolarEnergy._initialize (energies.py:416-463): positions of the
+        enlarged grid's columns and rows seen from each occupied unit-cell site, and the dumbbell
+        geometry of every site; the device builds the kernels and the pair weights follow."""
+        ux, uy, nx, ny = self.unitcell.x, self.unitcell.y, self.nx, self.ny
+        sites = [(y, x) for y in range(uy) for x in range(ux) if self.occupation[y, x] != 0]
+        if not sites:
+            raise ValueError("No occupied site in the unit cell: no DiMonopolar kernel to build.")
+        if len(sites) > _lib.HS_MAX_UC:
+            raise ValueError("Too many unit-cell sites for the device kernels.")
+
+        def large(spacing, n, u):  # positions of indices -(n-1) .. n+u-1, index n-1 at 0 (energies.py:416-428)
+            idx = np.arange(-(n - 1), n + u)
+            d = np.asarray(spacing)[idx % n]
+            pos = np.zeros(idx.size)
+            pos[1:] = np.cumsum(d)[:-1]
+            return pos - pos[n - 1]
+
+        x_large, y_large = large(self.dx, nx, ux), large(self.dy, ny, uy)
+        g = _lib.KernelGeom()
+        rrx = np.zeros((len(sites), 2 * nx - 1))
+        rry = np.zeros((len(sites), 2 * ny - 1))
+        for n, (y, x) in enumerate(sites):
+            rrx[n] = x_large[x:x + 2 * nx - 1]
+            rry[n] = y_large[y:y + 2 * ny - 1]
+            g.site_x[n], g.site_y[n] = x, y
+            g.shift_x[n] = (ux - ((nx - 1) % ux) + x) % ux
+            g.shift_y[n] = (uy - ((ny - 1) % uy) + y) % uy
+        ang = np.asarray(self.original_angles[:uy, :ux], dtype=float)
+        ca, sa = np.cos(ang), np.sin(ang)
+        cp, sp = np.cos(ang + np.pi / 2), np.sin(ang + np.pi / 2)
+        mg = _lib.MonopoleGeom()
+        for s in range(uy * ux):
+            y, x = divmod(s, ux)
+            g.uc_o[s] = float(self.occupation[y, x])
+            mg.ca[s], mg.sa[s], mg.cp[s], mg.sp[s] = ca[y, x], sa[y, x], cp[y, x], sp[y, x]
+            mg.h[s] = float(dim.dist_unitcell[y, x]) / 2
+        mg.h_small = float(dim.small_d) / 2
+        g.nx, g.ny, g.ux, g.uy, g.n_sub, g.kwx, g.kwy = nx, ny, ux, uy, len(sites), nx - 1, ny - 1
+        g.pbc, g.in_plane = int(self.PBC), 1
+        self._dev.build_kernels(g, rrx, rry, True, mono=mg)
+        self._kernel_sites = sites
+        self._sync_pair_weights()
+
     def _device_params_changed(self):
         """(Re)fills the hs_instance_t the kernels read."""
         d = self._dev
@@ -630,7 +707,9 @@ class Magnets(ABC):
             if self.in_plane:
                 I.ox_uc[s] = float(self.orientation[y, x, 0])
                 I.oy_uc[s] = float(self.orientation[y, x, 1])
-        dip = self.get_energy('dipolar', verbose=False)
+        dip = self._pair_energy()
+        I.pw64 = P(d.pw64) if d.pw64 is not None else P(d.mu64)
+        I.pair_src_unit = int(isinstance(dip, DiMonopolarEnergy))
         if dip is not None and d.kern is not None:
             k = d.kern
             I.dip_pf = float(dip.prefactor)

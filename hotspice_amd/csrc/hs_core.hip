@@ -55,7 +55,8 @@ extern "C" int hs_abi_layout(int64_t *out, int32_t n) {
       (int64_t)sizeof(hs_run_t), (int64_t)offsetof(hs_run_t, max_steps), (int64_t)offsetof(hs_run_t, begin),
       (int64_t)sizeof(hs_state_t), (int64_t)offsetof(hs_state_t, done),
       (int64_t)sizeof(hs_select_t), (int64_t)offsetof(hs_select_t, metropolis_time),
-      (int64_t)sizeof(hs_team_t), (int64_t)offsetof(hs_team_t, run), (int64_t)offsetof(hs_team_t, nblocks)};
+      (int64_t)sizeof(hs_team_t), (int64_t)offsetof(hs_team_t, run), (int64_t)offsetof(hs_team_t, nblocks),
+      (int64_t)offsetof(hs_instance_t, pair_src_unit), (int64_t)sizeof(hs_monopole_geom_t)};
   const int cnt = (int)(sizeof(v) / sizeof(v[0]));
   if (!out || n < cnt) return cnt;
   for (int i = 0; i < cnt; ++i) out[i] = v[i];
@@ -76,6 +77,7 @@ int hs_check_instance(const hs_instance_t *I) {
   if (I->use_perp && (!I->Ep || !I->Epc)) return hs_set_error(HS_ERR_ARG, "use_perp without E_perp arrays");
   if (I->dip_pf != 0.0) {
     if (!I->K32 || !I->K64) return hs_set_error(HS_ERR_ARG, "dipolar energy without kernels");
+    if (!I->pw64) return hs_set_error(HS_ERR_ARG, "pair energy without FP64 weights");
     if (I->use_perp && (!I->Kpo32 || !I->Kps64)) return hs_set_error(HS_ERR_ARG, "perp fields without perp kernels");
     if (I->kwx < 0 || I->kwx > I->nx - 1 || I->kwy < 0 || I->kwy > I->ny - 1)
       return hs_set_error(HS_ERR_ARG, "kernel window %d,%d outside grid", I->kwx, I->kwy);
@@ -200,6 +202,124 @@ extern "C" int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, 
   const int bs = 256;
   k_build_kernels<<<(unsigned)((total + bs - 1) / bs), bs, 0, (hipStream_t)stream>>>(*g, K64, Kps64, Kpo64, K32, Kpo32);
   HS_LAUNCH_CHECK("k_build_kernels");
+  return HS_OK;
+}
+
+// ---- DiMonopolarEnergy kernels (energies.py:406-641) -------------------------------------------
+// One magnet = charge +1 at (x + h cos a, y + h sin a) and -1 at (x - h cos a, y - h sin a). The
+// positions, differences and squares are evaluated in the reference's order without contraction,
+// so the FP64 squared distances (and their FP32 roundings for K) are the reference's.
+struct Dumbbell {
+  double nx_, ny_, sx_, sy_;
+};
+
+HS_INLINE Dumbbell dumbbell(double x, double y, double h, double c, double s) {
+#pragma clang fp contract(off)
+  const double hc = h * c, hs = h * s;
+  return Dumbbell{x + hc, y + hs, x - hc, y - hs};
+}
+
+// 1/|a-b|, 0 where the charges coincide (energies.py:516-525): FP32 square root of the FP32-rounded
+// square for K, FP64 for the perpendicular kernels.
+template <bool F32>
+HS_INLINE double inv_dist(double ax, double ay, double bx, double by) {
+#pragma clang fp contract(off)
+  const double dx = ax - bx, dy = ay - by;
+  const double sq = dx * dx + dy * dy;
+  if (F32) {
+    const float f = (float)sq;
+    if (f == 0.f) return 0.0;
+    return (double)(float)(1.0 / sqrt((double)f));
+  }
+  if (sq == 0.0) return 0.0;
+  return pow(sq, -0.5);
+}
+
+// NN + SS - NS - SN between the target magnet t and the centre magnet c; the cross terms of the
+// centre with itself are dropped (energies.py:494-498).
+template <bool F32>
+HS_INLINE double pair_sum(const Dumbbell &t, const Dumbbell &c, bool centre) {
+#pragma clang fp contract(off)
+  const double nn = inv_dist<F32>(t.nx_, t.ny_, c.nx_, c.ny_);
+  const double ss = inv_dist<F32>(t.sx_, t.sy_, c.sx_, c.sy_);
+  const double ns = centre ? 0.0 : inv_dist<F32>(t.nx_, t.ny_, c.sx_, c.sy_);
+  const double sn = centre ? 0.0 : inv_dist<F32>(t.sx_, t.sy_, c.nx_, c.ny_);
+  return nn + ss + -ns + -sn;
+}
+
+HS_INLINE KernVals mono_entry(const hs_kernel_geom_t &g, const hs_monopole_geom_t &mg, int n, int R, int C) {
+  KernVals v{0.0, 0.0, 0.0};
+  const int s1 = g.site_y[n] * g.ux + g.site_x[n];
+  const int s2 = ((R + g.shift_y[n]) % g.uy) * g.ux + ((C + g.shift_x[n]) % g.ux);
+  const double occ = g.uc_o[s2];
+  if (occ == 0.0) return v;
+  const bool centre = R == g.ny - 1 && C == g.nx - 1;
+  const double *X = g.rrx + (size_t)n * (2 * g.nx - 1), *Y = g.rry + (size_t)n * (2 * g.ny - 1);
+  const double xc = X[g.nx - 1], yc = Y[g.ny - 1], xt = X[C], yt = Y[R];
+  const Dumbbell cN = dumbbell(xc, yc, mg.h[s1], mg.ca[s1], mg.sa[s1]);     // centre, as it is
+  const Dumbbell cP = dumbbell(xc, yc, mg.h_small, mg.cp[s1], mg.sp[s1]);  // centre, perpendicular
+  const Dumbbell tN = dumbbell(xt, yt, mg.h[s2], mg.ca[s2], mg.sa[s2]);
+  v.k = pair_sum<true>(tN, cN, centre) * occ;
+  if (centre) {  // every term of the centre entry vanishes
+    v.k = v.ps = v.po = 0.0;
+    return v;
+  }
+  v.ps = pair_sum<false>(tN, cP, false) * occ;
+  const Dumbbell tP = dumbbell(xt, yt, mg.h_small, mg.cp[s2], mg.sp[s2]);
+  v.po = pair_sum<false>(tP, cN, false) * occ;
+  return v;
+}
+
+__global__ void k_build_mono_kernels(hs_kernel_geom_t g, const hs_monopole_geom_t *__restrict__ mgp, double *K64,
+                                     double *Kps64, double *Kpo64, float *K32, float *Kpo32) {
+  const hs_monopole_geom_t &mg = *mgp;
+  const int W = 2 * g.nx - 1, H = 2 * g.ny - 1;
+  const int64_t per = (int64_t)W * H;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * g.n_sub) return;
+  const int n = (int)(i / per);
+  const int rem = (int)(i % per);
+  const int R = rem / W, C = rem % W;
+  const int nx = g.nx, ny = g.ny;
+  KernVals v = mono_entry(g, mg, n, R, C);
+  if (g.pbc) {  // energies.py:610-623, statement order
+    if (C >= nx) kv_add(v, mono_entry(g, mg, n, R, C - nx));
+    if (R >= ny && C >= nx) kv_add(v, mono_entry(g, mg, n, R - ny, C - nx));
+    if (R >= ny) kv_add(v, mono_entry(g, mg, n, R - ny, C));
+    if (R >= ny && C <= nx - 2) kv_add(v, mono_entry(g, mg, n, R - ny, C + nx));
+    if (C <= nx - 2) kv_add(v, mono_entry(g, mg, n, R, C + nx));
+    if (R <= ny - 2 && C <= nx - 2) kv_add(v, mono_entry(g, mg, n, R + ny, C + nx));
+    if (R <= ny - 2) kv_add(v, mono_entry(g, mg, n, R + ny, C));
+    if (R <= ny - 2 && C >= nx) kv_add(v, mono_entry(g, mg, n, R + ny, C - nx));
+  }
+  v.k *= 1e-7;
+  v.ps *= 1e-7;
+  v.po *= 1e-7;
+  K64[i] = v.k;
+  K32[i] = (float)v.k;
+  if (Kps64) Kps64[i] = v.ps;
+  if (Kpo64) Kpo64[i] = v.po;
+  if (Kpo32) Kpo32[i] = (float)v.po;
+}
+
+extern "C" int hs_build_dimonopolar_kernels(const hs_kernel_geom_t *g, const hs_monopole_geom_t *mg, double *K64,
+                                            double *Kps64, double *Kpo64, float *K32, float *Kpo32, void *stream) {
+  hs_clear_error();
+  if (!g || !mg || !g->rrx || !g->rry || !K64 || !K32) return hs_set_error(HS_ERR_ARG, "missing kernel arrays");
+  if (g->nx <= 0 || g->ny <= 0) return hs_set_error(HS_ERR_ARG, "bad grid %dx%d", g->nx, g->ny);
+  if (g->n_sub <= 0 || g->n_sub > HS_MAX_UC || g->ux <= 0 || g->uy <= 0 || g->ux * g->uy > HS_MAX_UC)
+    return hs_set_error(HS_ERR_ARG, "bad unit cell / n_sub");
+  if (g->kwx != g->nx - 1 || g->kwy != g->ny - 1)
+    return hs_set_error(HS_ERR_ARG, "DiMonopolar kernels use the full window (kwx = nx-1, kwy = ny-1)");
+  for (int n = 0; n < g->n_sub; ++n)
+    if (g->site_x[n] < 0 || g->site_x[n] >= g->ux || g->site_y[n] < 0 || g->site_y[n] >= g->uy ||
+        g->shift_x[n] < 0 || g->shift_y[n] < 0)
+      return hs_set_error(HS_ERR_ARG, "bad site %d", n);
+  const int64_t total = (int64_t)(2 * g->nx - 1) * (2 * g->ny - 1) * g->n_sub;
+  const int bs = 256;
+  k_build_mono_kernels<<<(unsigned)((total + bs - 1) / bs), bs, 0, (hipStream_t)stream>>>(*g, mg, K64, Kps64, Kpo64,
+                                                                                         K32, Kpo32);
+  HS_LAUNCH_CHECK("k_build_mono_kernels");
   return HS_OK;
 }
 
@@ -345,7 +465,7 @@ __device__ __forceinline__ void rec_dip(const hs_instance_t &I, const RecPlan &p
           double v = 0.0;
           if (a <= r1 && b <= c1 && y < I.ny && x < I.nx) {
             const int cc = y * I.nx + x;
-            v = (double)I.m[cc] * I.mu64[cc];
+            v = I.pair_src_unit ? (double)I.m[cc] : (double)I.m[cc] * I.pw64[cc];
           }
           sS[e] = v;
         }
@@ -434,8 +554,10 @@ __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const 
   const bool do_e = which & 1, do_p = (which & 2) && I.use_perp;
   double e = 0.0, ep = 0.0;
   if (mi != 0.0) {
-    const double mu = I.mu64[c];
-    if (n >= 0) {  // prefactor*moment*(partial_m*conv) (energies.py:303, 307)
+    // prefactor*moment*(partial_m*conv) (energies.py:303, 307); DiMonopolarEnergy:
+    // prefactor*(partial_m*conv(m))*charge_sq (energies.py:654-659)
+    if (n >= 0) {
+      const double mu = I.pw64[c];
       double acc = 0.0, accp = 0.0;
       for (int g = 0; g < F; ++g) {
         if (do_e) acc += partE[(size_t)g * ncell + c];
@@ -481,7 +603,8 @@ __global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, 
 __device__ __forceinline__ void rec_small(const hs_instance_t &I, const double *Kbase, double *part, int c) {
   __shared__ double s_v[4096];
   const int nx = I.nx, ny = I.ny, ncell = nx * ny;
-  for (int s = threadIdx.x; s < ncell; s += blockDim.x) s_v[s] = (double)I.m[s] * I.mu64[s];
+  for (int s = threadIdx.x; s < ncell; s += blockDim.x)
+    s_v[s] = I.pair_src_unit ? (double)I.m[s] : (double)I.m[s] * I.pw64[s];
   __syncthreads();
   if (c >= ncell) return;
   const int y = c / nx, x = c % nx, n = I.sub_of[hs_site(I, y, x)];

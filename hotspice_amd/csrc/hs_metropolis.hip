@@ -502,7 +502,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
         ecv[u] = I.Ec[cs[u]];
         btv[u] = I.beta[cs[u]];
         mv[u] = (int)I.m[cs[u]];
-        muv[u] = I.mu[cs[u]];
+        muv[u] = I.pair_src_unit ? 1.f : I.mu[cs[u]];  // source weight (1 for DiMonopolarEnergy)
       }
     }
 #pragma unroll

@@ -1,7 +1,7 @@
-"""Energy plugins of the stepping path: ZeemanEnergy, DipolarEnergy, ExchangeEnergy.
+"""Energy plugins of the stepping path: ZeemanEnergy, DipolarEnergy, DiMonopolarEnergy, ExchangeEnergy.
 
 Same classes and members as hotspice/energies.py (Energy contract 22-83, ZeemanEnergy 86-140,
-DipolarEnergy 143-391, ExchangeEnergy 754-791). Unlike the reference, the per-magnet fields
+DipolarEnergy 143-391, DiMonopolarEnergy 394-751, ExchangeEnergy 754-791). Unlike the reference, the per-magnet fields
 live on the GPU as ONE combined FP32 local field with Kahan compensation (the sum of every
 energy's E), updated by the HIP kernels of libhotspice_hip.so. The per-energy arrays E and
 E_perp are reconstructed on demand (FP64) from that combined field and the closed forms of
@@ -212,6 +212,44 @@ class DipolarEnergy(Energy):
     def set_NN_interaction(self, value):
         """energies.py:389-391"""
         self.prefactor = value / self.get_NN_interaction()
+
+
+class DiMonopolarEnergy(DipolarEnergy):
+    """energies.py:394-751. Every magnet is a dumbbell of two opposite magnetic charges
+    q = moment/d, d apart along the magnet's angle; the interaction is the Coulomb-like sum of
+    the four charge pairs. The kernels are built on the device (hs_build_dimonopolar_kernels)
+    and the stepping kernels are the DipolarEnergy ones, with the pair term
+    prefactor * K * m_source * m_receiver * q_receiver^2 (energies.py:654-659, 669-671, 707-710).
+    The charges are taken from the moment when the energy is initialised, as in the reference.
+    Replaces a DipolarEnergy of the same Magnets (the device holds one pair-interaction kernel set).
+    """
+
+    def __init__(self, prefactor=1, d=200e-9, small_d=200e-9):
+        self._prefactor = prefactor
+        self.d = d
+        self.small_d = small_d
+
+    def _initialize(self):
+        """energies.py:406-415; the kernel construction (416-641) runs on the device."""
+        mm = self.mm
+        if not mm.in_plane:
+            raise ValueError("DiMonopolarEnergy needs an in-plane ASI (magnet angles).")
+        ux, uy = mm.unitcell.x, mm.unitcell.y
+        if isinstance(self.d, np.ndarray):
+            self.dist_unitcell = np.asarray(self.d, dtype=float)
+            if self.dist_unitcell.shape != (uy, ux):
+                raise ValueError(f"d must be a scalar or a ({uy}, {ux}) unit-cell array.")
+        else:
+            self.dist_unitcell = as_2D_array(self.d, (uy, ux))
+        self.dist = np.tile(self.dist_unitcell, (math.ceil(mm.ny / uy), math.ceil(mm.nx / ux)))
+        self.dist_exact = self.dist[:mm.ny, :mm.nx]
+        self.charge = mm.moment / self.dist_exact  # [Am]
+        self.charge_sq = self.charge ** 2
+        mm._device_build_kernels(self)
+
+    @property
+    def cutoff_range(self):
+        return math.inf
 
 
 class ExchangeEnergy(Energy):

@@ -53,8 +53,10 @@ typedef struct hs_instance {
   float *Ep, *Epc;      /* combined perpendicular field (without m_perp_factor) */
   const float *EB;      /* (ny,nx) E_B */
   const float *beta;    /* (ny,nx) 1/(kB T) */
-  const float *mu;      /* (ny,nx) moment, FP32 (stepping) */
-  const double *mu64;   /* (ny,nx) moment, FP64 (recompute, totals, Zeeman) */
+  const float *mu;      /* (ny,nx) pair weight of each magnet, FP32 (stepping): its moment for
+                           DipolarEnergy, its squared charge (moment/d)^2 for DiMonopolarEnergy */
+  const double *mu64;   /* (ny,nx) moment, FP64 (Zeeman, totals) */
+  const double *pw64;   /* (ny,nx) pair weight, FP64 (recompute); == mu64 for DipolarEnergy */
   const float *K32;     /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP32 (stepping) */
   const float *Kpo32;   /* same shape, K_perpother (stepping E_perp) */
   const double *K64;    /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP64 (recompute) */
@@ -79,6 +81,8 @@ typedef struct hs_instance {
   int32_t n_occ;        /* Magnets.n */
   int32_t has_zeeman;
   int32_t uniform_beta; /* 1 when T is uniform over the grid */
+  int32_t pair_src_unit; /* 0: a pair term is prefactor*K*(m w)_source*(m w)_receiver (DipolarEnergy);
+                            1: prefactor*K*m_source*(m w)_receiver (DiMonopolarEnergy), w = mu/pw64 */
   int32_t sub_of[HS_MAX_UC];    /* unit-cell site (y%uy)*ux + x%ux -> kernel index or -1 */
   double ox_uc[HS_MAX_UC];      /* in-plane orientation per unit-cell site */
   double oy_uc[HS_MAX_UC];
@@ -101,6 +105,14 @@ typedef struct hs_kernel_geom {
   double uc_ox[HS_MAX_UC], uc_oy[HS_MAX_UC]; /* orientation per site (IP) */
   double uc_o[HS_MAX_UC];                    /* occupation per site (OOP) */
 } hs_kernel_geom_t;
+
+/* Dumbbell geometry of DiMonopolarEnergy (energies.py:406-463), per unit-cell site s = y*ux + x. */
+typedef struct hs_monopole_geom {
+  double ca[HS_MAX_UC], sa[HS_MAX_UC];  /* cos, sin of the site's original angle */
+  double cp[HS_MAX_UC], sp[HS_MAX_UC];  /* cos, sin of (that angle + pi/2) */
+  double h[HS_MAX_UC];                  /* d/2: half the distance between the site's two charges */
+  double h_small;                       /* small_d/2, for a magnet turned perpendicular */
+} hs_monopole_geom_t;
 
 /* Run control for a stepping call (Magnets.progress, core.py:736-760). */
 typedef struct hs_run {
@@ -198,6 +210,23 @@ int hs_philox_uniform53(uint64_t seed, uint64_t step, uint32_t tag, int64_t inde
  */
 int hs_build_dipolar_kernels(const hs_kernel_geom_t *g, double *K64, double *Kps64, double *Kpo64,
                              float *K32, float *Kpo32, void *stream);
+
+/*
+ * The same three kernels for DiMonopolarEnergy (energies.py:406-641): every magnet is a pair of
+ * opposite charges d apart along its angle, and an entry is the sum of the four charge-charge
+ * 1/r terms (like charges +, unlike -) between the centre magnet and the magnet at that offset,
+ * times the target site's occupation. K takes the squared distances in FP32 as the reference
+ * does (energies.py:508-525); K_perpself turns the centre magnet by +90 degrees with charges
+ * small_d apart, K_perpother every other magnet. PBC folding and mu0/4pi as for the dipolar
+ * kernels. The window is always the full one (kwx = nx-1, kwy = ny-1). Here g->rrx[n][C] and
+ * g->rry[n][R] hold ABSOLUTE positions of the columns/rows of the reference's enlarged grid seen
+ * from kernel n (energies.py:416-428): the centre magnet sits at C = nx-1, R = ny-1; g->uc_o is
+ * the occupation per unit-cell site, g->I / cutoff / decay / in_plane are unused. mg is a DEVICE
+ * pointer (the per-site tables are read by every thread; passed by value they would exceed the
+ * kernel-argument space together with g).
+ */
+int hs_build_dimonopolar_kernels(const hs_kernel_geom_t *g, const hs_monopole_geom_t *mg, double *K64,
+                                 double *Kps64, double *Kpo64, float *K32, float *Kpo32, void *stream);
 
 /*
  * FP64 full recompute of the combined field (and E_perp when which&2), written as FP32 +

@@ -16,6 +16,7 @@ Restated reference code (file:line in bvwaeyen/Hotspice):
   ZeemanEnergy ............... hotspice/energies.py:86-140
   DipolarEnergy kernels ...... hotspice/energies.py:176-288
   DipolarEnergy.update ....... hotspice/energies.py:290-308
+  DiMonopolarEnergy .......... hotspice/energies.py:394-751
   update_single .............. hotspice/energies.py:310-329
   update_multiple ............ hotspice/energies.py:331-366
   ExchangeEnergy ............. hotspice/energies.py:754-791
@@ -178,7 +179,7 @@ class Dipolar(Energy):
         ux, uy = mm.ux, mm.uy
         total = np.zeros_like(mm.m)
         total_perp = np.zeros_like(mm.m)
-        mmoment = mm.m * mm.moment
+        mmoment = mm.m * self._w_src()
         for y in range(uy):
             for x in range(ux):
                 n = self.kernel_unitcell_indices[y, x]
@@ -189,9 +190,9 @@ class Dipolar(Energy):
                 total += partial_m * self._conv_valid(self.kernel_unitcell[n, ::-1, ::-1], mmoment)
                 if mm.USE_PERP_ENERGY:
                     total_perp += partial_m * self._conv_valid(self.kernel_perpself_unitcell[n, ::-1, ::-1], mmoment)
-        self.E = self.prefactor * mm.moment * total
+        self.E = self.prefactor * self._w_rx() * total
         if mm.USE_PERP_ENERGY:
-            self.E_perp = self.prefactor * mm.moment * total_perp
+            self.E_perp = self.prefactor * self._w_rx() * total_perp
 
     def update_single(self, idx):
         mm = self.mm
@@ -200,18 +201,19 @@ class Dipolar(Energy):
         if n < 0:
             return
         sl = np.s_[mm.ny - 1 - y:2 * mm.ny - 1 - y, mm.nx - 1 - x:2 * mm.nx - 1 - x]
-        mmoment = mm.m * mm.moment
-        self.E += 2 * (self.prefactor * mmoment[y, x] * np.multiply(mmoment, self.kernel_unitcell[n][sl]))
+        msrc, mrx = mm.m * self._w_src(), mm.m * self._w_rx()
+        self.E += 2 * (self.prefactor * msrc[y, x] * np.multiply(mrx, self.kernel_unitcell[n][sl]))
         self.E[y, x] *= -1
         if mm.USE_PERP_ENERGY:
-            self.E_perp += 2 * (self.prefactor * mmoment[y, x] * np.multiply(mmoment, self.kernel_perpother_unitcell[n][sl]))
+            self.E_perp += 2 * (self.prefactor * msrc[y, x] * np.multiply(mrx, self.kernel_perpother_unitcell[n][sl]))
             self.E_perp[y, x] *= -1
 
     def update_multiple(self, idx):
         mm = self.mm
         idx = np.asarray(idx)
         self.E[idx[0], idx[1]] *= -1
-        mmoment = mm.m * mm.moment
+        mmoment = mm.m * self._w_src()
+        mrx = mm.m * self._w_rx()
         if mm.USE_PERP_ENERGY:
             self.E_perp[idx[0], idx[1]] *= -1
         raveled = (idx[1] % mm.ux) + (idx[0] % mm.uy) * mm.ux
@@ -243,13 +245,132 @@ class Dipolar(Energy):
                     conv += mmoment[y, x] * kernel[sl]
                     if mm.USE_PERP_ENERGY:
                         conv_perp += mmoment[y, x] * kernel_perp[sl]
-            self.E += 2 * (self.prefactor * np.multiply(mmoment, conv))
+            self.E += 2 * (self.prefactor * np.multiply(mrx, conv))
             if mm.USE_PERP_ENERGY:
-                self.E_perp += 2 * (self.prefactor * np.multiply(mmoment, conv_perp))
+                self.E_perp += 2 * (self.prefactor * np.multiply(mrx, conv_perp))
+
+    def _w_src(self):
+        """Weight of the source magnet of a pair term: its moment (energies.py:318, 341)."""
+        return self.mm.moment
+
+    def _w_rx(self):
+        """Weight of the receiving magnet of a pair term: its moment (energies.py:303, 318)."""
+        return self.mm.moment
 
     @property
     def E_tot(self):
         return np.sum(self.E) / 2
+
+
+class DiMonopolar(Dipolar):
+    """energies.py:394-751: dumbbells of charges q = moment/d. A pair term is
+    prefactor * K * m_source * m_receiver * q_receiver^2 (energies.py:654-659, 669-671, 707-710)."""
+
+    def __init__(self, prefactor=1.0, d=200e-9, small_d=200e-9, recompute="auto"):
+        self.prefactor = prefactor
+        self.d = d
+        self.small_d = small_d
+        self.recompute = recompute
+
+    def _initialize(self):
+        mm = self.mm
+        dist_uc = np.asarray(self.d, dtype=float) if isinstance(self.d, np.ndarray) else as_2D_array(self.d, (mm.uy, mm.ux))
+        self.dist_unitcell = dist_uc
+        dist = np.tile(dist_uc, (math.ceil(mm.ny / mm.uy), math.ceil(mm.nx / mm.ux)))
+        self.dist_exact = dist[:mm.ny, :mm.nx]
+        self.charge = mm.moment / self.dist_exact
+        self.charge_sq = self.charge ** 2
+        K, Kps, Kpo, idx = build_monopole_kernels(mm, dist_uc, self.small_d)
+        self.kernel_unitcell, self.kernel_perpself_unitcell, self.kernel_perpother_unitcell = K, Kps, Kpo
+        self.kernel_unitcell_indices = idx
+
+    def _w_src(self):
+        return np.ones_like(self.mm.m)
+
+    def _w_rx(self):
+        return self.charge_sq
+
+
+def _large_positions(spacing, n, u):
+    """Positions of the enlarged grid's indices -(n-1) .. n+u-1 with index n-1 at 0 (energies.py:416-428)."""
+    idx = np.arange(-(n - 1), n + u)
+    steps = np.asarray(spacing)[idx % n]
+    pos = np.zeros(idx.size)
+    pos[1:] = np.cumsum(steps)[:-1]
+    return pos - pos[n - 1]
+
+
+def _inv_dist(dx, dy, f32):
+    """|r|^-1 with 0 where the charges coincide; FP32 squares for the plain kernel (energies.py:508-525)."""
+    sq = dx ** 2 + dy ** 2
+    if f32:
+        sq = sq.astype(np.float32)
+    sq[sq == 0] = np.inf
+    return sq ** -0.5
+
+
+def build_monopole_kernels(mm, dist_uc, small_d):
+    """DiMonopolarEnergy._initialize kernels (energies.py:416-641): K, K_perpself, K_perpother of
+    shape (n_sub, 2ny-1, 2nx-1) and the (uy, ux) site->kernel index map."""
+    ux, uy, nx, ny = mm.ux, mm.uy, mm.nx, mm.ny
+    xl, yl = _large_positions(mm.dx, nx, ux), _large_positions(mm.dy, ny, uy)
+    XX, YY = np.meshgrid(xl, yl)
+    iy = (np.arange(yl.size) - (ny - 1)) % uy
+    ix = (np.arange(xl.size) - (nx - 1)) % ux
+    ang_uc = np.where(mm.occupation[:uy, :ux] != 0, mm.geom.angles[:uy, :ux], 0.0)
+    ang = ang_uc[np.ix_(iy, ix)]
+    half = (dist_uc / 2)[np.ix_(iy, ix)]
+    occ = mm.occupation[:uy, :ux][np.ix_(iy, ix)].astype(float)
+    # charges (+ then -) of every magnet as it is, and turned by +90 degrees with small_d
+    cx = (XX + half * np.cos(ang), XX - half * np.cos(ang))
+    cy = (YY + half * np.sin(ang), YY - half * np.sin(ang))
+    px = (XX + small_d / 2 * np.cos(ang + np.pi / 2), XX - small_d / 2 * np.cos(ang + np.pi / 2))
+    py = (YY + small_d / 2 * np.sin(ang + np.pi / 2), YY - small_d / 2 * np.sin(ang + np.pi / 2))
+    index = -np.ones((uy, ux), dtype=int)
+    Ks, Kps, Kpo = [], [], []
+    for y0 in range(uy):
+        for x0 in range(ux):
+            if mm.occupation[y0, x0] == 0:
+                continue
+            Yc, Xc = ny - 1 + y0, nx - 1 + x0
+            sl = np.s_[y0:y0 + 2 * ny - 1, x0:x0 + 2 * nx - 1]
+            cen = (ny - 1, nx - 1)
+
+            def combine(tx, ty, sx, sy, f32):
+                """NN + SS - NS - SN between targets (tx, ty) and the centre's charges (sx, sy)."""
+                terms = []
+                for a, b in ((0, 0), (1, 1), (0, 1), (1, 0)):
+                    dx = tx[a][sl] - sx[b]
+                    dy = ty[a][sl] - sy[b]
+                    if a != b:  # no cross term of the centre with itself (energies.py:494-498)
+                        dx[cen] = 0
+                        dy[cen] = 0
+                    terms.append(_inv_dist(dx, dy, f32) * occ[sl])
+                return terms[0] + terms[1] + -terms[2] + -terms[3]
+
+            c_n = ((cx[0][Yc, Xc], cx[1][Yc, Xc]), (cy[0][Yc, Xc], cy[1][Yc, Xc]))
+            c_p = ((px[0][Yc, Xc], px[1][Yc, Xc]), (py[0][Yc, Xc], py[1][Yc, Xc]))
+            kernel = combine(cx, cy, c_n[0], c_n[1], True)
+            # perp-self: the centre turned; at the centre itself every term vanishes
+            tx_s = tuple(a.copy() for a in cx)
+            ty_s = tuple(a.copy() for a in cy)
+            for q in range(2):
+                tx_s[q][Yc, Xc], ty_s[q][Yc, Xc] = c_p[0][q], c_p[1][q]
+            kernel_ps = combine(tx_s, ty_s, c_p[0], c_p[1], False)
+            # perp-other: every other magnet turned, the centre as it is
+            tx_o = tuple(a.copy() for a in px)
+            ty_o = tuple(a.copy() for a in py)
+            for q in range(2):
+                tx_o[q][Yc, Xc], ty_o[q][Yc, Xc] = c_n[0][q], c_n[1][q]
+            kernel_po = combine(tx_o, ty_o, c_n[0], c_n[1], False)
+            if mm.PBC:
+                for k in (kernel, kernel_ps, kernel_po):
+                    _apply_pbc(k, nx, ny)
+            index[y0, x0] = len(Ks)
+            Ks.append(kernel * 1e-7)
+            Kps.append(kernel_ps * 1e-7)
+            Kpo.append(kernel_po * 1e-7)
+    return np.asarray(Ks), np.asarray(Kps), np.asarray(Kpo), index
 
 
 def build_kernels(mm, decay_exponent=-3, cutoff_range=np.inf):

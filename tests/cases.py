@@ -61,6 +61,17 @@ SYNTH = {
                                  params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
                                              SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
                                  kw=dict(PBC=True), update_kw=dict(r=10e-6), nsteps=60, seed=34),
+    # DiMonopolarEnergy: the same stepping kernels with dumbbell kernels and charge weights
+    "neel_kagome_mono": dict(lattice="IP_Kagome", a=1e-6, n=40, T=300, moment=2.37e-16,
+                             energies=[("dimonopolar", dict(d=350e-9, small_d=150e-9)),
+                                       ("zeeman", dict(magnitude=5e-4, angle=0.3))],
+                             params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=np.inf), nsteps=150, seed=35),
+    "metro_ipsq_mono_pbc": dict(lattice="IP_Square", a=1e-6, n=24, T=300, moment=2.37e-16,
+                                energies=[("dimonopolar", dict(d=400e-9, small_d=200e-9)),
+                                          ("zeeman", dict(magnitude=1e-3, angle=0.4))],
+                                params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
+                                            SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=2),
+                                kw=dict(PBC=True), update_kw=dict(r=2e-6), nsteps=120, seed=36),
     "metro_ipsq_window": dict(lattice="IP_Square", a=1e-6, n=48, T=400, moment=2.37e-16,
                               energies=[("dipolar", dict(cutoff_range=3e-6)), ("zeeman", dict(magnitude=1e-3, angle=0.7))],
                               params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,
@@ -75,12 +86,14 @@ def spec(name):
 
 def load(name):
     if name in SYNTH:
+        from oracle import lattices
         sp = SYNTH[name]
         rng = np.random.default_rng(sp["seed"])
-        n = sp["n"]
+        g = lattices.make(sp["lattice"], sp["a"], n=sp["n"], PBC=sp.get("kw", {}).get("PBC", False))
+        shape = (g.ny, g.nx)
         kT = kB * sp["T"]
-        return dict(pattern=np.where(rng.random((n, n)) < 0.5, -1.0, 1.0),
-                    E_B=20 * kT * rng.normal(1, 0.05, size=(n, n)))
+        return dict(pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0),
+                    E_B=20 * kT * rng.normal(1, 0.05, size=shape))
     return dict(np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False))
 
 
@@ -94,6 +107,8 @@ def oracle_magnets(name, fx=None):
     for kind, kw in sp["energies"]:
         if kind == "dipolar":
             ens.append(model.Dipolar(**kw))
+        elif kind == "dimonopolar":
+            ens.append(model.DiMonopolar(**kw))
         elif kind == "zeeman":
             ens.append(model.Zeeman(**kw))
         else:
@@ -115,6 +130,8 @@ def product_magnets(name, fx=None, **extra):
     for kind, kw in sp["energies"]:
         if kind == "dipolar":
             ens.append(hs.DipolarEnergy(**kw))
+        elif kind == "dimonopolar":
+            ens.append(hs.energies.DiMonopolarEnergy(**kw))
         elif kind == "zeeman":
             ens.append(hs.ZeemanEnergy(**kw))
         else:

@@ -136,3 +136,34 @@ def test_zero_temperature_returns_none():
     mm = hs.ASI.OOP_Square(1e-6, 8, T=0, seed=1, params=hs.SimParams(UPDATE_SCHEME='Néel'))
     with pytest.warns(UserWarning):
         assert mm.update() is None
+
+
+def test_pair_energy_swap_and_charges():
+    """DiMonopolarEnergy replaces DipolarEnergy only after the latter is removed; the device pair
+    weights follow (charges for the dumbbells, moments again afterwards), and the charges stay
+    those taken at initialisation when the moment changes later (energies.py:413-415)."""
+    import hotspice_amd as hs
+    from oracle import model
+    rng = np.random.default_rng(2)
+    mm = hs.ASI.IP_Square(1e-6, 12, T=300, moment=2e-16, seed=4, energies=[hs.DipolarEnergy()])
+    pat = mm.m.copy()
+    with pytest.raises(NotImplementedError):
+        mm.add_energy(hs.energies.DiMonopolarEnergy(d=300e-9))
+    mm.remove_energy('dipolar')
+    dm = mm.add_energy(hs.energies.DiMonopolarEnergy(d=300e-9, small_d=100e-9))
+    assert mm.has_energy('dimonopolar') and not mm.has_energy('dipolar')
+    om = model.make_magnets("IP_Square", 1e-6, n=12, T=300, moment=2e-16, pattern=pat,
+                            energies=[model.DiMonopolar(d=300e-9, small_d=100e-9)])
+    om.update_energy()
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-6 * scale)
+    # a later moment change leaves the charges (and the dumbbell energies) as they were
+    mm.moment = 2e-16 * (1 + rng.random(mm.m.shape))
+    mm.update_energy()
+    np.testing.assert_allclose(dm.E, om.E, rtol=0, atol=1e-6 * scale)
+    # and back to point dipoles with the new moments
+    mm.remove_energy('dimonopolar')
+    mm.add_energy(hs.DipolarEnergy())
+    od = model.make_magnets("IP_Square", 1e-6, n=12, T=300, moment=mm.moment, pattern=pat, energies=[model.Dipolar()])
+    od.update_energy()
+    np.testing.assert_allclose(mm.E, od.E, rtol=0, atol=1e-9 * np.abs(od.E).max())

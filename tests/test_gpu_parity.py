@@ -144,3 +144,60 @@ def test_dipolar_kernels_equal_oracle(lat, n, kw, cutoff):
         np.testing.assert_allclose(d.kernel_perpself_unitcell, Kps, rtol=0, atol=1e-13 * scale)
         np.testing.assert_allclose(d.kernel_perpother_unitcell, Kpo, rtol=0, atol=1e-13 * scale)
     np.testing.assert_array_equal(d.kernel_unitcell_indices, idx)
+
+
+@pytest.mark.parametrize("lat,n,kw", [("IP_Kagome", 24, {}), ("IP_Pinwheel", 16, dict(PBC=True)),
+                                      ("IP_Square", 14, {}), ("IP_Square", 12, dict(PBC=True))])
+def test_dimonopolar_kernels_equal_oracle(lat, n, kw):
+    """hs_build_dimonopolar_kernels against the oracle's restatement of energies.py:416-641. The
+    plain kernel rounds squared distances to FP32 (as the reference does), so its entries may
+    differ by the last FP32 bit of a 1/r term: the tolerance is 2^-22 of the largest 1/r term."""
+    import hotspice_amd as hs
+    from oracle import model
+    cls = getattr(hs.ASI, lat)
+    probe = cls(1e-6, n, T=300, moment=2e-16, energies=[], seed=1, **kw)
+    d_uc = 300e-9 * (1 + 0.05 * np.arange(probe.unitcell.y * probe.unitcell.x).reshape(probe.unitcell.y, probe.unitcell.x))
+    dm = hs.energies.DiMonopolarEnergy(d=d_uc, small_d=120e-9)
+    mm = cls(1e-6, n, T=300, moment=2e-16, energies=[dm], seed=1, **kw)
+    om = model.make_magnets(lat, 1e-6, n=n, T=300, moment=2e-16, pattern=mm.m, energies=[], PBC=kw.get("PBC", False))
+    K, Kps, Kpo, idx = model.build_monopole_kernels(om, d_uc, 120e-9)
+    np.testing.assert_array_equal(dm.kernel_unitcell_indices, idx)
+    ys, xs = np.nonzero(om.occupation)
+    ang = om.geom.angles[ys, xs]
+    h = np.tile(d_uc, (math.ceil(om.ny / om.uy), math.ceil(om.nx / om.ux)))[ys, xs] / 2
+    qx = np.concatenate([om.x[xs] + h * np.cos(ang), om.x[xs] - h * np.cos(ang)])
+    qy = np.concatenate([om.y[ys] + h * np.sin(ang), om.y[ys] - h * np.sin(ang)])
+    owner = np.concatenate([np.arange(len(ys))] * 2)
+    dist = np.hypot(qx[:, None] - qx[None, :], qy[:, None] - qy[None, :])
+    rmin = dist[owner[:, None] != owner[None, :]].min()  # closest charges of two magnets
+    tol = 4 * 1e-7 * 2.0 ** -22 / rmin
+    np.testing.assert_allclose(dm.kernel_unitcell, K, rtol=0, atol=tol)
+    scale = np.abs(Kps).max()
+    np.testing.assert_allclose(dm.kernel_perpself_unitcell, Kps, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(dm.kernel_perpother_unitcell, Kpo, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("lat,n,kw", [("IP_Kagome", 24, {}), ("IP_Square", 12, dict(PBC=True))])
+def test_dimonopolar_fields_equal_oracle(lat, n, kw):
+    """FP64 recompute with DiMonopolarEnergy: non-uniform moments (charge q = moment/d of the
+    receiving magnet, energies.py:654-659), Zeeman from the moments, E and E_perp."""
+    import hotspice_amd as hs
+    from oracle import model
+    cls = getattr(hs.ASI, lat)
+    probe = cls(1e-6, n, T=300, moment=2e-16, energies=[], seed=1, **kw)
+    rng = np.random.default_rng(3)
+    moment = 2e-16 * (1 + 0.3 * rng.random(probe.m.shape))
+    pat = np.where(rng.random(probe.m.shape) < .5, -1., 1.)
+    mm = cls(1e-6, n, T=300, moment=moment, pattern=pat, seed=1, m_perp_factor=0.7,
+             energies=[hs.energies.DiMonopolarEnergy(prefactor=0.8, d=280e-9, small_d=140e-9),
+                       hs.ZeemanEnergy(2e-3, 0.6)], **kw)
+    om = model.make_magnets(lat, 1e-6, n=n, T=300, moment=moment, pattern=mm.m, m_perp_factor=0.7,
+                            energies=[model.DiMonopolar(prefactor=0.8, d=280e-9, small_d=140e-9), model.Zeeman(2e-3, 0.6)],
+                            PBC=kw.get("PBC", False))
+    om.update_energy()
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-6 * scale)
+    np.testing.assert_allclose(mm.E_perp, om.E_perp, rtol=0, atol=1e-9 * scale)
+    assert mm.E_tot == pytest.approx(om.E_tot, rel=1e-6, abs=1e-6 * scale)
+    d = mm.get_energy('dimonopolar')
+    np.testing.assert_allclose(d.E, om.get_energy('dimonopolar').E, rtol=0, atol=1e-6 * scale)
