@@ -49,6 +49,7 @@ class DeviceState:
         self.zee_by = None
         self.kern = None  # dict of kernel tensors + window
         self.metro_ws = None
+        self.relax_ws = None
         self.rec_scratch = None
         self.team_ws = None
         self.sel_cache = {}
@@ -263,6 +264,19 @@ class DeviceState:
                                               self.ptr(self.metro_ws), self.ptr(log_cells), self.ptr(log_count),
                                               self.stream()))
         self._event_end(ev, [self.state.data_ptr()])
+
+    def relax_sweep(self, order, step_x, step_y, with_barrier, cutoff, reduced_k):
+        """hs_relax_sweep: one Magnets._minimize_all over the subsets in `order`."""
+        need = int(self.lib.hs_relax_ws_bytes(C.byref(self.inst), int(step_x), int(step_y)))
+        if need < 0:
+            raise _lib.HotspiceError("bad relax subsets")
+        if self.relax_ws is None or self.relax_ws.numel() < need:
+            self.relax_ws = self.torch.zeros(need, dtype=self.torch.uint8, device=self.device)
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        _lib.check(self.lib.hs_relax_sweep(C.byref(self.inst), o.ctypes.data_as(C.c_void_p), int(o.size), int(step_x),
+                                           int(step_y), int(bool(with_barrier)), int(cutoff), int(reduced_k),
+                                           self.ptr(self.state), self.ptr(self.relax_ws), self.relax_ws.numel(),
+                                           self.stream()))
 
     def philox_uniform53(self, seed, step, tag, index0, n, open_zero=False):
         out = self.torch.empty(max(1, n), dtype=self.torch.float64, device=self.device)

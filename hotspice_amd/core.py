@@ -1020,6 +1020,55 @@ class Magnets(ABC):
             yield
         return self.t - t_0, self.MCsteps - MCsteps_0
 
+    # ---- zero-temperature relaxation ------------------------------------------------------------
+    def _minimize_all(self, ignore_barrier=True, _simultaneous=False):
+        """core.py:785-806 on the device (hs_relax_sweep): the subsets of cells two unit cells apart,
+        in a random order (numpy.random.permutation, as the reference draws it), each switch every
+        magnet of theirs that gains energy by switching (with ignore_barrier=False: that switches
+        over its barrier, switch_energy/2 + E_B < 0), all at once, and update the fields."""
+        if _simultaneous:
+            warnings.warn("_simultaneous=True in _minimize_all(). I hope you know what you are doing, as this is very nonphysical ;)", stacklevel=2)
+            step_x = step_y = 1
+        else:
+            step_x, step_y = self.unitcell.x * 2, self.unitcell.y * 2
+        order = np.random.permutation(step_x * step_y)
+        # a subset whose first cell lies outside the grid ends the sweep with the reference's
+        # IndexError (core.py:797), after the subsets before it
+        bad = [k for k, i in enumerate(order) if i // step_x >= self.ny or i % step_x >= self.nx]
+        run = order[:bad[0]] if bad else order
+        self._pre_step()
+        p = self.params
+        self._dev.relax_sweep(run, step_x, step_y, not ignore_barrier, p.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF,
+                              p.REDUCED_KERNEL_SIZE)
+        self._after_device_step()
+        if bad:
+            i = int(order[bad[0]])
+            raise IndexError(f"subset origin {divmod(i, step_x)} is outside the {self.ny}x{self.nx} grid")
+        # the periodic FP64 recompute, unless truncated update_multiple windows are part of the result
+        k, kern = p.REDUCED_KERNEL_SIZE, self._dev.kern
+        if k == 0 or kern is None or (min(k, self.nx - 1) >= kern["kwx"] and min(k, self.ny - 1) >= kern["kwy"]):
+            self._since_recompute += len(order)
+            if self.recompute_every and self._since_recompute >= self.recompute_every:
+                self._device_recompute()
+
+    def relax(self, verbose=False, **kwargs):
+        """core.py:808-845: _minimize_all until a sweep switches nothing, the state repeats with
+        period 2, or sqrt(nx^2 + ny^2) sweeps (enough for a signal to cross the grid) are done."""
+        n, n_max = 0, int(math.sqrt(self.nx ** 2 + self.ny ** 2))
+        previous_states = [self.m.copy(), self._zeros, self._zeros]
+        switches = self.switches
+        while not np.allclose(previous_states[2], previous_states[0]) and n < n_max:
+            n += 1
+            self._minimize_all(**kwargs)
+            if switches == self.switches:
+                break
+            switches = self.switches
+            previous_states[2] = previous_states[1].copy()
+            previous_states[1] = previous_states[0].copy()
+            previous_states[0] = self.m.copy()
+        if verbose:
+            print(f"Used {n} (out of {n_max}) steps in relax().")
+
     # ---- observables (summed on the device: hs_binned_sums over one bin) ---------------------------
     def _m_sums(self):
         """(sum of m*ox, sum of m*oy) for in-plane lattices, (sum of m,) otherwise, over the grid."""

@@ -997,6 +997,71 @@ __global__ void k_state_begin_many(const MetroJob *__restrict__ jobs, int njobs)
   jobs[j].st[0] = s;
 }
 
+// ---- zero-temperature sweeps (Magnets._minimize_all, core.py:785-806) --------------------------
+// One subset = the cells (y0 + i*sy, x0 + j*sx). Every magnet of it that gains energy by switching
+// (switch_energy < 0, or switch_energy/2 + E_B < 0 with the barrier) switches, all together, and
+// the fields follow as update_multiple. The selection lists them in the reference's index order
+// (numpy.meshgrid(ys, xs) flattened: x outer, y inner); the update is metro_update over every tile.
+__global__ void __launch_bounds__(MS_THREADS) k_relax_select(MetroJob J, int y0, int x0, int sy, int sx,
+                                                             int with_barrier) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const MetroWS &ws = J.ws;
+  __shared__ int s_i[MS_THREADS / 64 + 1];
+  __shared__ int s_site[HS_MAX_UC];
+  const int tid = threadIdx.x, nx = I.nx, ny = I.ny;
+  if (tid < HS_MAX_UC) s_site[tid] = 0;
+  __syncthreads();
+  // core.py:797: a subset whose first cell is empty is skipped
+  const bool live = (I.flags[y0 * nx + x0] & HS_FLAG_OCC) != 0;
+  const int ky = live ? (ny - y0 + sy - 1) / sy : 0, kx = live ? (nx - x0 + sx - 1) / sx : 0;
+  const int N = kx * ky;
+  int base = 0;
+  for (int b = 0; b < N; b += MS_THREADS) {
+    const int j = b + tid;
+    int c = -1, acc = 0;
+    float mmu = 0.f;
+    if (j < N) {
+      c = (y0 + (j % ky) * sy) * nx + (x0 + (j / ky) * sx);
+      const int mc = (int)I.m[c];
+      if (mc != 0) {
+        const double e = (double)I.E[c] - (double)I.Ec[c];  // E of the magnet; switch_energy = -2E
+        acc = with_barrier ? ((double)I.EB[c] - e < 0.0) : (-2.0 * e < 0.0);
+        mmu = (float)(-mc) * (I.pair_src_unit ? 1.f : I.mu[c]);  // (m w) after the flip
+      }
+    }
+    int total = 0;
+    const int pos = base + block_exclusive_scan(acc, s_i, total);
+    if (acc) {
+      ws.acc_cell[pos] = c;
+      ws.acc_mmu[pos] = mmu;
+      atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
+    }
+    base += total;
+  }
+  __syncthreads();
+  if (tid < I.ux * I.uy) ws.trunc[tid] = (base > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
+  if (tid == 0) {
+    ws.info[0] = 0;
+    ws.info[1] = 0;
+    ws.info[2] = 1;  // not a supercell grid: the update filters the whole list
+    *ws.acc_n = base;
+    hs_state_t S = J.st[0];
+    S.switches += base;  // core.py:803-804
+    S.attempts += base;
+    J.st[0] = S;
+  }
+}
+
+// The update of a sweep's switches over every tile of the grid.
+__global__ void __launch_bounds__(MU_THREADS) k_relax_update(MetroJob J) {
+  const int nt = J.tiles_x * J.tiles_y;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+    metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
+    __syncthreads();
+  }
+}
+
 int check_select(const hs_select_t *sel) {
   if (sel->scheme == 0) {
     if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
@@ -1031,6 +1096,55 @@ MetroJob make_job(const hs_instance_t *inst, const hs_select_t *sel, const hs_ru
 }
 
 }  // namespace
+
+extern "C" int64_t hs_relax_ws_bytes(const hs_instance_t *inst, int32_t step_x, int32_t step_y) {
+  if (!inst || inst->nx < 1 || inst->ny < 1 || step_x < 1 || step_y < 1) return -1;
+  const int64_t n = (int64_t)((inst->nx + step_x - 1) / step_x) * ((inst->ny + step_y - 1) / step_y);
+  return (int64_t)ws_bytes((int)n);
+}
+
+extern "C" int hs_relax_sweep(const hs_instance_t *inst, const int32_t *order, int32_t n_order, int32_t step_x,
+                              int32_t step_y, int32_t with_barrier, int32_t cutoff, int32_t reduced_k,
+                              hs_state_t *state, void *ws, int64_t ws_bytes_, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!state || !ws || (n_order > 0 && !order) || n_order < 0) return hs_set_error(HS_ERR_ARG, "missing order/state/ws");
+  if (step_x < 1 || step_y < 1) return hs_set_error(HS_ERR_ARG, "bad subset steps %d,%d", step_x, step_y);
+  if (cutoff < 0 || reduced_k < 0) return hs_set_error(HS_ERR_ARG, "negative cutoff / reduced kernel size");
+  const int64_t need = hs_relax_ws_bytes(inst, step_x, step_y);
+  if (ws_bytes_ < need) return hs_set_error(HS_ERR_ARG, "relax workspace of %lld bytes < %lld", (long long)ws_bytes_,
+                                            (long long)need);
+  for (int i = 0; i < n_order; ++i)
+    if (order[i] < 0 || order[i] >= step_x * step_y) return hs_set_error(HS_ERR_ARG, "order[%d] out of range", i);
+  const int ncells = (int)((inst->nx + step_x - 1) / step_x) * ((inst->ny + step_y - 1) / step_y);
+  MetroJob J;
+  J.I = *inst;
+  J.SEL = hs_select_t{};
+  J.SEL.scheme = 1;
+  J.SEL.nsc = ncells;
+  J.SEL.cutoff = cutoff;
+  J.SEL.reduced_k = reduced_k;
+  J.R = hs_run_t{};
+  J.st = state;
+  J.ws = carve(ws, ncells);
+  J.log_cells = nullptr;
+  J.log_count = nullptr;
+  J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
+  J.tiles_y = (inst->ny + MU_TY - 1) / MU_TY;
+  const int nt = J.tiles_x * J.tiles_y;
+  const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < n_order; ++i) {
+    const int y0 = order[i] / step_x, x0 = order[i] % step_x;
+    if (y0 >= inst->ny || x0 >= inst->nx) continue;  // no cell of the grid in this subset
+    k_relax_select<<<1, MS_THREADS, 0, s>>>(J, y0, x0, step_y, step_x, with_barrier ? 1 : 0);
+    HS_LAUNCH_CHECK("k_relax_select");
+    k_relax_update<<<wgs, MU_THREADS, 0, s>>>(J);
+    HS_LAUNCH_CHECK("k_relax_update");
+  }
+  return HS_OK;
+}
 
 extern "C" int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel) {
   if (!sel || !inst || inst->nx < 1 || inst->ny < 1) return -1;

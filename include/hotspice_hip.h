@@ -321,6 +321,21 @@ int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const h
 int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel);
 
 /*
+ * One zero-temperature sweep: Magnets._minimize_all (core.py:785-806), the step of
+ * Magnets.relax (core.py:808-845). For each subset index i of order[0..n_order) (host array, the
+ * reference's random permutation), the subset is the cells (y0 + k*step_y, x0 + l*step_x) with
+ * (y0, x0) = divmod(i, step_x), skipped when cell (y0, x0) is empty. Every magnet of the subset
+ * with switch_energy < 0 (with_barrier: switch_energy/2 + E_B < 0) switches, all at once, and
+ * the fields are updated as update_multiple (energies.py:331-366, cutoff / reduced_k as
+ * SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF / REDUCED_KERNEL_SIZE). state[0].switches and
+ * .attempts count the switches. ws: device scratch of hs_relax_ws_bytes bytes. Asynchronous.
+ */
+int hs_relax_sweep(const hs_instance_t *inst, const int32_t *order, int32_t n_order, int32_t step_x,
+                   int32_t step_y, int32_t with_barrier, int32_t cutoff, int32_t reduced_k, hs_state_t *state,
+                   void *ws, int64_t ws_bytes, void *stream);
+int64_t hs_relax_ws_bytes(const hs_instance_t *inst, int32_t step_x, int32_t step_y);
+
+/*
  * Neel-Arrhenius steps for many instances in ONE persistent cooperative launch (same
  * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760, and
  * bit-identical results). Each team keeps its magnets' fields in registers and draws the same

@@ -27,6 +27,7 @@ Restated reference code (file:line in bvwaeyen/Hotspice):
   select / _select_grid ...... hotspice/core.py:456-558
   calc_r ..................... hotspice/core.py:720-733
   progress ................... hotspice/core.py:736-760, lower_than utils.py:90-92
+  _minimize_all / relax ...... hotspice/core.py:785-845
 """
 import math
 
@@ -795,6 +796,47 @@ class OracleMagnets:
             else:
                 self.update(**kwargs)
         return self.t - t0, self.MCsteps - mc0
+
+    # ---- zero-temperature relaxation ----------------------------------------------------
+    def _minimize_all(self, ignore_barrier=True, _simultaneous=False, order=None):
+        """core.py:785-806. `order`: the subset permutation (default: numpy.random.permutation,
+        as the reference)."""
+        if _simultaneous:
+            step_x = step_y = 1
+        else:
+            step_x, step_y = self.ux * 2, self.uy * 2
+        if order is None:
+            order = np.random.permutation(step_x * step_y)
+        for i in order:
+            y, x = divmod(int(i), step_x)
+            if not self.occupation[y, x]:
+                continue
+            sub = np.asarray(np.meshgrid(np.arange(y, self.ny, step_y), np.arange(x, self.nx, step_x))).reshape(2, -1)
+            if ignore_barrier:
+                idx = sub[:, np.where(self.switch_energy(sub) < 0)[0]]
+            else:
+                idx = sub[:, self.switch_energy(sub) / 2 + self.E_B[sub[0], sub[1]] < 0]
+            if idx.size > 0:
+                self.m[idx[0], idx[1]] = -self.m[idx[0], idx[1]]
+                self.switches += idx.shape[1]
+                self.attempted_switches += idx.shape[1]
+                self.update_energy(index=idx)
+
+    def relax(self, **kwargs):
+        """core.py:808-845 (without the plotting)."""
+        n, n_max = 0, int(math.sqrt(self.nx ** 2 + self.ny ** 2))
+        prev = [self.m.copy(), self._zeros, self._zeros]
+        switches = self.switches
+        while not np.allclose(prev[2], prev[0]) and n < n_max:
+            n += 1
+            self._minimize_all(**kwargs)
+            if switches == self.switches:
+                break
+            switches = self.switches
+            prev[2] = prev[1].copy()
+            prev[1] = prev[0].copy()
+            prev[0] = self.m.copy()
+        return n
 
 
 def make_magnets(lattice, a, n=None, nx=None, ny=None, angle=0.0, PBC=False, **kw):

@@ -167,3 +167,67 @@ def test_pair_energy_swap_and_charges():
     od = model.make_magnets("IP_Square", 1e-6, n=12, T=300, moment=mm.moment, pattern=pat, energies=[model.Dipolar()])
     od.update_energy()
     np.testing.assert_allclose(mm.E, od.E, rtol=0, atol=1e-9 * np.abs(od.E).max())
+
+
+RELAX_CASES = {
+    "ipsq_dip_zee": dict(lat="IP_Square", n=16, kw={}, ens=[("dipolar", {}), ("zeeman", (3e-4, 0.4))], params={},
+                         call={}),
+    "kagome_mono_barrier": dict(lat="IP_Kagome", n=32, kw={}, ens=[("dimonopolar", dict(d=350e-9)), ("zeeman", (2e-3, 1.2))],
+                                params={}, call=dict(ignore_barrier=False)),
+    "oop_conv_window": dict(lat="OOP_Square", n=40, kw={}, ens=[("dipolar", {}), ("exchange", 0.5)],
+                            params=dict(REDUCED_KERNEL_SIZE=5, SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=3), call={}),
+    "ipsq_pbc_window": dict(lat="IP_Square", n=24, kw=dict(PBC=True), ens=[("dipolar", {}), ("zeeman", (5e-4, 0.2))],
+                            params=dict(REDUCED_KERNEL_SIZE=4, SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=2), call={}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RELAX_CASES))
+def test_relax_matches_oracle(name):
+    """Magnets.relax (core.py:808-845) on the device against the oracle, with the same subset
+    orders (numpy.random seeded alike): spins bit-exact, fields at 1e-5 of their scale, counters."""
+    import hotspice_amd as hs
+    from oracle import lattices, model
+    c = RELAX_CASES[name]
+    kT = hs.kB * 300
+    g = lattices.make(c["lat"], 1e-6, n=c["n"], PBC=c["kw"].get("PBC", False))
+    rng = np.random.default_rng(9)
+    pat = np.where(rng.random((g.ny, g.nx)) < .5, -1., 1.)
+    EB = 2 * kT * rng.normal(1, 0.05, size=(g.ny, g.nx))
+    pe, oe = [], []
+    for kind, arg in c["ens"]:
+        if kind == "dipolar":
+            pe.append(hs.DipolarEnergy())
+            oe.append(model.Dipolar())
+        elif kind == "dimonopolar":
+            pe.append(hs.energies.DiMonopolarEnergy(**arg))
+            oe.append(model.DiMonopolar(**arg))
+        elif kind == "zeeman":
+            pe.append(hs.ZeemanEnergy(*arg))
+            oe.append(model.Zeeman(*arg))
+        else:
+            pe.append(hs.ExchangeEnergy(J=arg * kT))
+            oe.append(model.Exchange(J=arg * kT))
+    mm = getattr(hs.ASI, c["lat"])(1e-6, c["n"], T=300, E_B=EB, moment=2.37e-16, pattern=pat, energies=pe, seed=1,
+                                   params=hs.SimParams(**c["params"]), **c["kw"])
+    om = model.make_magnets(c["lat"], 1e-6, n=c["n"], T=300, E_B=EB, moment=2.37e-16, pattern=pat, energies=oe,
+                            params=dict(c["params"]), PBC=c["kw"].get("PBC", False))
+    np.random.seed(21)
+    mm.relax(**c["call"])
+    np.random.seed(21)
+    om.relax(**c["call"])
+    assert mm.switches == om.switches > 0
+    assert mm.attempted_switches == om.attempted_switches
+    np.testing.assert_array_equal(mm.m, om.m)
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * scale)
+
+
+def test_minimize_all_index_error_like_the_reference():
+    """A subset origin outside a small grid raises IndexError after the subsets before it."""
+    import hotspice_amd as hs
+    mm = hs.ASI.IP_Kagome(1e-6, 16, T=300, moment=2.37e-16, seed=1, energies=[hs.DipolarEnergy()])
+    assert mm.ny < 2 * mm.unitcell.y
+    np.random.seed(0)
+    with pytest.raises(IndexError):
+        for _ in range(20):  # some permutation reaches an origin row >= ny
+            mm._minimize_all()
