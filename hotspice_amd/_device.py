@@ -50,6 +50,9 @@ class DeviceState:
         self.kern = None  # dict of kernel tensors + window
         self.metro_ws = None
         self.relax_ws = None
+        self.flip_ws = None
+        self.min_cells = None
+        self.min_result = None
         self.rec_scratch = None
         self.team_ws = None
         self.sel_cache = {}
@@ -277,6 +280,33 @@ class DeviceState:
                                            int(step_y), int(bool(with_barrier)), int(cutoff), int(reduced_k),
                                            self.ptr(self.state), self.ptr(self.relax_ws), self.relax_ws.numel(),
                                            self.stream()))
+
+    def minimize_scan(self, with_barrier):
+        """hs_minimize_scan: (n, argmin cell, smallest barrier); the n cells below -kT are left in
+        self.min_cells[:n] (row-major order)."""
+        t = self.torch
+        if self.min_cells is None:
+            self.min_cells = t.empty(self.nx * self.ny, dtype=t.int32, device=self.device)
+            self.min_result = t.empty(3, dtype=t.int64, device=self.device)
+        _lib.check(self.lib.hs_minimize_scan(C.byref(self.inst), int(bool(with_barrier)), self.ptr(self.min_cells),
+                                             self.ptr(self.min_result), self.stream()))
+        n, amin, bits = (int(v) for v in self.min_result.cpu().numpy())
+        return n, amin, float(np.array([bits], dtype=np.int64).view(np.float64)[0])
+
+    def flip_cells(self, cells, cutoff, reduced_k):
+        """hs_flip_cells: the given distinct flat cell indices switch together."""
+        cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1)
+        if cells.size == 0:
+            return
+        if cells.min() < 0 or cells.max() >= self.nx * self.ny:
+            raise IndexError("cell index outside the grid")
+        need = int(self.lib.hs_flip_ws_bytes(C.byref(self.inst)))
+        if self.flip_ws is None or self.flip_ws.numel() < need:
+            self.flip_ws = self.torch.zeros(need, dtype=self.torch.uint8, device=self.device)
+        t_cells = self.torch.from_numpy(cells).to(self.device)
+        _lib.check(self.lib.hs_flip_cells(C.byref(self.inst), self.ptr(t_cells), int(cells.size), int(cutoff),
+                                          int(reduced_k), self.ptr(self.state), self.ptr(self.flip_ws),
+                                          self.flip_ws.numel(), self.stream()))
 
     def philox_uniform53(self, seed, step, tag, index0, n, open_zero=False):
         out = self.torch.empty(max(1, n), dtype=self.torch.float64, device=self.device)

@@ -22,7 +22,8 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_recompute_fields", "hs_recompute_scratch_bytes", "hs_field_components", "hs_energy_total", "hs_zeeman_swap", "hs_neel_run",
            "hs_metropolis_run", "hs_metropolis_ws_bytes", "hs_neel_run_teams", "hs_neel_teams_ws_bytes",
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
-           "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes")
+           "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
+           "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes")
 
 
 class Instance(C.Structure):
@@ -135,6 +136,9 @@ def load():
         "hs_binned_sums_scratch_bytes": ([C.POINTER(Instance), i32, i32], C.c_int64),
         "hs_relax_sweep": ([C.POINTER(Instance), vp, i32, i32, i32, i32, i32, i32, vp, vp, i64, vp], C.c_int),
         "hs_relax_ws_bytes": ([C.POINTER(Instance), i32, i32], C.c_int64),
+        "hs_minimize_scan": ([C.POINTER(Instance), i32, vp, vp, vp], C.c_int),
+        "hs_flip_cells": ([C.POINTER(Instance), vp, i32, i32, i32, vp, vp, i64, vp], C.c_int),
+        "hs_flip_ws_bytes": ([C.POINTER(Instance)], C.c_int64),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

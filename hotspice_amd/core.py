@@ -1021,6 +1021,37 @@ class Magnets(ABC):
         return self.t - t_0, self.MCsteps - MCsteps_0
 
     # ---- zero-temperature relaxation ------------------------------------------------------------
+    def minimize(self, ignore_barrier=True, verbose=False, _frac=0.3):
+        """core.py:763-783 on the device: while some magnets have a barrier below -kT, switch a
+        random ceil(_frac*n) of those n (numpy.random.choice, as the reference); otherwise switch
+        the magnet with the smallest barrier while that is negative, each magnet at most 3 times.
+        ignore_barrier=False ranks by the smaller E_barrier (the reference compares the pair of
+        barriers there, which its indexing cannot use)."""
+        self._pre_step()
+        p = self.params
+        visited = np.zeros_like(self.m)
+        while True:
+            n, amin, bmin = self._dev.minimize_scan(not ignore_barrier)
+            if n != 0:  # magnets without any barrier: a random part of them
+                cells = self._dev.min_cells[:n].cpu().numpy()
+                rand = np.random.choice(n, size=math.ceil(n * _frac), replace=False)
+                chosen = cells[rand]
+            else:  # the one with the most energy to gain
+                y, x = divmod(amin, self.nx)
+                if bmin >= 0:
+                    break
+                visited[y, x] += 1
+                if visited[y, x] >= 3:
+                    break
+                chosen = np.array([amin])
+            if verbose:
+                print(f"minimize: switching {chosen.size} magnet(s)")
+            self._dev.flip_cells(chosen, p.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF, p.REDUCED_KERNEL_SIZE)
+            self._after_device_step()
+        st = self._state()
+        if st.err:
+            raise _lib.HotspiceError(f"device switch failed (err={st.err})")
+
     def _minimize_all(self, ignore_barrier=True, _simultaneous=False):
         """core.py:785-806 on the device (hs_relax_sweep): the subsets of cells two unit cells apart,
         in a random order (numpy.random.permutation, as the reference draws it), each switch every

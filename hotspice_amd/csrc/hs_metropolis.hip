@@ -1062,6 +1062,98 @@ __global__ void __launch_bounds__(MU_THREADS) k_relax_update(MetroJob J) {
   }
 }
 
+// ---- sequential minimisation (Magnets.minimize, core.py:763-783) -------------------------------
+// The scan of one iteration: the barrier of every cell (switch_energy = -2E, 0 where empty; or
+// the smaller E_barrier of an occupied magnet), the cells with barrier < -kT in row-major order
+// (numpy.where order), and the first cell of the smallest barrier (numpy.argmin).
+__global__ void __launch_bounds__(MS_THREADS) k_min_scan(hs_instance_t I, int with_barrier, int32_t *cells,
+                                                         int64_t *result) {
+  __shared__ int s_i[MS_THREADS / 64 + 1];
+  __shared__ double s_v[MS_THREADS];
+  __shared__ int s_c[MS_THREADS];
+  const int tid = threadIdx.x, ncell = I.nx * I.ny;
+  double best = INFINITY;
+  int best_c = 0x7fffffff, base = 0;
+  for (int b = 0; b < ncell; b += MS_THREADS) {
+    const int c = b + tid;
+    int below = 0;
+    if (c < ncell) {
+      const int mc = (int)I.m[c];
+      double bar;
+      if (with_barrier)
+        bar = mc != 0 ? barrier_min_f64(I, c) : INFINITY;
+      else
+        bar = mc != 0 ? -2.0 * ((double)I.E[c] - (double)I.Ec[c]) : 0.0;
+      below = bar < -1.0 / (double)I.beta[c];  // barrier < -kT
+      if (bar < best) {  // cells come in ascending order per thread: the first minimum stays
+        best = bar;
+        best_c = c;
+      }
+    }
+    int total = 0;
+    const int pos = base + block_exclusive_scan(below, s_i, total);
+    if (below) cells[pos] = c;
+    base += total;
+  }
+  s_v[tid] = best;
+  s_c[tid] = best_c;
+  __syncthreads();
+  for (int w = MS_THREADS / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      const double v2 = s_v[tid + w];
+      const int c2 = s_c[tid + w];
+      if (v2 < s_v[tid] || (v2 == s_v[tid] && c2 < s_c[tid])) {
+        s_v[tid] = v2;
+        s_c[tid] = c2;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    result[0] = base;
+    result[1] = s_c[0] == 0x7fffffff ? 0 : s_c[0];
+    result[2] = (int64_t)__double_as_longlong(s_v[0]);
+  }
+}
+
+// k given cells (distinct, device array) switch together: the selection list of an update whose
+// fields follow as update_single (one cell) or update_multiple (several), counters += k.
+__global__ void __launch_bounds__(MS_THREADS) k_flip_select(MetroJob J, const int32_t *__restrict__ cells, int k) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const MetroWS &ws = J.ws;
+  __shared__ int s_site[HS_MAX_UC];
+  const int tid = threadIdx.x, nx = I.nx, ncell = I.nx * I.ny;
+  if (tid < HS_MAX_UC) s_site[tid] = 0;
+  int bad = 0;
+  for (int i = tid; i < k; i += MS_THREADS) bad |= (cells[i] < 0 || cells[i] >= ncell);
+  if (__syncthreads_or(bad)) {  // an index outside the grid: nothing switches, the state reports it
+    if (tid == 0) {
+      *ws.acc_n = 0;
+      J.st[0].err = 1;
+    }
+    return;
+  }
+  for (int i = tid; i < k; i += MS_THREADS) {
+    const int c = cells[i];
+    ws.acc_cell[i] = c;
+    ws.acc_mmu[i] = (float)(-(int)I.m[c]) * (I.pair_src_unit ? 1.f : I.mu[c]);
+    atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
+  }
+  __syncthreads();
+  if (tid < I.ux * I.uy) ws.trunc[tid] = (k > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
+  if (tid == 0) {
+    ws.info[0] = 0;
+    ws.info[1] = 0;
+    ws.info[2] = 1;
+    *ws.acc_n = k;
+    hs_state_t S = J.st[0];
+    S.switches += k;
+    S.attempts += k;
+    J.st[0] = S;
+  }
+}
+
 int check_select(const hs_select_t *sel) {
   if (sel->scheme == 0) {
     if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
@@ -1143,6 +1235,55 @@ extern "C" int hs_relax_sweep(const hs_instance_t *inst, const int32_t *order, i
     k_relax_update<<<wgs, MU_THREADS, 0, s>>>(J);
     HS_LAUNCH_CHECK("k_relax_update");
   }
+  return HS_OK;
+}
+
+extern "C" int hs_minimize_scan(const hs_instance_t *inst, int32_t with_barrier, int32_t *cells, int64_t *result,
+                                void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!cells || !result) return hs_set_error(HS_ERR_ARG, "missing cells/result");
+  k_min_scan<<<1, MS_THREADS, 0, (hipStream_t)stream>>>(*inst, with_barrier ? 1 : 0, cells, result);
+  HS_LAUNCH_CHECK("k_min_scan");
+  return HS_OK;
+}
+
+extern "C" int64_t hs_flip_ws_bytes(const hs_instance_t *inst) {
+  if (!inst || inst->nx < 1 || inst->ny < 1) return -1;
+  return (int64_t)ws_bytes(inst->nx * inst->ny);
+}
+
+extern "C" int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, int32_t k, int32_t cutoff,
+                             int32_t reduced_k, hs_state_t *state, void *ws, int64_t ws_bytes_, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  const int ncell = inst->nx * inst->ny;
+  if (!state || !ws || k < 0 || k > ncell || (k > 0 && !cells)) return hs_set_error(HS_ERR_ARG, "bad flip arguments");
+  if (cutoff < 0 || reduced_k < 0) return hs_set_error(HS_ERR_ARG, "negative cutoff / reduced kernel size");
+  if (ws_bytes_ < hs_flip_ws_bytes(inst)) return hs_set_error(HS_ERR_ARG, "flip workspace too small");
+  if (k == 0) return HS_OK;
+  MetroJob J;
+  J.I = *inst;
+  J.SEL = hs_select_t{};
+  J.SEL.scheme = 1;
+  J.SEL.nsc = ncell;
+  J.SEL.cutoff = cutoff;
+  J.SEL.reduced_k = reduced_k;
+  J.R = hs_run_t{};
+  J.st = state;
+  J.ws = carve(ws, ncell);
+  J.log_cells = nullptr;
+  J.log_count = nullptr;
+  J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
+  J.tiles_y = (inst->ny + MU_TY - 1) / MU_TY;
+  const int nt = J.tiles_x * J.tiles_y;
+  hipStream_t s = (hipStream_t)stream;
+  k_flip_select<<<1, MS_THREADS, 0, s>>>(J, cells, k);
+  HS_LAUNCH_CHECK("k_flip_select");
+  k_relax_update<<<(unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS), MU_THREADS, 0, s>>>(J);
+  HS_LAUNCH_CHECK("k_relax_update");
   return HS_OK;
 }
 

@@ -336,6 +336,27 @@ int hs_relax_sweep(const hs_instance_t *inst, const int32_t *order, int32_t n_or
 int64_t hs_relax_ws_bytes(const hs_instance_t *inst, int32_t step_x, int32_t step_y);
 
 /*
+ * The scan of one Magnets.minimize iteration (core.py:763-783): the barrier of every cell
+ * (switch_energy = -2E, 0 where empty; with_barrier: the smaller E_barrier of each occupied
+ * magnet, core.py:625-656), then cells[0..n) = the cells with barrier < -kT in row-major order
+ * (numpy.where order) and result = {n, first cell of the smallest barrier (numpy.argmin), that
+ * barrier as the bits of an FP64}. cells: device int32[nx*ny]; result: device int64[3].
+ * Asynchronous.
+ */
+int hs_minimize_scan(const hs_instance_t *inst, int32_t with_barrier, int32_t *cells, int64_t *result, void *stream);
+
+/*
+ * Switch the k distinct cells cells[0..k) (device array) together and update the fields as
+ * Magnets.update_energy(index) does after such a switch (core.py:241-253): update_single for
+ * one cell, update_multiple (cutoff / reduced_k as in hs_relax_sweep) for several.
+ * state[0].switches and .attempts grow by k. A cell index outside the grid switches nothing and
+ * sets state[0].err. ws: device scratch of hs_flip_ws_bytes bytes. Asynchronous.
+ */
+int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, int32_t k, int32_t cutoff, int32_t reduced_k,
+                  hs_state_t *state, void *ws, int64_t ws_bytes, void *stream);
+int64_t hs_flip_ws_bytes(const hs_instance_t *inst);
+
+/*
  * Neel-Arrhenius steps for many instances in ONE persistent cooperative launch (same
  * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760, and
  * bit-identical results). Each team keeps its magnets' fields in registers and draws the same

@@ -27,7 +27,7 @@ Restated reference code (file:line in bvwaeyen/Hotspice):
   select / _select_grid ...... hotspice/core.py:456-558
   calc_r ..................... hotspice/core.py:720-733
   progress ................... hotspice/core.py:736-760, lower_than utils.py:90-92
-  _minimize_all / relax ...... hotspice/core.py:785-845
+  minimize / _minimize_all / relax  hotspice/core.py:763-845
 """
 import math
 
@@ -798,6 +798,29 @@ class OracleMagnets:
         return self.t - t0, self.MCsteps - mc0
 
     # ---- zero-temperature relaxation ----------------------------------------------------
+    def minimize(self, ignore_barrier=True, _frac=0.3):
+        """core.py:763-783 (ignore_barrier=False ranks by the smaller E_barrier)."""
+        visited = np.zeros_like(self.m)
+        while True:
+            barrier = self.switch_energy() if ignore_barrier else np.where(self.m != 0, self.E_barrier(min_only=True),
+                                                                           np.inf)
+            nobarrier = np.where(barrier < -self.kBT)
+            if (n := nobarrier[0].size) != 0:
+                rand = np.random.choice(n, size=math.ceil(n * _frac), replace=False)
+                idx = (nobarrier[0][rand], nobarrier[1][rand])
+            else:
+                idx = divmod(int(np.argmin(barrier)), self.nx)
+                if barrier[idx] >= 0:
+                    break
+                visited[idx] += 1
+                if visited[idx] >= 3:
+                    break
+            idx = np.asarray(idx).reshape(2, -1)
+            self.m[idx[0], idx[1]] *= -1
+            self.switches += idx.shape[1]
+            self.attempted_switches += idx.shape[1]
+            self.update_energy(index=idx)
+
     def _minimize_all(self, ignore_barrier=True, _simultaneous=False, order=None):
         """core.py:785-806. `order`: the subset permutation (default: numpy.random.permutation,
         as the reference)."""

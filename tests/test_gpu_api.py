@@ -231,3 +231,69 @@ def test_minimize_all_index_error_like_the_reference():
     with pytest.raises(IndexError):
         for _ in range(20):  # some permutation reaches an origin row >= ny
             mm._minimize_all()
+
+
+@pytest.mark.parametrize("name", ["ipsq_dip_zee", "kagome_mono_barrier", "oop_conv_window"])
+def test_minimize_matches_oracle(name):
+    """Magnets.minimize (core.py:763-783) on the device against the oracle with the same
+    numpy.random draws: spins bit-exact, counters, fields at 1e-5 of their scale."""
+    import hotspice_amd as hs
+    from oracle import lattices, model
+    c = RELAX_CASES[name]
+    kT = hs.kB * 300
+    g = lattices.make(c["lat"], 1e-6, n=c["n"], PBC=c["kw"].get("PBC", False))
+    rng = np.random.default_rng(19)
+    pat = np.where(rng.random((g.ny, g.nx)) < .5, -1., 1.)
+    EB = 2 * kT * rng.normal(1, 0.05, size=(g.ny, g.nx))
+    pe, oe = [], []
+    for kind, arg in c["ens"]:
+        if kind == "dipolar":
+            pe.append(hs.DipolarEnergy())
+            oe.append(model.Dipolar())
+        elif kind == "dimonopolar":
+            pe.append(hs.energies.DiMonopolarEnergy(**arg))
+            oe.append(model.DiMonopolar(**arg))
+        elif kind == "zeeman":
+            pe.append(hs.ZeemanEnergy(*arg))
+            oe.append(model.Zeeman(*arg))
+        else:
+            pe.append(hs.ExchangeEnergy(J=arg * kT))
+            oe.append(model.Exchange(J=arg * kT))
+    mm = getattr(hs.ASI, c["lat"])(1e-6, c["n"], T=300, E_B=EB, moment=2.37e-16, pattern=pat, energies=pe, seed=1,
+                                   params=hs.SimParams(**c["params"]), **c["kw"])
+    om = model.make_magnets(c["lat"], 1e-6, n=c["n"], T=300, E_B=EB, moment=2.37e-16, pattern=pat, energies=oe,
+                            params=dict(c["params"]), PBC=c["kw"].get("PBC", False))
+    np.random.seed(5)
+    mm.minimize(**c["call"])
+    np.random.seed(5)
+    om.minimize(**c["call"])
+    assert mm.switches == om.switches > 0
+    assert mm.attempted_switches == om.attempted_switches
+    np.testing.assert_array_equal(mm.m, om.m)
+    scale = np.abs(om.E).max()
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * scale)
+
+
+def test_flip_cells_equals_update_energy():
+    """hs_flip_cells (the switch + update_energy(index) of core.py:241-253) against the oracle's
+    update_single / update_multiple for a single cell and a set of cells."""
+    import hotspice_amd as hs
+    from oracle import model
+    kT = hs.kB * 300
+    mm = hs.ASI.IP_Pinwheel(1e-6, 20, T=300, moment=2.37e-16, seed=3,
+                            energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(1e-3, 0.5), hs.ExchangeEnergy(J=0.3 * kT)])
+    om = model.make_magnets("IP_Pinwheel", 1e-6, n=20, T=300, moment=2.37e-16, pattern=mm.m,
+                            energies=[model.Dipolar(), model.Zeeman(1e-3, 0.5), model.Exchange(J=0.3 * kT)])
+    occ = np.argwhere(mm.occupation)
+    rng = np.random.default_rng(2)
+    for k in (1, 7, 40):
+        sel = occ[rng.choice(len(occ), k, replace=False)]
+        cells = sel[:, 0] * mm.nx + sel[:, 1]
+        mm._dev.flip_cells(cells, mm.params.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF, mm.params.REDUCED_KERNEL_SIZE)
+        mm._after_device_step()
+        om.m[sel[:, 0], sel[:, 1]] *= -1
+        om.update_energy(index=sel.T)
+        np.testing.assert_array_equal(mm.m, om.m)
+        np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * np.abs(om.E).max())
+    with pytest.raises(IndexError):
+        mm._dev.flip_cells([mm.nx * mm.ny], 20, 20)

@@ -62,3 +62,22 @@ def test_relax_ends_in_a_local_minimum(lat, n, pbc):
         assert np.all(E <= 0) or mm.switches > again
         if np.all(E <= 0):
             assert mm.switches == again
+
+
+@pytest.mark.parametrize("lat,n,ens", [("OOP_Square", 12, "exchange"), ("IP_Square", 10, "dipolar")])
+def test_minimize_ends_in_a_local_minimum(lat, n, ens):
+    g = lattices.make(lat, 1e-6, n=n)
+    kT = model.kB * 300
+    energies = [model.Exchange(J=kT), model.Zeeman(1e-5, 0.2)] if ens == "exchange" else [model.Dipolar()]
+    mm = model.OracleMagnets(g, T=300, moment=2e-16, pattern=_random(g, 4), energies=energies)
+    np.random.seed(3)
+    tot0 = mm.E_tot
+    mm.minimize()
+    occ = g.occupation.astype(bool)
+    assert mm.switches == mm.attempted_switches > 0
+    if np.all(mm.E[occ] <= 0):  # ended because no magnet gains by switching
+        s = mm.switches
+        mm.minimize()
+        assert mm.switches == s
+    assert mm.E_tot < tot0
+
