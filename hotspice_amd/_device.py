@@ -51,6 +51,7 @@ class DeviceState:
         self.metro_ws = None
         self.relax_ws = None
         self.flip_ws = None
+        self.wolff_ws = None
         self.min_cells = None
         self.min_result = None
         self.rec_scratch = None
@@ -307,6 +308,17 @@ class DeviceState:
         _lib.check(self.lib.hs_flip_cells(C.byref(self.inst), self.ptr(t_cells), int(cells.size), int(cutoff),
                                           int(reduced_k), self.ptr(self.state), self.ptr(self.flip_ws),
                                           self.flip_ws.numel(), self.stream()))
+
+    def wolff_run(self, run: _lib.Run, cutoff, reduced_k, log_cells=None, log_count=None):
+        """hs_wolff_run: Wolff cluster steps until run's stop predicate or step budget."""
+        need = int(self.lib.hs_wolff_ws_bytes(C.byref(self.inst)))
+        if need < 0:
+            raise _lib.HotspiceError("bad instance for Wolff steps")
+        if self.wolff_ws is None or self.wolff_ws.numel() < need:
+            self.wolff_ws = self.torch.zeros(need, dtype=self.torch.uint8, device=self.device)
+        _lib.check(self.lib.hs_wolff_run(C.byref(self.inst), C.byref(run), int(cutoff), int(reduced_k),
+                                         self.ptr(self.state), self.ptr(self.wolff_ws), self.wolff_ws.numel(),
+                                         self.ptr(log_cells), self.ptr(log_count), self.stream()))
 
     def philox_uniform53(self, seed, step, tag, index0, n, open_zero=False):
         out = self.torch.empty(max(1, n), dtype=self.torch.float64, device=self.device)

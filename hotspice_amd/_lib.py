@@ -15,7 +15,7 @@ HS_MAX_NN = 49
 HS_FLAG_OCC = 1
 HS_FLAG_NEEL = 2
 
-TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME = 1, 2, 3, 4, 5
+TAG_NEEL, TAG_SEL, TAG_CHOICE, TAG_ACCEPT, TAG_MTIME, TAG_WOLFF = 1, 2, 3, 4, 5, 7
 
 EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs_philox_uniform53", "hs_build_dipolar_kernels",
            "hs_build_dimonopolar_kernels",
@@ -23,7 +23,7 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_metropolis_run", "hs_metropolis_ws_bytes", "hs_neel_run_teams", "hs_neel_teams_ws_bytes",
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
            "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
-           "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes")
+           "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes")
 
 
 class Instance(C.Structure):
@@ -139,6 +139,8 @@ def load():
         "hs_minimize_scan": ([C.POINTER(Instance), i32, vp, vp, vp], C.c_int),
         "hs_flip_cells": ([C.POINTER(Instance), vp, i32, i32, i32, vp, vp, i64, vp], C.c_int),
         "hs_flip_ws_bytes": ([C.POINTER(Instance)], C.c_int64),
+        "hs_wolff_run": ([C.POINTER(Instance), C.POINTER(Run), i32, i32, vp, vp, i64, vp, vp, vp], C.c_int),
+        "hs_wolff_ws_bytes": ([C.POINTER(Instance)], C.c_int64),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

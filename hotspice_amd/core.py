@@ -790,7 +790,7 @@ class Magnets(ABC):
             case Scheme.METROPOLIS:
                 return self._update_Metropolis(*args, **kwargs)
             case Scheme.WOLFF:
-                raise NotImplementedError("The Wolff cluster update has no device implementation.")
+                return self._update_Wolff(*args, **kwargs)
             case other:
                 raise ValueError(f"The Magnets.params object has an invalid value for UPDATE_SCHEME='{other}'.")
 
@@ -841,13 +841,39 @@ class Magnets(ABC):
         c = cells[:a].cpu().numpy().astype(np.int64)
         return np.asarray([c // self.nx, c % self.nx])
 
+    def _wolff_check(self):
+        """core.py:577-590: clusters need an exchange energy and no DipolarEnergy."""
+        if self.get_energy('dipolar', verbose=False) is not None:
+            raise NotImplementedError("Can not (yet?) select Wolff cluster if long-range interactions are present.")
+        ex = self.get_energy('exchange', verbose=False)
+        if ex is None:
+            raise NotImplementedError("Can not select Wolff cluster if there is no exchange energy in the system.")
+
+    def _update_Wolff(self):
+        """core.py:711-718: grows a Wolff cluster from a random magnet (_select_cluster_exchange,
+        core.py:588-602) and switches it; returns the cluster as a (2, k) array of [y; x]."""
+        self._wolff_check()
+        self._pre_step()
+        t = self._dev.torch
+        cells = t.zeros(self.nx * self.ny, dtype=t.int32, device=self._dev.device)
+        count = t.zeros(1, dtype=t.int32, device=self._dev.device)
+        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=1e10, max_steps=1, log=1, nblocks=0, begin=1)
+        p = self.params
+        self._dev.wolff_run(run, p.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF, p.REDUCED_KERNEL_SIZE, cells, count)
+        self._after_device_step()
+        k = int(count.item())
+        self._since_recompute += 1
+        self._maybe_recompute()
+        c = cells[:k].cpu().numpy().astype(np.int64)
+        return np.asarray([c // self.nx, c % self.nx])
+
     def _maybe_recompute(self):
         if self.recompute_every and self._since_recompute >= self.recompute_every and self._incremental_exact():
             self._device_recompute()
 
     def _incremental_exact(self):
         """True when incremental updates reproduce a full recompute (no kernel truncation)."""
-        if self.params.UPDATE_SCHEME != Scheme.METROPOLIS:
+        if self.params.UPDATE_SCHEME == Scheme.NEEL:
             return True
         k = self.params.REDUCED_KERNEL_SIZE
         kern = self._dev.kern
@@ -934,7 +960,7 @@ class Magnets(ABC):
         self._pre_step()
         scheme = self.params.UPDATE_SCHEME
         if scheme == Scheme.WOLFF:
-            raise NotImplementedError("The Wolff cluster update has no device implementation.")
+            return self._progress_wolff(t_max, MCsteps_max, **kwargs)
         attempt_freq = kwargs.get('attempt_freq', 1e10)
         st0 = self._state()
         t0, a0 = st0.t, st0.attempts
@@ -1000,6 +1026,37 @@ class Magnets(ABC):
                     self._metro_rate = max(1.0, (st.attempts - a_before) / st.steps_done)
                 self._since_recompute += int(st.steps_done)
                 self._maybe_recompute()
+        self._after_device_step()
+        st = self._state()
+        return st.t - t0, st.attempts / self.n - a0 / self.n
+
+    def _progress_wolff(self, t_max, MCsteps_max, **kwargs):
+        """core.py:736-751 with _update_Wolff steps (which do not advance t), chunked on the device."""
+        if kwargs:
+            raise TypeError(f"unexpected Wolff keywords {sorted(kwargs)}")
+        self._wolff_check()
+        st0 = self._state()
+        t0, a0 = st0.t, st0.attempts
+        begin = True
+        p = self.params
+        while True:
+            st = self._state()
+            if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
+                break
+            chunk = 256
+            if self.recompute_every and self._incremental_exact():
+                chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
+            run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=1e10, max_steps=int(chunk),
+                           log=0, nblocks=0, begin=int(begin))
+            if begin:
+                self._set_progress_origin(t0, a0)
+            self._dev.wolff_run(run, p.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF, p.REDUCED_KERNEL_SIZE)
+            begin = False
+            st = self._state()
+            if st.err:
+                raise _lib.HotspiceError(f"device Wolff step failed (err={st.err})")
+            self._since_recompute += int(st.steps_done)
+            self._maybe_recompute()
         self._after_device_step()
         st = self._state()
         return st.t - t0, st.attempts / self.n - a0 / self.n

@@ -1154,6 +1154,121 @@ __global__ void __launch_bounds__(MS_THREADS) k_flip_select(MetroJob J, const in
   }
 }
 
+// ---- Wolff cluster steps (Magnets._update_Wolff / _select_cluster_exchange, core.py:588-602,
+// 711-718) -----------------------------------------------------------------------------------
+struct WolffWS {
+  int8_t *cl;  // [ncell] in the cluster
+  int8_t *ck;  // [ncell] checked (had its chance, or is anti-aligned with the seed)
+  float *fr;   // [ncell] this iteration's frontier value: sum of NN weights of cluster neighbours
+};
+
+// One workgroup grows the cluster of one step and lists it (row-major, numpy.where order) as the
+// step's switches. The seed is _select_single's draw; every iteration, each unchecked cell with
+// cluster neighbours joins with probability 1 - exp(-2 J/kT * frontier) (Swendsen-Wang bonds) on
+// the Philox uniform (cell, step, tag 7 | iteration << 8).
+__global__ void __launch_bounds__(MS_THREADS) k_wolff_select(MetroJob J, WolffWS W) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const hs_run_t &R = J.R;
+  const MetroWS &ws = J.ws;
+  __shared__ int s_i[MS_THREADS / 64 + 1];
+  __shared__ int s_site[HS_MAX_UC];
+  __shared__ float s_nn[HS_MAX_NN];
+  const int tid = threadIdx.x, nx = I.nx, ny = I.ny, ncell = nx * ny;
+  hs_state_t S = J.st[0];
+  const double n = (double)I.n_occ;
+  if (tid < HS_MAX_UC) s_site[tid] = 0;
+  if (tid < I.nn_w * I.nn_h) s_nn[tid] = I.nn_mask[tid];
+  const bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
+                  hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  if (!go) {
+    if (tid == 0) {
+      *ws.acc_n = 0;
+      if (!S.done) {
+        S.done = 1;
+        J.st[0] = S;
+      }
+    }
+    return;
+  }
+  const uint64_t step = (uint64_t)S.step;
+  // _select_single: np.random.randint(self.n) over the initial nonzero list (core.py:489-494)
+  const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 2u, 2ull), 0) * n);
+  const int seed = I.occ_list[min(pick, I.n_occ - 1)];
+  const int mseed = (int)I.m[seed];
+  for (int c = tid; c < ncell; c += MS_THREADS) {
+    W.cl[c] = c == seed;
+    W.ck[c] = c == seed || (int)I.m[c] != mseed;
+  }
+  __syncthreads();
+  const int hy = I.nn_h / 2, hx = I.nn_w / 2;
+  const double J2 = 2.0 * I.J;
+  for (uint32_t it = 0;; ++it) {
+    // frontier = (1 - checked) * convolve2d(cluster, NN, 'same', wrap if PBC else fill)
+    int any = 0;
+    for (int c = tid; c < ncell; c += MS_THREADS) {
+      float f = 0.f;
+      if (!W.ck[c]) {
+        const int y = c / nx, x = c % nx;
+        for (int qy = 0; qy < I.nn_h; ++qy)
+          for (int qx = 0; qx < I.nn_w; ++qx) {
+            const float w = s_nn[qy * I.nn_w + qx];
+            if (w == 0.f) continue;
+            int yy = y + hy - qy, xx = x + hx - qx;
+            if (I.pbc) {
+              yy = hs_pmod(yy, ny);
+              xx = hs_pmod(xx, nx);
+            } else if (yy < 0 || yy >= ny || xx < 0 || xx >= nx) {
+              continue;
+            }
+            if (W.cl[yy * nx + xx]) f += w;
+          }
+      }
+      W.fr[c] = f;
+      any |= f != 0.f;
+    }
+    if (!__syncthreads_or(any)) break;
+    for (int c = tid; c < ncell; c += MS_THREADS) {
+      const float f = W.fr[c];
+      if (f == 0.f) continue;
+      W.ck[c] = 1;
+      const double p = 1.0 - exp(-J2 * (double)I.beta[c] * (double)f);  // bond probability
+      const double u = hs_u53(hs_bits53(I.seed, step, 7u | (it << 8), (uint64_t)c), 0);
+      if (!(u > p)) W.cl[c] = 1;
+    }
+    __syncthreads();
+  }
+  // the cluster in row-major order: the step's switches
+  int base = 0;
+  for (int b = 0; b < ncell; b += MS_THREADS) {
+    const int c = b + tid;
+    const int in = c < ncell ? (int)W.cl[c] : 0;
+    int total = 0;
+    const int pos = base + block_exclusive_scan(in, s_i, total);
+    if (in) {
+      ws.acc_cell[pos] = c;
+      ws.acc_mmu[pos] = (float)(-(int)I.m[c]) * (I.pair_src_unit ? 1.f : I.mu[c]);
+      atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
+      if (R.log) J.log_cells[pos] = c;
+    }
+    base += total;
+  }
+  __syncthreads();
+  if (tid < I.ux * I.uy) ws.trunc[tid] = (base > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
+  if (tid == 0) {
+    ws.info[0] = 0;
+    ws.info[1] = 0;
+    ws.info[2] = 1;
+    *ws.acc_n = base;
+    if (R.log) J.log_count[0] = base;
+    S.switches += base;  // core.py:713-715
+    S.attempts += base;
+    S.step += 1;
+    S.steps_done += 1;
+    J.st[0] = S;
+  }
+}
+
 int check_select(const hs_select_t *sel) {
   if (sel->scheme == 0) {
     if (sel->Rx < 1 || sel->Ry < 1 || sel->ncols < 1 || sel->nsc < 1 || !sel->pat_off || !sel->pat_cells)
@@ -1284,6 +1399,58 @@ extern "C" int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, in
   HS_LAUNCH_CHECK("k_flip_select");
   k_relax_update<<<(unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS), MU_THREADS, 0, s>>>(J);
   HS_LAUNCH_CHECK("k_relax_update");
+  return HS_OK;
+}
+
+extern "C" int64_t hs_wolff_ws_bytes(const hs_instance_t *inst) {
+  if (!inst || inst->nx < 1 || inst->ny < 1) return -1;
+  const size_t ncell = (size_t)inst->nx * inst->ny;
+  return (int64_t)(ws_bytes((int)ncell) + 2 * align256(ncell) + align256(4 * ncell));
+}
+
+extern "C" int hs_wolff_run(const hs_instance_t *inst, const hs_run_t *run, int32_t cutoff, int32_t reduced_k,
+                            hs_state_t *state, void *ws, int64_t ws_bytes_, int32_t *log_cells, int32_t *log_count,
+                            void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!run || !state || !ws) return hs_set_error(HS_ERR_ARG, "missing run/state/ws");
+  if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
+  if (run->log && (!log_cells || !log_count || run->max_steps > 1))
+    return hs_set_error(HS_ERR_ARG, "the cluster log needs its arrays and a single step");
+  if (inst->J == 0.0 || inst->nn_w <= 0) return hs_set_error(HS_ERR_ARG, "Wolff clusters need an exchange energy");
+  if (cutoff < 0 || reduced_k < 0) return hs_set_error(HS_ERR_ARG, "negative cutoff / reduced kernel size");
+  if (ws_bytes_ < hs_wolff_ws_bytes(inst)) return hs_set_error(HS_ERR_ARG, "Wolff workspace too small");
+  const int ncell = inst->nx * inst->ny;
+  MetroJob J;
+  J.I = *inst;
+  J.SEL = hs_select_t{};
+  J.SEL.scheme = 1;
+  J.SEL.nsc = ncell;
+  J.SEL.cutoff = cutoff;
+  J.SEL.reduced_k = reduced_k;
+  J.R = *run;
+  J.st = state;
+  J.ws = carve(ws, ncell);
+  J.log_cells = log_cells;
+  J.log_count = log_count;
+  J.tiles_x = (inst->nx + MU_TX - 1) / MU_TX;
+  J.tiles_y = (inst->ny + MU_TY - 1) / MU_TY;
+  char *p = (char *)ws + ws_bytes(ncell);
+  WolffWS W;
+  W.cl = (int8_t *)p;
+  W.ck = (int8_t *)(p + align256(ncell));
+  W.fr = (float *)(p + 2 * align256(ncell));
+  const int nt = J.tiles_x * J.tiles_y;
+  hipStream_t s = (hipStream_t)stream;
+  k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
+  HS_LAUNCH_CHECK("k_state_begin");
+  for (int64_t j = 0; j < run->max_steps; ++j) {
+    k_wolff_select<<<1, MS_THREADS, 0, s>>>(J, W);
+    HS_LAUNCH_CHECK("k_wolff_select");
+    k_relax_update<<<(unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS), MU_THREADS, 0, s>>>(J);
+    HS_LAUNCH_CHECK("k_relax_update");
+  }
   return HS_OK;
 }
 

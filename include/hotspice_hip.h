@@ -357,6 +357,21 @@ int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, int32_t k, in
 int64_t hs_flip_ws_bytes(const hs_instance_t *inst);
 
 /*
+ * Wolff cluster steps (Magnets._update_Wolff core.py:711-718 with _select_cluster_exchange
+ * core.py:588-602), looped as Magnets.progress until run's stop predicate or max_steps: a seed
+ * drawn as _select_single draws it (Philox tag 2, index 2), then every iteration each unchecked
+ * cell aligned with the seed and next to the cluster joins with probability
+ * 1 - exp(-2 J/kT * sum of the NN weights of its cluster neighbours), on the Philox uniform
+ * (cell, step, tag 7 | iteration << 8); the whole cluster switches and the fields follow as
+ * update_single / update_multiple (cutoff / reduced_k as in hs_relax_sweep). Needs an exchange
+ * energy. run->log: log_cells[0..k) = the cluster (row-major), log_count[0] = k (one step only).
+ * ws: device scratch of hs_wolff_ws_bytes bytes. Asynchronous.
+ */
+int hs_wolff_run(const hs_instance_t *inst, const hs_run_t *run, int32_t cutoff, int32_t reduced_k, hs_state_t *state,
+                 void *ws, int64_t ws_bytes, int32_t *log_cells, int32_t *log_count, void *stream);
+int64_t hs_wolff_ws_bytes(const hs_instance_t *inst);
+
+/*
  * Neel-Arrhenius steps for many instances in ONE persistent cooperative launch (same
  * semantics per instance as hs_neel_run: core.py:679-709 looped as core.py:736-760, and
  * bit-identical results). Each team keeps its magnets' fields in registers and draws the same

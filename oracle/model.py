@@ -28,6 +28,7 @@ Restated reference code (file:line in bvwaeyen/Hotspice):
   calc_r ..................... hotspice/core.py:720-733
   progress ................... hotspice/core.py:736-760, lower_than utils.py:90-92
   minimize / _minimize_all / relax  hotspice/core.py:763-845
+  _update_Wolff / _select_cluster_exchange  hotspice/core.py:588-602, 711-718
 """
 import math
 
@@ -782,10 +783,46 @@ class OracleMagnets:
             return None
         if self.params["UPDATE_SCHEME"] in ("Neel", "Néel"):
             out = self._update_Neel(*args, **kwargs)
+        elif self.params["UPDATE_SCHEME"] == "Wolff":
+            out = self._update_Wolff(*args, **kwargs)
         else:
             out = self._update_Metropolis(*args, **kwargs)
         self.step += 1
         return out
+
+    def _select_cluster_exchange(self):
+        """core.py:588-602. Philox mode: seed = _select_single's draw, bonds on the uniform
+        (cell, step, tag 7 | iteration << 8); NumPy mode: the reference's own draws."""
+        ex = self.get_energy("exchange")
+        neighbors = ex.local_interaction
+        seed = tuple(self._select_single(0).flat)
+        cluster = np.zeros_like(self.m)
+        cluster[seed] = 1
+        checked = np.copy(cluster)
+        checked[np.where(self.m != self.m[seed])] = 1
+        bnd = "wrap" if self.PBC else "fill"
+        it = 0
+        while np.any(frontier := (1 - checked) * signal.convolve2d(cluster, neighbors, mode="same", boundary=bnd)):
+            checked[frontier != 0] = 1
+            bond_prob = 1 - np.exp(-2 * ex.J / self.kBT * frontier)
+            if self.rng_mode == "numpy":
+                u = self._np_rng.random(size=cluster.shape)
+            else:
+                u = philox.uniform53(self.seed, self.step, philox.TAG_WOLFF | (it << 8),
+                                     np.arange(self.nx * self.ny)).reshape(cluster.shape)
+            frontier[u > bond_prob] = 0
+            cluster[frontier != 0] = 1
+            it += 1
+        return np.asarray(np.where(cluster == 1))
+
+    def _update_Wolff(self):
+        """core.py:711-718"""
+        cluster = self._select_cluster_exchange()
+        self.m[cluster[0], cluster[1]] *= -1
+        self.switches += cluster.shape[1]
+        self.attempted_switches += cluster.shape[1]
+        self.update_energy(index=cluster)
+        return cluster
 
     def progress(self, t_max=1, MCsteps_max=4.0, **kwargs):
         """core.py:736-760"""

@@ -34,6 +34,7 @@ TAG_SEL = 2       # grid selection: supergrid offsets and single-magnet fallback
 TAG_CHOICE = 3    # grid selection: occupied cell inside each supercell (core.py:538)
 TAG_ACCEPT = 4    # Metropolis acceptance, one per sample, indexed by its grid cell (core.py:671)
 TAG_MTIME = 5     # METROPOLIS_TIME exponentials, indexed by the sample's grid cell (core.py:668)
+TAG_WOLFF = 7     # Wolff bonds: one per grid cell per cluster iteration, tag 7 | iteration << 8 (core.py:599)
 
 TWO_M53 = 2.0 ** -53
 

@@ -297,3 +297,54 @@ def test_flip_cells_equals_update_energy():
         np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * np.abs(om.E).max())
     with pytest.raises(IndexError):
         mm._dev.flip_cells([mm.nx * mm.ny], 20, 20)
+
+
+@pytest.mark.parametrize("lat,n,pbc,extra", [("OOP_Square", 24, True, "zeeman"), ("OOP_Square", 40, False, None),
+                                             ("IP_Square", 20, True, "dimonopolar")])
+def test_wolff_matches_oracle(lat, n, pbc, extra):
+    """Wolff cluster steps (core.py:588-602, 711-718) on the device against the oracle's Philox
+    replay: the same clusters step by step, spins bit-exact, fields at 1e-5 of their scale."""
+    import hotspice_amd as hs
+    from oracle import lattices, model
+    kT = hs.kB * 300
+    g = lattices.make(lat, 1e-6, n=n, PBC=pbc)
+    rng = np.random.default_rng(12)
+    pat = np.where(rng.random((g.ny, g.nx)) < .5, -1., 1.)
+    pe, oe = [hs.ExchangeEnergy(J=0.45 * kT)], [model.Exchange(J=0.45 * kT)]
+    if extra == "zeeman":
+        pe.append(hs.ZeemanEnergy(1e-4, 0.3))
+        oe.append(model.Zeeman(1e-4, 0.3))
+    elif extra == "dimonopolar":
+        pe.append(hs.energies.DiMonopolarEnergy(d=300e-9))
+        oe.append(model.DiMonopolar(d=300e-9))
+    mm = getattr(hs.ASI, lat)(1e-6, n, T=300, moment=2.37e-16, pattern=pat, energies=pe, seed=6, PBC=pbc,
+                              params=hs.SimParams(UPDATE_SCHEME='Wolff'))
+    om = model.make_magnets(lat, 1e-6, n=n, T=300, moment=2.37e-16, pattern=pat, energies=oe, PBC=pbc, seed=6,
+                            params=dict(UPDATE_SCHEME="Wolff"))
+    sizes = []
+    for s in range(40):
+        a = mm.update()
+        b = om.update()
+        np.testing.assert_array_equal(a, b, err_msg=f"step {s}")
+        sizes.append(a.shape[1])
+    assert max(sizes) > 1
+    np.testing.assert_array_equal(mm.m, om.m)
+    assert mm.switches == om.switches and mm.attempted_switches == om.attempted_switches
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * np.abs(om.E).max())
+    # progress() chunks the same steps on the device
+    mm.progress(MCsteps_max=2.0)
+    om.progress(MCsteps_max=2.0)
+    np.testing.assert_array_equal(mm.m, om.m)
+    assert mm.attempted_switches == om.attempted_switches
+
+
+def test_wolff_needs_exchange_and_no_dipolar():
+    import hotspice_amd as hs
+    mm = hs.ASI.OOP_Square(1e-6, 8, T=300, seed=1, energies=[hs.DipolarEnergy()],
+                           params=hs.SimParams(UPDATE_SCHEME='Wolff'))
+    with pytest.raises(NotImplementedError):
+        mm.update()
+    mm = hs.ASI.OOP_Square(1e-6, 8, T=300, seed=1, energies=[hs.ZeemanEnergy(1e-3)],
+                           params=hs.SimParams(UPDATE_SCHEME='Wolff'))
+    with pytest.raises(NotImplementedError):
+        mm.progress()
