@@ -232,21 +232,24 @@ class FieldInputterBinary(FieldInputter):
 class PerpFieldInputter(FieldInputter):
     def __init__(self, datastream: BinaryDatastream, magnitude=1, angle=0, n=2, relax=True, frequency=1, **kwargs):
         """io.py:259-268: bit 0 at angle `angle`, bit 1 at `angle + pi/2`, each as a positive then
-        a negative field. relax=True (zero-temperature minimisation between the two) is not part of
-        the device path and raises."""
+        a negative field; with relax=True the magnets are minimised (Magnets.minimize) under each
+        of the two fields instead of stepped."""
         self.relax = relax
         super().__init__(datastream, magnitude=magnitude, angle=angle, n=n, frequency=frequency, **kwargs)
 
     def input_single(self, mm: Magnets, value):
-        """io.py:270-288 (relax=False branch)"""
+        """io.py:270-288"""
         if (self.sine or self.frequency) and mm.params.UPDATE_SCHEME != Scheme.NEEL:
             raise AttributeError("Can not use temporal sine=True or nonzero frequency if UPDATE_SCHEME != Scheme.NEEL.")
         if not mm.in_plane:
             raise AttributeError("Can not use PerpFieldInputter on an out-of-plane ASI.")
-        if self.relax:
-            raise NotImplementedError("PerpFieldInputter(relax=True) needs Magnets.minimize(), which is not on the device path.")
         Zeeman = _zeeman(mm)
         angle = self.angle + value * math.pi / 2
+        if self.relax:
+            for mag in (self.magnitude, -self.magnitude):
+                Zeeman.set_field(magnitude=mag, angle=angle)
+                mm.minimize()
+            return
         Zeeman.set_field(magnitude=self.magnitude, angle=angle)
         mm.progress(t_max=0.5 / self.frequency, MCsteps_max=self.n)
         Zeeman.set_field(magnitude=-self.magnitude, angle=angle)

@@ -148,3 +148,22 @@ def test_chess_steps_inputter_metropolis():
         Z.set_field(magnitude=(mask - 1) * mag)
         b.progress(t_max=1., MCsteps_max=1.0)
     _same(_state(a), _state(b))
+
+
+def test_perp_inputter_relax_minimizes_under_each_field():
+    """PerpFieldInputter(relax=True) (io.py:280-284): a positive then a negative field at the bit's
+    angle, each followed by Magnets.minimize, equal to that sequence applied by hand to a twin
+    with the same numpy.random draws."""
+    from hotspice_amd import io
+    a, b = _pinwheel(n=20, seed=10), _pinwheel(n=20, seed=10)
+    inp = io.PerpFieldInputter(io.RandomBinaryDatastream(), magnitude=3e-3, angle=0.2, relax=True, frequency=0)
+    np.random.seed(1)
+    inp.input(a, [0, 1])
+    np.random.seed(1)
+    Z = b.get_energy('Zeeman')
+    for bit in [0, 1]:
+        for mag in (3e-3, -3e-3):
+            Z.set_field(magnitude=mag, angle=0.2 + bit * math.pi / 2)
+            b.minimize()
+    _same(_state(a), _state(b))
+    assert a.switches > 0
