@@ -227,8 +227,15 @@ class DeviceState:
                                         self.stream()))
 
     def run_teams(self, teams, kind):
-        """One cooperative launch stepping every team (instance) in `teams`."""
-        arr = (_lib.Team * len(teams))(*teams)
+        """One cooperative launch stepping every team (instance) in `teams`: a list of _lib.Team or
+        a NumPy array of the same records."""
+        if isinstance(teams, np.ndarray):
+            keep = teams
+            arr = keep.ctypes.data_as(C.POINTER(_lib.Team))
+            states = keep['state'].tolist()
+        else:
+            arr = (_lib.Team * len(teams))(*teams)
+            states = [t.state for t in teams]
         lib = self.lib
         size_fn, run_fn = (lib.hs_neel_teams_ws_bytes, lib.hs_neel_run_teams) if kind == "neel" else \
             (lib.hs_metro_teams_ws_bytes, lib.hs_metro_run_teams)
@@ -239,7 +246,7 @@ class DeviceState:
             self.team_ws = self.torch.zeros(max(need, 256), dtype=self.torch.uint8, device=self.device)
         ev = self._event_start()
         _lib.check(run_fn(arr, len(teams), self.ptr(self.team_ws), self.team_ws.numel(), self.stream()))
-        self._event_end(ev, [t.state for t in teams])
+        self._event_end(ev, states)
 
     def _event_start(self):
         """Timing hook: when launch_events is a list, HIP events on the launch stream bracket the

@@ -11,6 +11,8 @@ them. launch_neel / launch_metropolis are the single-launch building blocks (use
 """
 import math
 
+import numpy as np
+
 from . import _lib
 from .core import Scheme
 from .utils import lower_than
@@ -42,15 +44,16 @@ def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None,
     nbl = nblocks if isinstance(nblocks, (list, tuple)) else [nblocks or 0] * len(mags)
     if scheme == Scheme.NEEL:  # per-instance team sizes ('auto' instances are tuned first)
         nbl = [nb or (m._neel_nblocks() if m.neel_blocks else 0) for m, nb in zip(mags, nbl)]
-    starts = [(st.t, st.attempts) for st in read_states(mags)]
+    sts = read_states(mags)
+    starts = [(st.t, st.attempts) for st in sts]
     attempt_freq = float(kwargs.get('attempt_freq', 1e10))
     if per_instance is not None and len(per_instance) != len(mags):
         raise ValueError("per_instance needs one keyword dict per instance")
     kws = [dict(kwargs, **(per_instance[i] if per_instance is not None else {})) for i in range(len(mags))]
     if scheme == Scheme.NEEL:
-        return _progress_many_neel(mags, starts, t_max, MCsteps_max, attempt_freq, nbl, chunk)
+        return _progress_many_neel(mags, starts, t_max, MCsteps_max, attempt_freq, nbl, chunk, sts)
     if scheme == Scheme.METROPOLIS:
-        return _progress_many_metropolis(mags, starts, t_max, MCsteps_max, attempt_freq, chunk, kws)
+        return _progress_many_metropolis(mags, starts, t_max, MCsteps_max, attempt_freq, chunk, kws, sts)
     raise NotImplementedError("batched Wolff updates are not supported")
 
 
@@ -59,8 +62,8 @@ def read_states(mags):
     if not mags:
         return []
     torch = mags[0]._dev.torch
-    raw = torch.stack([m._dev.state[:_lib.STATE_BYTES] for m in mags]).cpu().numpy()
-    return [_lib.State.from_buffer_copy(raw[i].tobytes()) for i in range(len(mags))]
+    raw = torch.stack([m._dev.state for m in mags]).cpu().numpy()[:, :_lib.STATE_BYTES]
+    return [_lib.State.from_buffer_copy(row) for row in np.ascontiguousarray(raw)]
 
 
 def _pending(m, start, t_max, mc, st=None):
@@ -87,17 +90,36 @@ def launch_neel(items, attempt_freq=1e10, begin=False):
     when due."""
     if not items:
         return
-    dev = items[0][0]._dev
-    teams = []
-    for m, steps, t_max, mc, nb in items:
+    mags = [it[0] for it in items]
+    dev = mags[0]._dev
+    for m in mags:
         if not m._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-        run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=float(attempt_freq), max_steps=int(steps),
-                       log=0, nblocks=0, begin=int(begin))
-        teams.append(_lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=None, run=run, state=m._dev.state.data_ptr(),
-                               log_idx=None, log_dt=None, log_cells=None, nblocks=int(nb)))
+    teams = _team_array(mags, [it[1] for it in items], [it[2] for it in items], [it[3] for it in items],
+                        attempt_freq, begin, nblocks=[it[4] for it in items])
     dev.run_teams(teams, "neel")
-    return _after_launch([it[0] for it in items], "Neel")
+    return _after_launch(mags, "Neel")
+
+
+_TEAM_DTYPE = np.dtype(_lib.Team)
+
+
+def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=None):
+    """The hs_team_t descriptors of one launch, filled column by column."""
+    arr = np.zeros(len(mags), dtype=_TEAM_DTYPE)
+    arr['inst'] = [_lib.C.addressof(m._dev.inst) for m in mags]
+    arr['state'] = [m._dev.state.data_ptr() for m in mags]
+    if sels is not None:
+        arr['sel'] = [_lib.C.addressof(s) for s in sels]
+    run = arr['run']
+    run['t_max'] = np.asarray(t_max, dtype=float)
+    run['mc_max'] = np.asarray(mc, dtype=float)
+    run['attempt_freq'] = float(attempt_freq)
+    run['max_steps'] = np.asarray(steps, dtype=np.int64)
+    run['begin'] = int(begin)
+    if nblocks is not None:
+        arr['nblocks'] = np.asarray(nblocks, dtype=np.int32)
+    return arr
 
 
 def launch_metropolis(items, attempt_freq=1e10, begin=False):
@@ -106,16 +128,13 @@ def launch_metropolis(items, attempt_freq=1e10, begin=False):
     launch_neel)."""
     if not items:
         return
-    dev = items[0][0]._dev
-    teams = []
-    for m, sel, steps, t_max, mc in items:
-        run = _lib.Run(t_max=float(t_max), mc_max=float(mc), attempt_freq=float(attempt_freq), max_steps=int(steps),
-                       log=0, nblocks=0, begin=int(begin))
-        teams.append(_lib.Team(inst=_lib.C.addressof(m._dev.inst), sel=_lib.C.addressof(sel), run=run,
-                               state=m._dev.state.data_ptr(), log_idx=None, log_dt=None, log_cells=None, nblocks=0))
-    before = [st.attempts for st in read_states([it[0] for it in items])]
+    mags = [it[0] for it in items]
+    dev = mags[0]._dev
+    teams = _team_array(mags, [it[2] for it in items], [it[3] for it in items], [it[4] for it in items],
+                        attempt_freq, begin, sels=[it[1] for it in items])
+    before = [st.attempts for st in read_states(mags)]
     dev.run_teams(teams, "metropolis")
-    return _after_launch([it[0] for it in items], "Metropolis", before)
+    return _after_launch(mags, "Metropolis", before)
 
 
 def _after_launch(mags, what, before=None):
@@ -170,22 +189,48 @@ def metropolis_chunk(m, sel, start, mc, chunk=None, st=None):
     return max(1, steps)
 
 
-def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk):
+def _lower_than_vec(x, y, rtol=1e-10):
+    """lower_than (utils.py:90-92) over arrays."""
+    x, y = np.asarray(x, dtype=float), np.asarray(y, dtype=float)
+    return (x < y) & ~np.isclose(x, y, rtol=rtol, atol=0)
+
+
+def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk, sts):
+    """The pending test and the chunk sizes of neel_chunk, over all instances at once."""
     first = True
-    sts = read_states(mags)
+    n = np.array([m.n for m in mags], dtype=float)
+    t0 = np.array([s[0] for s in starts], dtype=float)
+    a0 = np.array([s[1] for s in starts], dtype=np.int64)
+    every = np.array([m.recompute_every or 0 for m in mags], dtype=np.int64)
     while True:
-        items = [(m, neel_chunk(m, s0, t_max, mc, chunk, st), t_max, mc, nb)
-                 for m, s0, nb, st in zip(mags, starts, nbl, sts) if _pending(m, s0, t_max, mc, st)[0]]
-        if not items:
+        t = np.array([st.t for st in sts], dtype=float)
+        att = np.array([st.attempts for st in sts], dtype=np.int64)
+        pend = _lower_than_vec(t - t0, t_max) & _lower_than_vec(att / n - a0 / n, mc)
+        idx = np.flatnonzero(pend)
+        if idx.size == 0:
             break
+        if math.isfinite(mc):
+            steps = np.ceil(mc * n[idx] - (att[idx] - a0[idx])).astype(np.int64) + 1
+        else:
+            steps = np.full(idx.size, 4096, dtype=np.int64)
+        if chunk:
+            steps = np.minimum(steps, chunk)
+        if math.isfinite(t_max):
+            steps = np.minimum(steps, 4096)
+        since = np.array([mags[i]._since_recompute for i in idx], dtype=np.int64)
+        ev = every[idx]
+        steps = np.where(ev > 0, np.minimum(steps, np.maximum(1, ev - since)), steps)
+        steps = np.maximum(1, steps)
+        items = [(mags[i], int(k), t_max, mc, nbl[i]) for i, k in zip(idx.tolist(), steps.tolist())]
         # the first launch opens every pending instance's progress call (origin = starts)
-        after = dict(zip((id(it[0]) for it in items), launch_neel(items, f0, begin=first)))
+        after = launch_neel(items, f0, begin=first)
         first = False
-        sts = [after.get(id(m), st) for m, st in zip(mags, sts)]
+        for i, st in zip(idx.tolist(), after):
+            sts[i] = st
     return _results(mags, starts, sts)
 
 
-def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws):
+def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws, sts):
     sels = []
     for m, kw in zip(mags, kws):
         if kw.get('idx') is not None:
@@ -194,7 +239,6 @@ def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws):
         Q = kw.get('Q', 0.05)
         sels.append(m._select_geometry(m.calc_r(Q) if r == 0 else r))
     first = True
-    sts = read_states(mags)
     while True:
         items = [(m, sel, metropolis_chunk(m, sel, s0, mc, chunk, st), t_max, mc)
                  for m, s0, sel, st in zip(mags, starts, sels, sts) if _pending(m, s0, t_max, mc, st)[0]]

@@ -357,6 +357,7 @@ class Magnets(ABC):
 
     @property
     def E_B(self):
+        self._EB_exposed = True  # the caller may edit the array in place: compare before the next step
         return self._E_B
 
     @E_B.setter
@@ -367,9 +368,12 @@ class Magnets(ABC):
             self._E_B_inv_over_four = .25 / self._E_B
         self._dev.upload(self._dev.EB, self._E_B.astype(np.float32))
         self._EB_ref = self._E_B.copy()
+        self._EB_exposed = False  # a new array: no reference to it has been handed out yet
 
     def _sync_EB(self):
-        if not np.array_equal(self._E_B, self._EB_ref):
+        """Uploads E_B if it was edited in place since the last upload (only possible once the
+        array has been handed out by the E_B getter)."""
+        if self._EB_exposed and not np.array_equal(self._E_B, self._EB_ref):
             self.E_B = self._E_B
 
     @property
@@ -1196,7 +1200,7 @@ class Magnets(ABC):
 
     @property
     def E_B_avg(self) -> float:
-        return float(np.mean(self.E_B))
+        return float(np.mean(self._E_B))
 
     @property
     def moment_avg(self) -> float:

@@ -159,16 +159,15 @@ class Server:
                 items.append((a.mm, steps, math.inf, math.inf, g))
                 used += g
                 self.attempts -= st.attempts
-            batch.launch_neel(items, 1e10)
-            self.attempts += sum(st.attempts for st in batch.read_states([it[0] for it in items]))
+            self.attempts += sum(st.attempts for st in batch.launch_neel(items, 1e10))
         if metro:
             items = []
             for a, st in zip(metro, batch.read_states([a.mm for a in metro])):
                 steps = batch.metropolis_chunk(a.mm, a.sel, a.start, a.job.work, self.metro_chunk, st)
                 items.append((a.mm, a.sel, steps, math.inf, a.job.work))
                 self.attempts -= st.attempts
-            batch.launch_metropolis(items, metro[0].job.kwargs.get('attempt_freq', 1e10))
-            self.attempts += sum(st.attempts for st in batch.read_states([a.mm for a in metro]))
+            after = batch.launch_metropolis(items, metro[0].job.kwargs.get('attempt_freq', 1e10))
+            self.attempts += sum(st.attempts for st in after)
 
     def _retire(self, now):
         keep = []
