@@ -538,6 +538,9 @@ __global__ void __launch_bounds__(256) k_recompute_dip_many(const RecJob *__rest
   rec_dip(J.I, J.plan, J.K, J.part, lb % nt, (lb / nt) % J.I.n_sub, lb / (nt * J.I.n_sub));
 }
 
+__global__ void __launch_bounds__(1024) k_recompute_small_lds_many(const RecJob *__restrict__ jobs,
+                                                                   const int32_t *__restrict__ job_of_block);
+
 __global__ void __launch_bounds__(256) k_recompute_small_many(const RecJob *__restrict__ jobs,
                                                               const int32_t *__restrict__ block_job) {
   const RecJob &J = jobs[block_job[blockIdx.x]];
@@ -630,6 +633,57 @@ __global__ void k_recompute_small(hs_instance_t I, const double *Kbase, double *
   rec_small(I, Kbase, part, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Small grids whose kernels (every sublattice's window) fit in SMALL_LDS_K doubles: one
+// 1024-thread workgroup per grid stages (m w) and the kernels in LDS once, so the inner loop reads
+// LDS only (rec_small re-reads every kernel entry from L2 for every target). Same arithmetic and
+// order as rec_small.
+constexpr int SMALL_LDS_K = 10240;
+constexpr int SMALL_LDS_THREADS = 1024;
+
+__device__ __forceinline__ void rec_small_lds(const hs_instance_t &I, const double *Kbase, double *part) {
+  __shared__ double s_v[4096];
+  __shared__ double s_k[SMALL_LDS_K];
+  const int nx = I.nx, ny = I.ny, ncell = nx * ny;
+  const int KW = 2 * I.kwx + 1, KS = KW * (2 * I.kwy + 1);
+  for (int s = threadIdx.x; s < ncell; s += blockDim.x)
+    s_v[s] = I.pair_src_unit ? (double)I.m[s] : (double)I.m[s] * I.pw64[s];
+  for (int e = threadIdx.x; e < KS * I.n_sub; e += blockDim.x) s_k[e] = Kbase[e];
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const int y = c / nx, x = c % nx, n = I.sub_of[hs_site(I, y, x)];
+    double acc = 0.0;
+    if (n >= 0 && I.m[c] != 0) {
+      const double *Kn = s_k + n * KS;
+      const int y0 = max(0, y - I.kwy), y1 = min(ny - 1, y + I.kwy);
+      const int x0 = max(0, x - I.kwx), x1 = min(nx - 1, x + I.kwx);
+      for (int ys = y0; ys <= y1; ++ys) {
+        const double *krow = Kn + (ys - y + I.kwy) * KW + (I.kwx - x);
+        const double *vrow = s_v + ys * nx;
+        for (int xs = x0; xs <= x1; ++xs) {
+          const double v = vrow[xs];
+          if (v != 0.0) acc = fma(krow[xs], v, acc);
+        }
+      }
+    }
+    part[c] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_recompute_small_lds_many(const RecJob *__restrict__ jobs,
+                                                                   const int32_t *__restrict__ job_of_block) {
+  const RecJob &J = jobs[job_of_block[blockIdx.x]];
+  rec_small_lds(J.I, J.K, J.part);
+}
+
+static bool small_lds_rec(const hs_instance_t *I) {
+  return I->nx * I->ny <= 4096 && (int64_t)(2 * I->kwx + 1) * (2 * I->kwy + 1) * I->n_sub <= SMALL_LDS_K;
+}
+
+__global__ void __launch_bounds__(SMALL_LDS_THREADS) k_recompute_small_lds(hs_instance_t I, const double *Kbase,
+                                                                          double *part) {
+  rec_small_lds(I, Kbase, part);
+}
+
 struct FinJob {
   hs_instance_t I;
   const double *partE, *partP;
@@ -697,12 +751,14 @@ extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, voi
     const dim3 grid(pl.tA * pl.tB, inst->n_sub, pl.F);
     const unsigned sb = (unsigned)((ncell + 255) / 256);
     if (which & 1) {
-      if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->K64, partE);
+      if (small_lds_rec(inst)) k_recompute_small_lds<<<1, SMALL_LDS_THREADS, 0, s>>>(*inst, inst->K64, partE);
+      else if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->K64, partE);
       else k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->K64, partE);
       HS_LAUNCH_CHECK("k_recompute_dip");
     }
     if (do_p) {
-      if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->Kps64, partP);
+      if (small_lds_rec(inst)) k_recompute_small_lds<<<1, SMALL_LDS_THREADS, 0, s>>>(*inst, inst->Kps64, partP);
+      else if (small_rec(inst)) k_recompute_small<<<sb, 256, 0, s>>>(*inst, inst->Kps64, partP);
       else k_recompute_dip<<<grid, 256, 0, s>>>(*inst, pl, inst->Kps64, partP);
       HS_LAUNCH_CHECK("k_recompute_dip");
     }
@@ -715,8 +771,8 @@ extern "C" int hs_recompute_fields(const hs_instance_t *inst, int32_t which, voi
 namespace {
 
 struct ManyLayout {
-  size_t dip_off, fin_off, dmap_off, fmap_off, total;
-  int ndip, dip_blocks, fin_blocks;
+  size_t dip_off, fin_off, dmap_off, fmap_off, lmap_off, total;
+  int ndip, dip_blocks, fin_blocks, nlds;
 };
 
 size_t al256c(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -727,14 +783,13 @@ ManyLayout many_layout(const hs_instance_t *const *insts, const int32_t *which, 
     const hs_instance_t &I = *insts[i];
     const RecPlan pl = make_plan(&I);
     if (I.dip_pf != 0.0) {
-      const int per = small_rec(&I) ? (I.nx * I.ny + 255) / 256 : pl.tA * pl.tB * I.n_sub * pl.F;
-      if (which[i] & 1) {
+      const bool lds = small_lds_rec(&I);
+      const int per = lds ? 0 : small_rec(&I) ? (I.nx * I.ny + 255) / 256 : pl.tA * pl.tB * I.n_sub * pl.F;
+      for (int w = 1; w <= 2; w <<= 1) {
+        if (!(which[i] & w) || (w == 2 && !I.use_perp)) continue;
         ++L.ndip;
         L.dip_blocks += per;
-      }
-      if ((which[i] & 2) && I.use_perp) {
-        ++L.ndip;
-        L.dip_blocks += per;
+        L.nlds += lds ? 1 : 0;
       }
     }
     L.fin_blocks += (I.nx * I.ny + 255) / 256;
@@ -743,7 +798,8 @@ ManyLayout many_layout(const hs_instance_t *const *insts, const int32_t *which, 
   L.fin_off = al256c(sizeof(RecJob) * (size_t)(L.ndip > 0 ? L.ndip : 1));
   L.dmap_off = L.fin_off + al256c(sizeof(FinJob) * (size_t)n);
   L.fmap_off = L.dmap_off + al256c(4 * (size_t)(L.dip_blocks > 0 ? L.dip_blocks : 1));
-  L.total = L.fmap_off + al256c(4 * (size_t)L.fin_blocks);
+  L.lmap_off = L.fmap_off + al256c(4 * (size_t)L.fin_blocks);
+  L.total = L.lmap_off + al256c(4 * (size_t)(L.nlds > 0 ? L.nlds : 1));
   return L;
 }
 
@@ -785,11 +841,13 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
   FinJob *fj = (FinJob *)&host[L.fin_off];
   int32_t *dmap = (int32_t *)&host[L.dmap_off];
   int32_t *fmap = (int32_t *)&host[L.fmap_off];
-  // block numbering: every small job's blocks first, then the tiled jobs' blocks
-  int dbs = 0;
+  int32_t *lmap = (int32_t *)&host[L.lmap_off];
+  // block numbering: every small job's blocks first, then the tiled jobs' blocks (jobs whose
+  // kernels fit in LDS take one workgroup each, listed in lmap)
+  int dbs = 0, nl = 0;
   for (int i = 0; i < n; ++i) {
     const hs_instance_t &I = *insts[i];
-    if (I.dip_pf == 0.0 || !small_rec(&I)) continue;
+    if (I.dip_pf == 0.0 || !small_rec(&I) || small_lds_rec(&I)) continue;
     const int per = (I.nx * I.ny + 255) / 256;
     dbs += ((which[i] & 1) ? per : 0) + (((which[i] & 2) && I.use_perp) ? per : 0);
   }
@@ -802,6 +860,7 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
     double *partE = dip ? (double *)scratch[i] : nullptr;
     double *partP = dip ? partE + (size_t)pl.F * ncell : nullptr;
     if (dip) {
+      const bool lds = small_lds_rec(&I);
       const int per = small_rec(&I) ? (ncell + 255) / 256 : pl.tA * pl.tB * I.n_sub * pl.F;
       for (int w = 1; w <= 2; w <<= 1) {
         if (!(which[i] & w) || (w == 2 && !I.use_perp)) continue;
@@ -811,10 +870,15 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
         J.K = w == 1 ? I.K64 : I.Kps64;
         J.part = w == 1 ? partE : partP;
         J.small = small_rec(&I) ? 1 : 0;
-        int &next = J.small ? db_small : db;
-        J.first = next;
-        for (int b = 0; b < per; ++b) dmap[next + b] = nd;
-        next += per;
+        if (lds) {
+          J.first = 0;
+          lmap[nl++] = nd;
+        } else {
+          int &next = J.small ? db_small : db;
+          J.first = next;
+          for (int b = 0; b < per; ++b) dmap[next + b] = nd;
+          next += per;
+        }
         ++nd;
       }
     }
@@ -833,6 +897,11 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
   char *base = (char *)ws;
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), L.total, hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  if (nl > 0) {
+    k_recompute_small_lds_many<<<nl, SMALL_LDS_THREADS, 0, s>>>((const RecJob *)(base + L.dip_off),
+                                                                (const int32_t *)(base + L.lmap_off));
+    HS_LAUNCH_CHECK("k_recompute_small_lds_many");
+  }
   if (dbs > 0) {
     k_recompute_small_many<<<dbs, 256, 0, s>>>((const RecJob *)(base + L.dip_off), (const int32_t *)(base + L.dmap_off));
     HS_LAUNCH_CHECK("k_recompute_small_many");
