@@ -47,6 +47,8 @@ constexpr int MARK_MAX_TILES = 256;
 constexpr int MARK_WORDS = 4096;
 // samples per thread whose loads the selection kernels issue together (and keep in registers)
 constexpr int SELECT_BATCH = 2;
+// workgroup size of the presampling kernel (one sample per thread)
+constexpr int PRESAMPLE_THREADS = 256;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -61,6 +63,7 @@ struct MetroWS {
   uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
   int32_t *pwin;     // [nsc] Poisson: 1 = this round's candidate is accepted
   float *cmmu;       // [nsc] (m mu) of each candidate after its flip (read where accf is set)
+  long long *part;   // [nsc/PRESAMPLE_THREADS] presampling: in-grid samples << 32 | valid samples, per workgroup
 };
 
 // The step's (job, tile) worklist of an update launch: count[par] entries in wl. The selection
@@ -116,6 +119,8 @@ MetroWS carve(void *base, int nsc) {
   p += align256(n * 4);
   w.cmmu = (float *)p;
   p += align256(n * 4);
+  w.part = (long long *)p;
+  p += align256(8 * ((n + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS));
   return w;
 }
 
@@ -123,7 +128,7 @@ MetroWS carve(void *base, int nsc) {
 size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
   return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
-         7 * align256(n * 4);
+         7 * align256(n * 4) + align256(8 * ((n + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS));
 }
 
 // A worklist for `ntiles` tiles: the entries and the two step counters.
@@ -325,7 +330,50 @@ HS_INLINE bool mark_rect(uint32_t *bits, int tiles_x, int nx, int ny, int sx, in
 // [j0, j1) and works on them in batches of QB: within a batch the independent loads of all its
 // samples are issued together, and when a thread has at most QB samples their candidate cells,
 // acceptance flags and (m mu) stay in registers from phase to phase instead of being re-read.
-template <int QB>
+// The grid selection's supergrid offset of one attempt (core.py:527-531).
+HS_INLINE void grid_offsets(const hs_instance_t &I, const hs_select_t &SEL, uint64_t b0, uint64_t b1, int &ox,
+                            int &oy) {
+  if (I.pbc) {  // np.random.randint(nx), randint(ny) (core.py:527)
+    ox = (int)floor(hs_u53(b0, 0) * I.nx);
+    oy = (int)floor(hs_u53(b1, 0) * I.ny);
+  } else {  // randint(-Rx, Rx), randint(-Ry, Ry) (core.py:529)
+    ox = -SEL.Rx + (int)floor(hs_u53(b0, 0) * (2.0 * SEL.Rx));
+    oy = -SEL.Ry + (int)floor(hs_u53(b1, 0) * (2.0 * SEL.Ry));
+  }
+  if (SEL.Rx == I.nx) ox = 0;  // core.py:530-531
+  if (SEL.Ry == I.ny) oy = 0;
+}
+
+// The cell of sample j (chosen supercell j of the every-other grid) for pattern cell pc, and
+// whether it lies in the grid (core.py:540-552).
+HS_INLINE int grid_cell(const hs_instance_t &I, const hs_select_t &SEL, int ox, int oy, int j, int pc, bool &ok) {
+  const int nx = I.nx, ny = I.ny;
+  const int sy = 2 * (j / SEL.ncols), sx = 2 * (j % SEL.ncols);
+  int x = sx * SEL.Rx + (pc & 0xffff), y = sy * SEL.Ry + (pc >> 16);
+  if (I.pbc) {
+    ok = (x < max(SEL.Rx, nx - SEL.Rx)) && (y < max(SEL.Ry, ny - SEL.Ry));  // core.py:545
+    x = (x + ox) % nx;
+    y = (y + oy) % ny;
+  } else {
+    x += ox;
+    y += oy;
+    ok = x >= 0 && x < nx && y >= 0 && y < ny;  // core.py:552
+  }
+  return y * nx + x;
+}
+
+// Metropolis acceptance of a sample at cell c (core.py:662-671): exp(-switch_energy*beta) with
+// switch_energy = -2E from the compensated field, relative error ~3e-7 (FP64 exponent split, FP32
+// mantissa), clipped to [1e-10, 1e10], against the Philox uniform (cell, step, tag 4).
+HS_INLINE bool metro_accepts(const hs_instance_t &I, uint64_t step, int c, float E, float Ec, float beta) {
+  const double e = (double)E - (double)Ec;
+  double p = hs_scaled_exp(1.f, 2.0 * e * (double)beta, 1.f, 1.0);
+  p = fmin(fmax(p, 1e-10), 1e10);
+  const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
+  return ua < p;
+}
+
+template <int QB, bool PRE = false>
 __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
                                              int job = 0) {
   const hs_instance_t &I = J.I;
@@ -366,7 +414,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   const int nx = I.nx, ny = I.ny;
   // all of this thread's grid samples fit one batch: they stay in registers (the Poisson scheme
   // writes its candidates to the workspace only)
-  const bool regs = q <= QB && SEL.scheme == 0;
+  const bool regs = !PRE && q <= QB && SEL.scheme == 0;
   int candr[QB], accr[QB];
   float mmur[QB];
 #pragma unroll
@@ -379,7 +427,21 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   bool single = false;
   int attempt = 0;
   int ox = 0, oy = 0;
-  for (; attempt < MAX_ATTEMPTS; ++attempt) {
+  // PRE: k_metro_presample drew attempt 0's samples and their acceptance already (cand, accf,
+  // cmmu in the workspace, per-workgroup counts in part); used when that attempt has valid samples
+  bool pre = false;
+  if (PRE) {
+    long long mine = 0;
+    for (int b = tid; b < (nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; b += MS_THREADS) mine += ws.part[b];
+    const long long both = block_sum_i64(mine, s_l);
+    const int tot_inb = (int)(both >> 32), tot_val = (int)(both & 0xffffffffll);
+    pre = tot_inb > 0 && tot_val > 0;
+    if (pre) {
+      nvalid = tot_val;
+      grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
+    }
+  }
+  for (; !pre && attempt < MAX_ATTEMPTS; ++attempt) {
     const uint64_t b0 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0));
     const uint64_t b1 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1));
     const uint64_t b2 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 2));
@@ -392,15 +454,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
       continue;  // no occupied sample: select() again (core.py:487)
     }
     if (SEL.scheme == 0) {
-      if (I.pbc) {  // np.random.randint(nx), randint(ny) (core.py:527)
-        ox = (int)floor(hs_u53(b0, 0) * nx);
-        oy = (int)floor(hs_u53(b1, 0) * ny);
-      } else {  // randint(-Rx, Rx), randint(-Ry, Ry) (core.py:529)
-        ox = -SEL.Rx + (int)floor(hs_u53(b0, 0) * (2.0 * SEL.Rx));
-        oy = -SEL.Ry + (int)floor(hs_u53(b1, 0) * (2.0 * SEL.Ry));
-      }
-      if (SEL.Rx == nx) ox = 0;  // core.py:530-531
-      if (SEL.Ry == ny) oy = 0;
+      grid_offsets(I, SEL, b0, b1, ox, oy);
       const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
       const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
       int inb = 0, val = 0;
@@ -424,20 +478,9 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
           cl[u] = -1;
           fl[u] = 0;
           if (j < j1) {
-            const int sy = 2 * (j / SEL.ncols), sx = 2 * (j % SEL.ncols);
-            int x = sx * SEL.Rx + (pcs[u] & 0xffff), y = sy * SEL.Ry + (pcs[u] >> 16);
             bool ok;
-            if (I.pbc) {
-              ok = (x < max(SEL.Rx, nx - SEL.Rx)) && (y < max(SEL.Ry, ny - SEL.Ry));  // core.py:545
-              x = (x + ox) % nx;
-              y = (y + oy) % ny;
-            } else {
-              x += ox;
-              y += oy;
-              ok = x >= 0 && x < nx && y >= 0 && y < ny;  // core.py:552
-            }
+            cl[u] = grid_cell(I, SEL, ox, oy, j, pcs[u], ok);
             oks[u] = ok;
-            cl[u] = y * nx + x;
             if (ok) fl[u] = I.flags[cl[u]];
           }
         }
@@ -488,7 +531,9 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   int my_acc = 0;
   double tsum = 0.0;
   const int a0 = single ? (tid == 0 ? 0 : 1) : j0, a1 = single ? 1 : j1;
-  for (int jb = a0; jb < a1; jb += QB) {
+  if (pre)
+    for (int j = a0; j < a1; ++j) my_acc += ws.accf[j];
+  for (int jb = a0; !pre && jb < a1; jb += QB) {
     int cs[QB], mv[QB];
     float ev[QB], ecv[QB], btv[QB], muv[QB];
 #pragma unroll
@@ -513,19 +558,12 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
       bool acc = false;
       float mmu = 0.f;
       if (c >= 0) {
-        const double e = (double)ev[u] - (double)ecv[u];
-        const double beta = (double)btv[u];
         mmu = (float)(-mv[u]) * muv[u];  // (m mu) after a flip
-        // exp(-switch_energy*beta) with switch_energy = -2E, relative error ~3e-7 (FP64 exponent
-        // split, FP32 mantissa), clipped to [1e-10, 1e10]
-        double p = hs_scaled_exp(1.f, 2.0 * e * beta, 1.f, 1.0);
-        p = fmin(fmax(p, 1e-10), 1e10);
         if (SEL.metropolis_time) {
           const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
-          tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * beta);
+          tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * (double)btv[u]);
         }
-        const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
-        acc = ua < p;
+        acc = metro_accepts(I, step, c, ev[u], ecv[u], btv[u]);
       }
       if (single) {
         if (tid == 0) {
@@ -922,6 +960,59 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
 
 __global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J, WorkList WL, int par) {
   metro_select<SELECT_BATCH>(J, WL, par, 0);
+}
+
+// Grid selection of large instances, first half: attempt 0's samples and their acceptance, one
+// sample per thread over many workgroups (the single-workgroup selection would walk them 3+ per
+// thread through chains of dependent loads). Writes cand / accf / cmmu of every sample and the
+// workgroup's counts; k_metro_select_pre lists them (or redoes the step when attempt 0 has no
+// valid sample). Same draws and arithmetic as metro_select.
+__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const hs_run_t &R = J.R;
+  const MetroWS &ws = J.ws;
+  __shared__ long long s_l[PRESAMPLE_THREADS / 64];
+  const hs_state_t S = J.st[0];
+  const double n = (double)I.n_occ;
+  const bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
+                  hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  if (!go) return;
+  const uint64_t step = (uint64_t)S.step;
+  int ox, oy;
+  grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
+  const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
+  const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
+  const int j = blockIdx.x * PRESAMPLE_THREADS + threadIdx.x;
+  int inb = 0, val = 0;
+  if (j < SEL.nsc) {
+    const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u, (uint64_t)j), 0) * cnt);
+    bool ok;
+    const int cell = grid_cell(I, SEL, ox, oy, j, SEL.pat_cells[p0 + pick], ok);
+    const bool v = ok && (I.flags[cell] & HS_FLAG_OCC);
+    const int c = v ? cell : -1;
+    inb = ok;
+    val = v;
+    bool acc = false;
+    float mmu = 0.f;
+    if (c >= 0) {
+      const float E = I.E[c], Ec = I.Ec[c], beta = I.beta[c];
+      const int mc = (int)I.m[c];
+      const float muv = I.pair_src_unit ? 1.f : I.mu[c];
+      mmu = (float)(-mc) * muv;
+      acc = metro_accepts(I, step, c, E, Ec, beta);
+    }
+    ws.cand[j] = c;
+    ws.accf[j] = acc ? 1 : 0;
+    ws.cmmu[j] = mmu;
+  }
+  const long long both = block_sum_i64(((long long)inb << 32) + val, s_l);
+  if (threadIdx.x == 0) ws.part[blockIdx.x] = both;
+}
+
+// Second half: metro_select on the presampled attempt (listing, tiles, state), one workgroup.
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_pre(MetroJob J, WorkList WL, int par) {
+  metro_select<SELECT_BATCH, true>(J, WL, par, 0);
 }
 
 // One selection workgroup per job of the list `which`.
@@ -1485,10 +1576,20 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   k_metro_clear<<<1, 64, 0, s>>>(WL);
   HS_LAUNCH_CHECK("k_metro_clear");
   const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
+  // grid selection without METROPOLIS_TIME: the samples are drawn and accepted by a wide launch
+  const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time;
+  const unsigned pblocks = (unsigned)((J.SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS);
   for (int64_t j = 0; j < run->max_steps; ++j) {
     const int par = (int)(j & 1);
-    k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J, WL, par);
-    HS_LAUNCH_CHECK("k_metro_select");
+    if (presample) {
+      k_metro_presample<<<pblocks, PRESAMPLE_THREADS, 0, s>>>(J);
+      HS_LAUNCH_CHECK("k_metro_presample");
+      k_metro_select_pre<<<1, MS_THREADS, 0, s>>>(J, WL, par);
+      HS_LAUNCH_CHECK("k_metro_select_pre");
+    } else {
+      k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J, WL, par);
+      HS_LAUNCH_CHECK("k_metro_select");
+    }
     k_metro_update_one<<<wgs, MU_THREADS, 0, s>>>(J, WL, par);
     HS_LAUNCH_CHECK("k_metro_update");
   }
