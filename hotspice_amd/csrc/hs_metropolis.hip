@@ -49,6 +49,8 @@ constexpr int MARK_WORDS = 4096;
 constexpr int SELECT_BATCH = 2;
 // workgroup size of the presampling kernel (one sample per thread)
 constexpr int PRESAMPLE_THREADS = 256;
+// presampled candidates per selection thread kept in registers (up to 4096 samples)
+constexpr int PRESELECT_BATCH = 4;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -414,7 +416,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   const int nx = I.nx, ny = I.ny;
   // all of this thread's grid samples fit one batch: they stay in registers (the Poisson scheme
   // writes its candidates to the workspace only)
-  const bool regs = !PRE && q <= QB && SEL.scheme == 0;
+  const bool regs = q <= QB && SEL.scheme == 0;
   int candr[QB], accr[QB];
   float mmur[QB];
 #pragma unroll
@@ -433,6 +435,17 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   if (PRE) {
     long long mine = 0;
     for (int b = tid; b < (nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; b += MS_THREADS) mine += ws.part[b];
+    if (regs) {  // this thread's presampled candidates, loaded together with the counts
+#pragma unroll
+      for (int u = 0; u < QB; ++u) {
+        const int j = j0 + u;
+        if (j < j1) {
+          candr[u] = ws.cand[j];
+          accr[u] = ws.accf[j];
+          mmur[u] = ws.cmmu[j];
+        }
+      }
+    }
     const long long both = block_sum_i64(mine, s_l);
     const int tot_inb = (int)(both >> 32), tot_val = (int)(both & 0xffffffffll);
     pre = tot_inb > 0 && tot_val > 0;
@@ -531,8 +544,14 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
   int my_acc = 0;
   double tsum = 0.0;
   const int a0 = single ? (tid == 0 ? 0 : 1) : j0, a1 = single ? 1 : j1;
-  if (pre)
-    for (int j = a0; j < a1; ++j) my_acc += ws.accf[j];
+  if (pre) {
+    if (regs) {
+#pragma unroll
+      for (int u = 0; u < QB; ++u) my_acc += j0 + u < j1 ? accr[u] : 0;
+    } else {
+      for (int j = a0; j < a1; ++j) my_acc += ws.accf[j];
+    }
+  }
   for (int jb = a0; !pre && jb < a1; jb += QB) {
     int cs[QB], mv[QB];
     float ev[QB], ecv[QB], btv[QB], muv[QB];
@@ -619,7 +638,9 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
     // truncates per sublattice above the cutoff when REDUCED_KERNEL_SIZE != 0 (energies.py:343-349)
     ws.trunc[tid] = (total > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
   }
-  if (WL.wl && total > 0) {
+  // the presampled grid scheme without PBC leaves the tiles to k_metro_tiles (many workgroups)
+  const bool defer_tiles = PRE && !single && !I.pbc;
+  if (WL.wl && total > 0 && !defer_tiles) {
     // the tiles this step changes: the windows metro_update adds (dipolar: stored or truncated
     // window, wrapped when truncated with PBC; exchange: the NN stencil, wrapped with PBC) and the
     // tile of every switch (its flip), appended once each to the worklist
@@ -1010,9 +1031,61 @@ __global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob 
   if (threadIdx.x == 0) ws.part[blockIdx.x] = both;
 }
 
-// Second half: metro_select on the presampled attempt (listing, tiles, state), one workgroup.
+// Second half: metro_select on the presampled attempt (listing, truncation flags, state), one
+// workgroup.
 __global__ void __launch_bounds__(MS_THREADS) k_metro_select_pre(MetroJob J, WorkList WL, int par) {
-  metro_select<SELECT_BATCH, true>(J, WL, par, 0);
+  metro_select<PRESELECT_BATCH, true>(J, WL, par, 0);
+}
+
+// The worklist of a presampled grid step without PBC: one thread per update tile decides whether
+// a switch reaches it, from the chosen supercells whose samples can (the update's direct lookup),
+// with the update's reach rules (dipolar window, truncated per site; exchange stencil; the tile of
+// the switch itself). Reached tiles are appended to the worklist, one atomic per wave (the order
+// of the entries does not matter: every tile's update is independent of the others).
+__global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, int par) {
+  const hs_instance_t &I = J.I;
+  const hs_select_t &SEL = J.SEL;
+  const MetroWS &ws = J.ws;
+  const int a = *ws.acc_n;
+  if (a == 0 || ws.info[2] != 0) return;  // nothing switched, or the single-sample step (listed already)
+  const int ox = ws.info[0], oy = ws.info[1];
+  const int nx = I.nx, ny = I.ny, tiles_x = J.tiles_x, ntile = J.tiles_x * J.tiles_y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool touched = false;
+  if (t < ntile) {
+    const int x0 = (t % tiles_x) * MU_TX, y0 = (t / tiles_x) * MU_TY;
+    const int x1 = min(nx - 1, x0 + MU_TX - 1), y1 = min(ny - 1, y0 + MU_TY - 1);
+    const bool has_dip = I.dip_pf != 0.0, has_ex = I.J != 0.0 && I.nn_w > 0;
+    const int hy = I.nn_h / 2, hx = I.nn_w / 2;
+    const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
+    const int Wx = max(has_dip ? I.kwx : 0, has_ex ? hx : 0), Wy = max(has_dip ? I.kwy : 0, has_ex ? hy : 0);
+    const int nrows = SEL.nsc / SEL.ncols;
+    const int cs0 = max(0, -floor_div(-(x0 - Wx - ox - SEL.Rx + 1), 2 * SEL.Rx));
+    const int cs1 = min(SEL.ncols - 1, floor_div(x0 + MU_TX - 1 + Wx - ox, 2 * SEL.Rx));
+    const int rs0 = max(0, -floor_div(-(y0 - Wy - oy - SEL.Ry + 1), 2 * SEL.Ry));
+    const int rs1 = min(nrows - 1, floor_div(y0 + MU_TY - 1 + Wy - oy, 2 * SEL.Ry));
+    for (int rs = rs0; rs <= rs1 && !touched; ++rs)
+      for (int cs = cs0; cs <= cs1 && !touched; ++cs) {
+        const int j = rs * SEL.ncols + cs;
+        if (!ws.accf[j]) continue;
+        const int sc = ws.cand[j], sy = sc / nx, sx = sc % nx, ss = hs_site(I, sy, sx);
+        bool rel = sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1;  // the switch's own tile (its flip)
+        if (!rel && has_dip && I.sub_of[ss] >= 0) {
+          const bool tr = ws.trunc[ss] != 0;
+          rel = reaches(sx, tr ? tkx : I.kwx, nx, false, x0, x0 + MU_TX - 1) &&
+                reaches(sy, tr ? tky : I.kwy, ny, false, y0, y0 + MU_TY - 1);
+        }
+        if (!rel && has_ex)
+          rel = reaches(sx, hx, nx, false, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, false, y0, y0 + MU_TY - 1);
+        touched = rel;
+      }
+  }
+  const unsigned long long ball = __ballot(touched);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && ball) base = atomicAdd(&WL.count[par], __popcll(ball));
+  base = __shfl(base, 0, 64);
+  if (touched) WL.wl[base + __popcll(ball & ((1ull << lane) - 1ull))] = make_int2(0, t);
 }
 
 // One selection workgroup per job of the list `which`.
@@ -1586,6 +1659,10 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
       HS_LAUNCH_CHECK("k_metro_presample");
       k_metro_select_pre<<<1, MS_THREADS, 0, s>>>(J, WL, par);
       HS_LAUNCH_CHECK("k_metro_select_pre");
+      if (!inst->pbc) {
+        k_metro_tiles<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(J, WL, par);
+        HS_LAUNCH_CHECK("k_metro_tiles");
+      }
     } else {
       k_metro_select_one<<<1, MS_THREADS, 0, s>>>(J, WL, par);
       HS_LAUNCH_CHECK("k_metro_select");
