@@ -38,8 +38,10 @@ constexpr int MU_TX = 32, MU_TY = 8, MU_THREADS = MU_TX * MU_TY;
 constexpr int MAX_ATTEMPTS = 4096;
 // instances up to this many grid cells step in the persistent one-workgroup kernel
 constexpr int PERSIST_CELLS = 80 * 80;
-// workgroups of an update launch over the worklist (8 per CU of a 256-CU device)
+// workgroups of an update launch over the worklist (8 per CU of a 256-CU device), and the waves
+// per SIMD the update kernels are compiled for, so that all of them are resident at once
 constexpr int UPDATE_WGS = 2048;
+constexpr int UPDATE_WAVES = 8;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
 // the selection marks tiles in an LDS bitmap of this many 32-bit words (jobs with more tiles list
@@ -1098,7 +1100,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob
 
 // The update of the step's listed tiles (one tile at a time per workgroup); then the counter of
 // the next step is cleared.
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J, WorkList WL, int par) {
+__global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_one(MetroJob J, WorkList WL, int par) {
   const int n = WL.count[par];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int t = WL.wl[i].y;
@@ -1108,7 +1110,7 @@ __global__ void __launch_bounds__(MU_THREADS) k_metro_update_one(MetroJob J, Wor
   if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
 }
 
-__global__ void __launch_bounds__(MU_THREADS) k_metro_update_many(const MetroJob *__restrict__ jobs, WorkList WL,
+__global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_many(const MetroJob *__restrict__ jobs, WorkList WL,
                                                                   int par) {
   const int n = WL.count[par];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
