@@ -247,7 +247,9 @@ def bench_metro(args, large):
     line = base_line(args, attempts / wall, steps, 0, wall, "f32+kahan (fields), f64 (acceptance)",
                      {"workload": desc, "attempts": attempts, "accepted": accepted, "device_steps": nsteps,
                       "rel_E_err_vs_fp64": err, "window_cells_per_accept": nt_per_acc, "parallelism": "single GPU"})
-    line["roofline"] = roofline("k_metro_select_one + k_metro_update_one (per progress call)", launches, bytes_total)
+    kname = ("k_metro_presample + k_metro_select_pre + k_metro_tiles + k_metro_update_one (per progress call)" if large
+             else "k_metro_persist_one (per progress call)")
+    line["roofline"] = roofline(kname, launches, bytes_total)
     if not args.no_cpu:
         if large:
             line["cpu_baseline"] = None
