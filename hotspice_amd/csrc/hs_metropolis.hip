@@ -53,6 +53,9 @@ constexpr int SELECT_BATCH = 2;
 constexpr int PRESAMPLE_THREADS = 256;
 // presampled candidates per selection thread kept in registers (up to 4096 samples)
 constexpr int PRESELECT_BATCH = 4;
+// steps with at most this many samples (one register batch per selection thread) select in one
+// workgroup: below it the extra launches of the presampled step cost more than they save
+constexpr int PRESAMPLE_MIN = SELECT_BATCH * MS_THREADS;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
@@ -990,7 +993,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J, Wor
 // thread through chains of dependent loads). Writes cand / accf / cmmu of every sample and the
 // workgroup's counts; k_metro_select_pre lists them (or redoes the step when attempt 0 has no
 // valid sample). Same draws and arithmetic as metro_select.
-__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J) {
+__device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const hs_run_t &R = J.R;
@@ -1006,7 +1009,7 @@ __global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob 
   grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
   const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
   const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
-  const int j = blockIdx.x * PRESAMPLE_THREADS + threadIdx.x;
+  const int j = blk * PRESAMPLE_THREADS + threadIdx.x;
   int inb = 0, val = 0;
   if (j < SEL.nsc) {
     const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u, (uint64_t)j), 0) * cnt);
@@ -1030,7 +1033,16 @@ __global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob 
     ws.cmmu[j] = mmu;
   }
   const long long both = block_sum_i64(((long long)inb << 32) + val, s_l);
-  if (threadIdx.x == 0) ws.part[blockIdx.x] = both;
+  if (threadIdx.x == 0) ws.part[blk] = both;
+}
+
+__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J) { metro_presample(J, blockIdx.x); }
+
+// A batch: block b presamples block map[b].y of job map[b].x.
+__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample_many(const MetroJob *__restrict__ jobs,
+                                                                          const int2 *__restrict__ map) {
+  const int2 e = map[blockIdx.x];
+  metro_presample(jobs[e.x], e.y);
 }
 
 // Second half: metro_select on the presampled attempt (listing, truncation flags, state), one
@@ -1039,20 +1051,26 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_pre(MetroJob J, Wor
   metro_select<PRESELECT_BATCH, true>(J, WL, par, 0);
 }
 
+__global__ void __launch_bounds__(MS_THREADS) k_metro_select_pre_many(const MetroJob *__restrict__ jobs,
+                                                                      const int32_t *__restrict__ which, WorkList WL,
+                                                                      int par) {
+  const int j = which[blockIdx.x];
+  metro_select<PRESELECT_BATCH, true>(jobs[j], WL, par, j);
+}
+
 // The worklist of a presampled grid step without PBC: one thread per update tile decides whether
 // a switch reaches it, from the chosen supercells whose samples can (the update's direct lookup),
 // with the update's reach rules (dipolar window, truncated per site; exchange stencil; the tile of
 // the switch itself). Reached tiles are appended to the worklist, one atomic per wave (the order
 // of the entries does not matter: every tile's update is independent of the others).
-__global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, int par) {
+__device__ __forceinline__ void metro_tiles(const MetroJob &J, WorkList WL, int par, int job, int t) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const MetroWS &ws = J.ws;
   const int a = *ws.acc_n;
-  if (a == 0 || ws.info[2] != 0) return;  // nothing switched, or the single-sample step (listed already)
+  if (a == 0 || ws.info[2] != 0 || I.pbc) return;  // nothing switched, the single-sample step or PBC (listed already)
   const int ox = ws.info[0], oy = ws.info[1];
   const int nx = I.nx, ny = I.ny, tiles_x = J.tiles_x, ntile = J.tiles_x * J.tiles_y;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   bool touched = false;
   if (t < ntile) {
     const int x0 = (t % tiles_x) * MU_TX, y0 = (t / tiles_x) * MU_TY;
@@ -1087,7 +1105,18 @@ __global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, in
   int base = 0;
   if (lane == 0 && ball) base = atomicAdd(&WL.count[par], __popcll(ball));
   base = __shfl(base, 0, 64);
-  if (touched) WL.wl[base + __popcll(ball & ((1ull << lane) - 1ull))] = make_int2(0, t);
+  if (touched) WL.wl[base + __popcll(ball & ((1ull << lane) - 1ull))] = make_int2(job, t);
+}
+
+__global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, int par) {
+  metro_tiles(J, WL, par, 0, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// A batch: block b covers tiles 256*map[b].y .. +255 of job map[b].x.
+__global__ void __launch_bounds__(256) k_metro_tiles_many(const MetroJob *__restrict__ jobs, const int2 *__restrict__ map,
+                                                          WorkList WL, int par) {
+  const int2 e = map[blockIdx.x];
+  metro_tiles(jobs[e.x], WL, par, e.x, e.y * 256 + threadIdx.x);
 }
 
 // One selection workgroup per job of the list `which`.
@@ -1651,8 +1680,9 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   k_metro_clear<<<1, 64, 0, s>>>(WL);
   HS_LAUNCH_CHECK("k_metro_clear");
   const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
-  // grid selection without METROPOLIS_TIME: the samples are drawn and accepted by a wide launch
-  const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time;
+  // grid selection without METROPOLIS_TIME and with more samples than one batch per selection
+  // thread: the samples are drawn and accepted by a wide launch
+  const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time && J.SEL.nsc > PRESAMPLE_MIN;
   const unsigned pblocks = (unsigned)((J.SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS);
   for (int64_t j = 0; j < run->max_steps; ++j) {
     const int par = (int)(j & 1);
@@ -1678,10 +1708,12 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
 namespace {
 
 struct MetroLayout {
-  size_t jobs_off, small_off, large_off, wl_off, total;
+  size_t jobs_off, small_off, large_off, pmap_off, tmap_off, wl_off, total;
   std::vector<size_t> ws_off;
   std::vector<int32_t> small, large;
   int64_t ntiles;  // tiles of the two-launch (large) jobs
+  bool pre;        // every large job is a grid selection without METROPOLIS_TIME: presampled steps
+  std::vector<int2> pmap, tmap;  // presample blocks (job, block) and tile blocks (job, block) of the large jobs
 };
 
 bool persistent(const hs_instance_t &in) { return in.nx * in.ny <= PERSIST_CELLS; }
@@ -1704,9 +1736,27 @@ void metro_layout(const hs_team_t *teams, int n, MetroLayout &L) {
     L.large.push_back(i);
     L.ntiles += tiles_of(*teams[i].inst);
   }
+  L.pre = !L.large.empty();
+  int max_nsc = 0;
+  for (int i : L.large) {
+    L.pre = L.pre && teams[i].sel->scheme == 0 && !teams[i].sel->metropolis_time;
+    max_nsc = std::max(max_nsc, (int)teams[i].sel->nsc);
+  }
+  L.pre = L.pre && max_nsc > PRESAMPLE_MIN;
+  L.pmap.clear();
+  L.tmap.clear();
+  if (L.pre)
+    for (int i : L.large) {
+      for (int b = 0; b < (teams[i].sel->nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; ++b)
+        L.pmap.push_back(make_int2(i, b));
+      if (!teams[i].inst->pbc)
+        for (int b = 0; b < (tiles_of(*teams[i].inst) + 255) / 256; ++b) L.tmap.push_back(make_int2(i, b));
+    }
   L.small_off = align256(sizeof(MetroJob) * (size_t)n);
   L.large_off = L.small_off + align256(4 * (size_t)n);
-  L.wl_off = L.large_off + align256(4 * (size_t)n);
+  L.pmap_off = L.large_off + align256(4 * (size_t)n);
+  L.tmap_off = L.pmap_off + align256(8 * (L.pmap.size() + 1));
+  L.wl_off = L.tmap_off + align256(8 * (L.tmap.size() + 1));
   size_t off = L.wl_off + wl_bytes(L.ntiles);
   L.ws_off.resize(n);
   for (int i = 0; i < n; ++i) {
@@ -1760,6 +1810,10 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   for (size_t k = 0; k < L.small.size(); ++k) small[k] = L.small[k];
   int32_t *large = (int32_t *)(host.data() + L.large_off);
   for (size_t k = 0; k < L.large.size(); ++k) large[k] = L.large[k];
+  int2 *pmap = (int2 *)(host.data() + L.pmap_off);
+  for (size_t k = 0; k < L.pmap.size(); ++k) pmap[k] = L.pmap[k];
+  int2 *tmap = (int2 *)(host.data() + L.tmap_off);
+  for (size_t k = 0; k < L.tmap.size(); ++k) tmap[k] = L.tmap[k];
   hipStream_t s = (hipStream_t)stream;
   HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
@@ -1777,11 +1831,22 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
     k_metro_clear<<<1, 64, 0, s>>>(WL);
     HS_LAUNCH_CHECK("k_metro_clear");
     const unsigned wgs = (unsigned)(L.ntiles < UPDATE_WGS ? L.ntiles : UPDATE_WGS);
+    const int32_t *lj = (const int32_t *)(base + L.large_off);
     for (int64_t j = 0; j < steps; ++j) {
       const int par = (int)(j & 1);
-      k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, (const int32_t *)(base + L.large_off),
-                                                                            WL, par);
-      HS_LAUNCH_CHECK("k_metro_select_many");
+      if (L.pre) {  // as hs_metropolis_run's large steps, for every job of the batch
+        k_metro_presample_many<<<(unsigned)L.pmap.size(), PRESAMPLE_THREADS, 0, s>>>(dj, (const int2 *)(base + L.pmap_off));
+        HS_LAUNCH_CHECK("k_metro_presample_many");
+        k_metro_select_pre_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, lj, WL, par);
+        HS_LAUNCH_CHECK("k_metro_select_pre_many");
+        if (!L.tmap.empty()) {
+          k_metro_tiles_many<<<(unsigned)L.tmap.size(), 256, 0, s>>>(dj, (const int2 *)(base + L.tmap_off), WL, par);
+          HS_LAUNCH_CHECK("k_metro_tiles_many");
+        }
+      } else {
+        k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, lj, WL, par);
+        HS_LAUNCH_CHECK("k_metro_select_many");
+      }
       k_metro_update_many<<<wgs, MU_THREADS, 0, s>>>(dj, WL, par);
       HS_LAUNCH_CHECK("k_metro_update_many");
     }

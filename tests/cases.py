@@ -72,6 +72,19 @@ SYNTH = {
                                 params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
                                             SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=2),
                                 kw=dict(PBC=True), update_kw=dict(r=2e-6), nsteps=120, seed=36),
+    # more samples than 2 per selection thread: the presampled step (wide sampling launch; tiles
+    # found by a wide launch without PBC, by the selection workgroup with PBC)
+    "metro_oop_presample": dict(lattice="OOP_Square", a=1e-6, n=200, T=900, moment=1.1e-15,
+                                energies=[("dipolar", dict(cutoff_range=3e-6)), ("exchange", dict(J_kT=0.1))],
+                                params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=3,
+                                            SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
+                                update_kw=dict(r=2e-6), nsteps=40, seed=37),
+    "metro_ipsq_presample_pbc": dict(lattice="IP_Square", a=1e-6, n=192, T=300, moment=2.37e-16,
+                                     energies=[("dipolar", dict(cutoff_range=2e-6)),
+                                               ("zeeman", dict(magnitude=1e-3, angle=0.4))],
+                                     params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=4,
+                                                 SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=0),
+                                     kw=dict(PBC=True), update_kw=dict(r=1e-6), nsteps=30, seed=38),
     "metro_ipsq_window": dict(lattice="IP_Square", a=1e-6, n=48, T=400, moment=2.37e-16,
                               energies=[("dipolar", dict(cutoff_range=3e-6)), ("zeeman", dict(magnitude=1e-3, angle=0.7))],
                               params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=6,

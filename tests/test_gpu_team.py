@@ -137,3 +137,21 @@ def test_autotuner_keeps_state_and_results():
     a.progress(t_max=math.inf, MCsteps_max=0.5)
     b.progress(t_max=math.inf, MCsteps_max=0.5)
     _same(_snapshot(a), _snapshot(b))
+
+
+def test_presampled_metropolis_batch_equals_alone():
+    """A batch with instances of more than 2048 samples per step takes the presampled steps for
+    all its instances (wide sampling and tile launches); every instance still ends bit-identical
+    to stepping it alone."""
+    from cases import spec
+    from hotspice_amd import batch
+    names = ["metro_oop_presample", "metro_ipsq_presample_pbc", "metro_oop_window"]
+    kws = [spec(n)["update_kw"] for n in names]
+    alone = [_metro(n) for n in names]
+    for m, kw in zip(alone, kws):
+        m.progress(t_max=math.inf, MCsteps_max=0.5, **kw)
+    mixed = [_metro(n) for n in names]
+    res = batch.progress_many(mixed, t_max=math.inf, MCsteps_max=0.5, per_instance=kws)
+    for m1, m2, r in zip(alone, mixed, res):
+        _same(_snapshot(m1), _snapshot(m2))
+        assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
