@@ -381,7 +381,7 @@ HS_INLINE bool metro_accepts(const hs_instance_t &I, uint64_t step, int c, float
 }
 
 template <int QB, bool PRE = false>
-__device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
+__device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
                                              int job = 0) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
@@ -412,7 +412,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
         st[0] = S;
       }
     }
-    return false;
+    return -1;
   }
   const uint64_t step = (uint64_t)S.step;
   const int nsc = SEL.nsc;
@@ -543,7 +543,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
       st[0] = S;
       *ws.acc_n = 0;
     }
-    return false;
+    return -1;
   }
   // acceptance (core.py:662-671); with a single sample thread 0 plays candidate 0
   int my_acc = 0;
@@ -714,7 +714,7 @@ __device__ __forceinline__ bool metro_select(const MetroJob &J, WorkList WL = Wo
     if (SEL.metropolis_time) S.t += tsum;
     st[0] = S;
   }
-  return true;
+  return total;
 }
 
 // Displacement candidates d = i - s (+k*n) with |d| <= w: wrap enumerates images, fill only k = 0.
@@ -1163,9 +1163,10 @@ __global__ void k_metro_clear(WorkList WL) {
 __device__ __forceinline__ void metro_persist(const MetroJob &J) {
   const int tx = (J.I.nx + 31) / 32, ty = (J.I.ny + 31) / 32;
   for (int64_t s = 0; s < J.R.max_steps; ++s) {
-    const bool stepped = metro_select<1>(J);
+    const int accepted = metro_select<1>(J);
     __syncthreads();
-    if (!stepped) break;
+    if (accepted < 0) break;
+    if (accepted == 0) continue;  // nothing switched: no field changes this step
     for (int t = 0; t < tx * ty; ++t) metro_update<32, 32, false>(J, t % tx, t / tx);
     __syncthreads();
   }
