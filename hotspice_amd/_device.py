@@ -42,6 +42,7 @@ class DeviceState:
         self.mu = torch.zeros(n, dtype=torch.float32, device=dev)
         self.mu64 = torch.zeros(n, dtype=torch.float64, device=dev)
         self.pw64 = None  # FP64 pair weights when they are not the moments (DiMonopolarEnergy)
+        self.src_ones = None  # source weights of DiMonopolarEnergy (ones)
         self.occ_list = torch.zeros(max(1, n), dtype=torch.int32, device=dev)
         self.state = torch.zeros(2 * _lib.STATE_BYTES, dtype=torch.uint8, device=dev)
         self.partials = torch.zeros(2 * 1024 * 32, dtype=torch.uint8, device=dev)

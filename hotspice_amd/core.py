@@ -587,10 +587,12 @@ class Magnets(ABC):
             q2 = np.ascontiguousarray(pair.charge_sq, dtype=np.float64)
             if d.pw64 is None:
                 d.pw64 = d.torch.empty(self.nx * self.ny, dtype=d.torch.float64, device=d.device)
+                d.src_ones = d.torch.ones(self.nx * self.ny, dtype=d.torch.float32, device=d.device)
             d.upload(d.pw64, q2)
             d.upload(d.mu, q2.astype(np.float32))
         else:
             d.pw64 = None
+            d.src_ones = None
             d.upload(d.mu, self._moment.astype(np.float32))
 
     def _device_build_kernels(self, dip: DipolarEnergy):
@@ -713,6 +715,7 @@ class Magnets(ABC):
                 I.oy_uc[s] = float(self.orientation[y, x, 1])
         dip = self._pair_energy()
         I.pw64 = P(d.pw64) if d.pw64 is not None else P(d.mu64)
+        I.mu_src = P(d.src_ones) if d.src_ones is not None else P(d.mu)
         I.pair_src_unit = int(isinstance(dip, DiMonopolarEnergy))
         if dip is not None and d.kern is not None:
             k = d.kern

@@ -77,7 +77,7 @@ int hs_check_instance(const hs_instance_t *I) {
   if (I->use_perp && (!I->Ep || !I->Epc)) return hs_set_error(HS_ERR_ARG, "use_perp without E_perp arrays");
   if (I->dip_pf != 0.0) {
     if (!I->K32 || !I->K64) return hs_set_error(HS_ERR_ARG, "dipolar energy without kernels");
-    if (!I->pw64) return hs_set_error(HS_ERR_ARG, "pair energy without FP64 weights");
+    if (!I->pw64 || !I->mu_src) return hs_set_error(HS_ERR_ARG, "pair energy without its weights");
     if (I->use_perp && (!I->Kpo32 || !I->Kps64)) return hs_set_error(HS_ERR_ARG, "perp fields without perp kernels");
     if (I->kwx < 0 || I->kwx > I->nx - 1 || I->kwy < 0 || I->kwy > I->ny - 1)
       return hs_set_error(HS_ERR_ARG, "kernel window %d,%d outside grid", I->kwx, I->kwy);

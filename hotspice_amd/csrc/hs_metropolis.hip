@@ -571,7 +571,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
         ecv[u] = I.Ec[cs[u]];
         btv[u] = I.beta[cs[u]];
         mv[u] = (int)I.m[cs[u]];
-        muv[u] = I.pair_src_unit ? 1.f : I.mu[cs[u]];  // source weight (1 for DiMonopolarEnergy)
+        muv[u] = I.mu_src[cs[u]];  // source weight (1 for DiMonopolarEnergy)
       }
     }
 #pragma unroll
@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     if (c >= 0) {
       const float E = I.E[c], Ec = I.Ec[c], beta = I.beta[c];
       const int mc = (int)I.m[c];
-      const float muv = I.pair_src_unit ? 1.f : I.mu[c];
+      const float muv = I.mu_src[c];
       mmu = (float)(-mc) * muv;
       acc = metro_accepts(I, step, c, E, Ec, beta);
     }
@@ -1223,7 +1223,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_relax_select(MetroJob J, int y0,
       if (mc != 0) {
         const double e = (double)I.E[c] - (double)I.Ec[c];  // E of the magnet; switch_energy = -2E
         acc = with_barrier ? ((double)I.EB[c] - e < 0.0) : (-2.0 * e < 0.0);
-        mmu = (float)(-mc) * (I.pair_src_unit ? 1.f : I.mu[c]);  // (m w) after the flip
+        mmu = (float)(-mc) * I.mu_src[c];  // (m w) after the flip
       }
     }
     int total = 0;
@@ -1333,7 +1333,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_flip_select(MetroJob J, const in
   for (int i = tid; i < k; i += MS_THREADS) {
     const int c = cells[i];
     ws.acc_cell[i] = c;
-    ws.acc_mmu[i] = (float)(-(int)I.m[c]) * (I.pair_src_unit ? 1.f : I.mu[c]);
+    ws.acc_mmu[i] = (float)(-(int)I.m[c]) * I.mu_src[c];
     atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
   }
   __syncthreads();
@@ -1443,7 +1443,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_wolff_select(MetroJob J, WolffWS
     const int pos = base + block_exclusive_scan(in, s_i, total);
     if (in) {
       ws.acc_cell[pos] = c;
-      ws.acc_mmu[pos] = (float)(-(int)I.m[c]) * (I.pair_src_unit ? 1.f : I.mu[c]);
+      ws.acc_mmu[pos] = (float)(-(int)I.m[c]) * I.mu_src[c];
       atomicAdd(&s_site[hs_site(I, c / nx, c % nx)], 1);
       if (R.log) J.log_cells[pos] = c;
     }

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
     sitew = hs_site(I, yw, xw);
     nw = I.dip_pf != 0.0 ? I.sub_of[sitew] : -1;
     const double mnew = -(double)mw;
-    A = (float)(2.0 * I.dip_pf * (mnew * (I.pair_src_unit ? 1.0 : (double)I.mu[w])));  // 2*prefactor*(m mu)_w
+    A = (float)(2.0 * I.dip_pf * (mnew * (double)I.mu_src[w]));  // 2*prefactor*(m w)_w
     Ax = (float)(-2.0 * I.J * mnew);                          // exchange: -J m_i * (2 m_w)
     oxw = I.ox_uc[sitew];
     oyw = I.oy_uc[sitew];

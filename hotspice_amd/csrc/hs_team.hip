@@ -130,9 +130,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   const float mpf = (float)I.m_perp_factor;
   const double mpf64 = I.m_perp_factor, dip_pf = I.dip_pf, Jx = I.J;
   const int barrier = I.barrier, kwx = I.kwx, kwy = I.kwy, ny = I.ny, pbc = I.pbc, in_plane = I.in_plane;
-  const int src_unit = I.pair_src_unit;
   const int nn_h = I.nn_h, nn_w = I.nn_w;
-  const float *const K32 = I.K32, *const Kpo32 = I.Kpo32, *const mug = I.mu;
+  const float *const K32 = I.K32, *const Kpo32 = I.Kpo32, *const mug = I.mu_src;
   const uint32_t seed_lo = (uint32_t)I.seed, seed_hi = (uint32_t)(I.seed >> 32);
   const uint32_t ux = (uint32_t)I.ux, uy = (uint32_t)I.uy;
   const uint64_t Mux = ((1ull << 32) + ux - 1) / ux, Muy = ((1ull << 32) + uy - 1) / uy;
@@ -374,7 +373,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     const int w = yw * nx + xw, sitew = site_of(yw, xw);
     const int nw = dip_pf != 0.0 ? I.sub_of[sitew] : -1;
     const double mnew = -(double)mw;
-    const float A = (float)(2.0 * dip_pf * (mnew * (src_unit ? 1.0 : (double)mug[w])));
+    const float A = (float)(2.0 * dip_pf * (mnew * (double)mug[w]));
     const float Ax = (float)(-2.0 * Jx * mnew);
     const float oxw = (float)I.ox_uc[sitew], oyw = (float)I.oy_uc[sitew];
     const float *Kw = nw >= 0 ? K32 + (size_t)nw * KS : nullptr;

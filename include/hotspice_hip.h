@@ -55,6 +55,8 @@ typedef struct hs_instance {
   const float *beta;    /* (ny,nx) 1/(kB T) */
   const float *mu;      /* (ny,nx) pair weight of each magnet, FP32 (stepping): its moment for
                            DipolarEnergy, its squared charge (moment/d)^2 for DiMonopolarEnergy */
+  const float *mu_src;  /* (ny,nx) weight of a switching magnet in the pair terms it sends, FP32:
+                           == mu for DipolarEnergy, ones for DiMonopolarEnergy */
   const double *mu64;   /* (ny,nx) moment, FP64 (Zeeman, totals) */
   const double *pw64;   /* (ny,nx) pair weight, FP64 (recompute); == mu64 for DipolarEnergy */
   const float *K32;     /* [n_sub][2kwy+1][2kwx+1] dipolar kernel, FP32 (stepping) */
@@ -82,7 +84,8 @@ typedef struct hs_instance {
   int32_t has_zeeman;
   int32_t uniform_beta; /* 1 when T is uniform over the grid */
   int32_t pair_src_unit; /* 0: a pair term is prefactor*K*(m w)_source*(m w)_receiver (DipolarEnergy);
-                            1: prefactor*K*m_source*(m w)_receiver (DiMonopolarEnergy), w = mu/pw64 */
+                            1: prefactor*K*m_source*(m w)_receiver (DiMonopolarEnergy), w = mu/pw64;
+                            the stepping kernels read the source weight from mu_src */
   int32_t sub_of[HS_MAX_UC];    /* unit-cell site (y%uy)*ux + x%ux -> kernel index or -1 */
   double ox_uc[HS_MAX_UC];      /* in-plane orientation per unit-cell site */
   double oy_uc[HS_MAX_UC];
