@@ -82,12 +82,15 @@ def neel_team_size(m, nblocks=0):
     return max(1, min(cus, -(-m.n // 512)))
 
 
-def launch_neel(items, attempt_freq=1e10, begin=False):
+def launch_neel(items, attempt_freq=1e10, begin=False, full=None):
     """ONE persistent launch stepping every (mm, max_steps, t_max, mc_max, nblocks) item. With
     begin=True the launch starts each instance's progress call (its origin t0, attempts0 := the
     current t, attempts); otherwise the origin must have been set. Afterwards checks each
     instance's error word, counts its steps towards the periodic FP64 recompute and runs that
-    when due."""
+    when due. `full[i]`: item i is known to run all its max_steps (no stop predicate can end it
+    earlier); when those steps complete its recompute period, the recompute is queued right
+    behind the launch instead of after reading the states back (the same recompute at the same
+    point of the trajectory)."""
     if not items:
         return
     mags = [it[0] for it in items]
@@ -98,7 +101,13 @@ def launch_neel(items, attempt_freq=1e10, begin=False):
     teams = _team_array(mags, [it[1] for it in items], [it[2] for it in items], [it[3] for it in items],
                         attempt_freq, begin, nblocks=[it[4] for it in items])
     dev.run_teams(teams, "neel")
-    return _after_launch(mags, "Neel")
+    queued = []
+    if full is not None:
+        queued = [m for (m, steps, *_), f in zip(items, full)
+                  if f and m.recompute_every and m._since_recompute + steps >= m.recompute_every]
+        if queued:
+            dev.recompute_many([m._dev for m in queued], [3 if m.USE_PERP_ENERGY else 1 for m in queued])
+    return _after_launch(mags, "Neel", recomputed={id(m) for m in queued})
 
 
 _TEAM_DTYPE = np.dtype(_lib.Team)
@@ -137,8 +146,9 @@ def launch_metropolis(items, attempt_freq=1e10, begin=False):
     return _after_launch(mags, "Metropolis", before)
 
 
-def _after_launch(mags, what, before=None):
-    """Checks errors, records samples per step and runs due recomputes; returns the states."""
+def _after_launch(mags, what, before=None, recomputed=()):
+    """Checks errors, records samples per step and runs due recomputes (except those already
+    queued behind the launch, `recomputed`: ids of instances); returns the states."""
     sts = read_states(mags)
     due = []
     for i, (m, st) in enumerate(zip(mags, sts)):
@@ -147,6 +157,9 @@ def _after_launch(mags, what, before=None):
         if before is not None and st.steps_done > 0:  # samples per step, to size the next launch
             m._metro_rate = max(1.0, (st.attempts - before[i]) / st.steps_done)
         m._since_recompute += int(st.steps_done)
+        if id(m) in recomputed:
+            m._since_recompute = 0
+            continue
         if m.recompute_every and m._since_recompute >= m.recompute_every and m._incremental_exact():
             due.append(m)
     if due:  # the periodic FP64 recomputes of all instances due, in one call
@@ -222,8 +235,14 @@ def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk, sts):
         steps = np.where(ev > 0, np.minimum(steps, np.maximum(1, ev - since)), steps)
         steps = np.maximum(1, steps)
         items = [(mags[i], int(k), t_max, mc, nbl[i]) for i, k in zip(idx.tolist(), steps.tolist())]
+        # an instance runs all its steps when t_max is infinite and its MC budget lasts past them
+        if math.isinf(t_max) and t_max > 0:
+            left = np.ceil(mc * n[idx] - (att[idx] - a0[idx])) if math.isfinite(mc) else np.full(idx.size, np.inf)
+            full = (steps <= left).tolist()
+        else:
+            full = None
         # the first launch opens every pending instance's progress call (origin = starts)
-        after = launch_neel(items, f0, begin=first)
+        after = launch_neel(items, f0, begin=first, full=full)
         first = False
         for i, st in zip(idx.tolist(), after):
             sts[i] = st
