@@ -53,9 +53,9 @@ constexpr int SELECT_BATCH = 2;
 constexpr int PRESAMPLE_THREADS = 256;
 // presampled candidates per selection thread kept in registers (up to 4096 samples)
 constexpr int PRESELECT_BATCH = 4;
-// steps with at most this many samples (one register batch per selection thread) select in one
-// workgroup: below it the extra launches of the presampled step cost more than they save
-constexpr int PRESAMPLE_MIN = SELECT_BATCH * MS_THREADS;
+// steps with at most this many samples (one per selection thread) select in one workgroup: below
+// it the extra launches of the presampled step cost more than they save
+constexpr int PRESAMPLE_MIN = MS_THREADS;
 
 struct MetroWS {
   int32_t *cand;     // [nsc] candidate cell, -1 if not a valid sample
