@@ -72,7 +72,7 @@ SYNTH = {
                                 params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=8,
                                             SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=2),
                                 kw=dict(PBC=True), update_kw=dict(r=2e-6), nsteps=120, seed=36),
-    # more samples than 2 per selection thread: the presampled step (wide sampling launch; tiles
+    # more samples than selection threads: the presampled step (wide sampling launch; tiles
     # found by a wide launch without PBC, by the selection workgroup with PBC)
     "metro_oop_presample": dict(lattice="OOP_Square", a=1e-6, n=200, T=900, moment=1.1e-15,
                                 energies=[("dipolar", dict(cutoff_range=3e-6)), ("exchange", dict(J_kT=0.1))],

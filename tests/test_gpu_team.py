@@ -140,7 +140,7 @@ def test_autotuner_keeps_state_and_results():
 
 
 def test_presampled_metropolis_batch_equals_alone():
-    """A batch with instances of more than 2048 samples per step takes the presampled steps for
+    """A batch with instances of more than 1024 samples per step takes the presampled steps for
     all its instances (wide sampling and tile launches); every instance still ends bit-identical
     to stepping it alone."""
     from cases import spec
