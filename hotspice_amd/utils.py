@@ -52,3 +52,24 @@ def check_repetition(arr, nx: int, ny: int):
 def full_obj_name(obj):
     klass = type(obj)
     return f"{klass.__module__}.{klass.__qualname__}"
+
+
+ELEMENTARY_CHARGE = 1.602176634e-19  # C (exact, SI 2019): joules per electronvolt
+
+
+def eV_to_J(E, /):
+    """utils.py:239-240"""
+    return E * ELEMENTARY_CHARGE
+
+
+def J_to_eV(E, /):
+    """utils.py:241-242"""
+    return E / ELEMENTARY_CHARGE
+
+
+def asnumpy(array) -> np.ndarray:
+    """utils.py:230-237: a NumPy array of `array` (host arrays pass through; torch tensors are
+    copied to the host)."""
+    if hasattr(array, "detach") and hasattr(array, "cpu"):
+        return array.detach().cpu().numpy()
+    return np.asarray(array)

@@ -57,3 +57,12 @@ def test_inputter_transforms_and_bits():
     assert np.array_equal(a, ds.get_next(8)) and a.max() < 32
     c = io.ClockingFieldInputter(io.RandomBinaryDatastream(), angle=0.3, spread=0.1)
     assert c.bit_to_angles(0) == (0.3 + 0.1, 0.3 - 0.1) and c.bit_to_angles(1) == (0.3 + 0.1 + np.pi, 0.3 - 0.1 + np.pi)
+
+
+def test_unit_helpers():
+    from hotspice_amd import utils
+    assert utils.eV_to_J(1.0) == 1.602176634e-19
+    assert utils.J_to_eV(utils.eV_to_J(2.5)) == pytest.approx(2.5, rel=1e-15)
+    import numpy as np
+    a = np.arange(4.0)
+    assert utils.asnumpy(a) is a or np.array_equal(utils.asnumpy(a), a)
