@@ -492,7 +492,8 @@ class OracleMagnets:
         self.orientation = geom.orientation
         self.ux, self.uy = geom.ux, geom.uy
         self.nn = geom.nn
-        self.PBC = bool(PBC)
+        # core.py:361-375: PBC is refused (with a warning) unless whole unit cells tile the grid.
+        self.PBC = bool(PBC) and self.nx % self.ux == 0 and self.ny % self.uy == 0
         self.params = dict(DEFAULT_PARAMS)
         if params:
             self.params.update(params)

@@ -152,3 +152,18 @@ def product_magnets(name, fx=None, **extra):
     cls = getattr(hs.ASI, sp["lattice"])
     return cls(sp["a"], sp["n"], T=sp["T"], E_B=fx["E_B"], moment=sp["moment"], pattern=fx["pattern"],
                energies=ens, params=hs.SimParams(**sp["params"]), seed=sp["seed"], **sp.get("kw", {}), **extra)
+
+
+def energy_scale(om):
+    """Magnitude that energy errors are measured against: the largest |E|, or the size of the
+    individual terms where they cancel. Exchange between perpendicular neighbours (all of the
+    pinwheel's) and a Zeeman field perpendicular to every magnet sum to zero up to the rounding of
+    the orientations, which the FP32 path holds to ~1e-8 of the term, not of the zero."""
+    from oracle import model
+    scale = float(np.abs(om.E).max())
+    for e in om._energies:
+        if isinstance(e, model.Exchange):
+            scale = max(scale, abs(e.J) * float(np.abs(om.nn).sum()))
+        elif isinstance(e, model.Zeeman):
+            scale = max(scale, float(np.max(np.abs(e._magnitude) * np.abs(om.moment))))
+    return scale

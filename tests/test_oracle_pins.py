@@ -202,3 +202,13 @@ def test_cutoff_at_a_lattice_distance(widen, consistent):
         assert err < 1e-12
     else:
         assert err > 1e-5
+
+
+@pytest.mark.parametrize("lat,n,expect", [("IP_Pinwheel", 33, False), ("IP_Pinwheel", 32, True),
+                                          ("OOP_Square", 33, True), ("IP_Square", 31, False)])
+def test_pbc_refused_without_whole_unit_cells(lat, n, expect):
+    """core.py:361-375 keeps PBC off unless nx and ny are multiples of the unit cell (2x2 for the
+    square and pinwheel lattices, 1x1 for OOP_Square)."""
+    om = model.make_magnets(lat, 1e-6, n=n, PBC=True, T=300, moment=1e-16,
+                            pattern=1.0, energies=[model.Exchange(J=1e-21)])
+    assert om.PBC is expect
