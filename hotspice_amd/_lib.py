@@ -34,7 +34,7 @@ class Instance(C.Structure):
         ("mu_src", C.c_void_p), ("mu64", C.c_void_p), ("pw64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
         ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p),
         ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
-        ("J", C.c_double), ("m_perp_factor", C.c_double),
+        ("J", C.c_double), ("m_perp_factor", C.c_double), ("beta64", C.c_double),
         ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),
         ("kwx", C.c_int32), ("kwy", C.c_int32), ("pbc", C.c_int32), ("in_plane", C.c_int32),
         ("use_perp", C.c_int32), ("barrier", C.c_int32), ("nn_w", C.c_int32), ("nn_h", C.c_int32),

@@ -704,6 +704,7 @@ class Magnets(ABC):
         I.barrier = 0 if self.params.ENERGY_BARRIER_METHOD == 'simple' else 1
         I.n_occ = int(np.count_nonzero(np.isfinite(self._occupation_inf)))
         I.uniform_beta = int(getattr(self, "_uniform_beta", True))
+        I.beta64 = float(self.beta.reshape(-1)[0]) if I.uniform_beta else 0.0
         for s in range(_lib.HS_MAX_UC):
             I.sub_of[s] = -1
             I.ox_uc[s] = 0.0

@@ -68,8 +68,9 @@ HS_INLINE float hs_exp1_fast(uint32_t a, uint32_t b) {
 // Natural log with the hardware log2 (keys only: their rounding matters for near-ties alone).
 HS_INLINE float hs_log_fast(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
-// FP64 value with its low 6 mantissa bits cleared (toward zero, relative change < 2^-46), so a
-// waiting time and a 6-bit round tag fit one 64-bit word; every NaN becomes the canonical NaN.
+// FP64 value with its low 6 mantissa bits cleared (toward zero, relative change < 2^-46 for a
+// normal value; for a subnormal one, below 2.2e-308, up to 63 subnormal units), so a waiting
+// time and a 6-bit round tag fit one 64-bit word; every NaN becomes the canonical NaN.
 HS_INLINE double hs_dt_q(double v) {
   if (v != v) return __longlong_as_double(0x7FF8000000000000ll);
   return __longlong_as_double(__double_as_longlong(v) & ~63ll);

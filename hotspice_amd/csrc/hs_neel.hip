@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
       uint32_t wa, wb;
       hs_words(I.seed, (uint64_t)step, 1u, (uint64_t)c, wa, wb);
       const double ep64 = PERP ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
-      p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], (double)I.beta[c], I.m_perp_factor,
+      p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], I.beta64, I.m_perp_factor,
                         I.barrier, PERP, hs_exp1_fast(wa, wb), R.attempt_freq);
       p.mw = I.m[c];
     }
@@ -319,6 +319,7 @@ extern "C" int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_st
   if (run->nblocks < 0 || run->nblocks > 1024) return hs_set_error(HS_ERR_ARG, "nblocks must be in [0, 1024]");
   if (!inst->uniform_beta)
     return hs_set_error(HS_ERR_ARG, "Neel stepping needs a uniform temperature (beta) over the grid");
+  if (!(inst->beta64 > 0.0)) return hs_set_error(HS_ERR_ARG, "Neel stepping needs beta64 > 0 (FP64 1/(kB T))");
   hipStream_t s = (hipStream_t)stream;
   const int P = hs_neel_blocks(inst, run->nblocks);
   k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);

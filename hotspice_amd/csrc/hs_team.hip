@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
         const float *in = s_in[src];
         const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
-                                     (double)in[4], (double)beta, mpf64, barrier, PERP, in[5], R.attempt_freq);
+                                     (double)in[4], I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
         pd = (unsigned long long)__double_as_longlong(dt);
       }
       bool bad = false;
@@ -572,6 +572,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     if (t.run.max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
     if (t.run.log && (!t.log_idx || !t.log_dt)) return hs_set_error(HS_ERR_ARG, "team %d: log without arrays", i);
     if (!t.inst->uniform_beta) return hs_set_error(HS_ERR_ARG, "Neel stepping needs a uniform temperature");
+    if (!(t.inst->beta64 > 0.0)) return hs_set_error(HS_ERR_ARG, "team %d: beta64 must be > 0 (FP64 1/(kB T))", i);
     if (i > 0 && (bool)t.inst->use_perp != perp) return hs_set_error(HS_ERR_ARG, "teams of one launch must agree on use_perp");
     perp = t.inst->use_perp;
     if (t.nblocks < 0 || t.nblocks > 4096) return hs_set_error(HS_ERR_ARG, "team %d: bad nblocks", i);

@@ -1,0 +1,65 @@
+"""Lock-step parity of the HIP path with the oracle over seeded random configurations: lattice,
+size, PBC, update scheme, barrier method, energy mix (dipolar with and without a cutoff, dumbbell
+dipoles, Zeeman, exchange), temperature, Metropolis kernel size, sum-or-convolution cutoff and
+sampling radius. The bars are those of the fixed cases (test_gpu_parity.lockstep)."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+from test_gpu_parity import lockstep
+
+pytestmark = pytest.mark.gpu
+
+LATTICES = ["OOP_Square", "IP_Square", "IP_Pinwheel", "IP_Kagome", "IP_Ising"]
+
+
+def _random_spec(rng, k):
+    lat = str(rng.choice(LATTICES))
+    n = int(rng.integers(20, 44)) if lat == "IP_Kagome" else int(rng.integers(10, 36))
+    pbc = bool(rng.random() < 0.4) and lat != "IP_Kagome"
+    scheme = "Néel" if rng.random() < 0.5 else "Metropolis"
+    energies = []
+    r = rng.random()
+    if r < 0.45:
+        cut = float(rng.choice([math.inf, 3.1e-6, 5.1e-6])) if not pbc else math.inf
+        energies.append(("dipolar", dict(cutoff_range=cut) if math.isfinite(cut) else {}))
+    elif r < 0.7 and lat.startswith("IP"):
+        energies.append(("dimonopolar", dict(d=float(rng.uniform(150e-9, 350e-9)),
+                                             small_d=float(rng.uniform(80e-9, 200e-9)))))
+    if rng.random() < 0.6:
+        energies.append(("zeeman", dict(magnitude=float(rng.uniform(0, 3e-3)), angle=float(rng.uniform(0, 6.28)))))
+    if rng.random() < 0.4 or not energies:
+        energies.append(("exchange", dict(J_kT=float(rng.uniform(0.05, 0.6)))))
+    params = dict(UPDATE_SCHEME=scheme)
+    if scheme == "Metropolis":
+        params.update(REDUCED_KERNEL_SIZE=int(rng.choice([0, 3, 5, 20])),
+                      SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=int(rng.choice([0, 2, 20])))
+        if pbc and params["REDUCED_KERNEL_SIZE"] == 0:
+            # With PBC and the full kernel, the reference's convolution branch counts a pair once
+            # per periodic image it reaches; the sum branch is the one both sides share.
+            params["SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF"] = 1000
+        update_kw = dict(r=float(rng.uniform(1.5e-6, 6e-6)))
+    else:
+        if rng.random() < 0.3:
+            params["ENERGY_BARRIER_METHOD"] = "parabolic"
+        update_kw = dict(t_max=np.inf)
+    moment = 1.1e-15 if lat == "OOP_Square" else 2.37e-16
+    return dict(lattice=lat, a=1e-6, n=n, T=float(rng.uniform(200, 900)), moment=moment, energies=energies,
+                params=params, update_kw=update_kw, nsteps=60, seed=1000 + k, kw=dict(PBC=pbc) if pbc else {})
+
+
+N_CONFIGS = 48
+_rng = np.random.default_rng(20261203)
+RANDOM = {f"random_{k:02d}": _random_spec(_rng, k) for k in range(N_CONFIGS)}
+
+
+@pytest.mark.parametrize("name", sorted(RANDOM))
+def test_random_config_lockstep(name):
+    cases.SYNTH[name] = RANDOM[name]
+    try:
+        done, tie = lockstep(name, check_every=20)
+    finally:
+        del cases.SYNTH[name]
+    assert done == RANDOM[name]["nsteps"] or tie is not None
