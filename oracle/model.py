@@ -303,10 +303,19 @@ def _large_positions(spacing, n, u):
 
 
 def _inv_dist(dx, dy, f32):
-    """|r|^-1 with 0 where the charges coincide; FP32 squares for the plain kernel (energies.py:508-525)."""
+    """|r|^-1 with 0 where the charges coincide. The plain kernel rounds the squares to FP32 and
+    takes their FP32 power -0.5 (energies.py:508-525). That power is taken here as the correctly
+    rounded FP32 value of sq^-0.5, formed in FP64 (two FP64 roundings stay far below half an FP32
+    ulp): the last bit of the reference's own FP32 power depends on its array library, and NumPy's
+    float32 pow misrounds about 1 in 5 of these values, which moves a net dumbbell kernel entry, a
+    near-cancelling sum of four such terms, by up to ~1.5e-5 relative."""
     sq = dx ** 2 + dy ** 2
     if f32:
-        sq = sq.astype(np.float32)
+        sq32 = sq.astype(np.float32).astype(np.float64)
+        out = np.zeros(sq.shape, dtype=np.float32)
+        nz = sq32 != 0
+        out[nz] = (1.0 / np.sqrt(sq32[nz])).astype(np.float32)
+        return out
     sq[sq == 0] = np.inf
     return sq ** -0.5
 

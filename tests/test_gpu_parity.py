@@ -167,8 +167,11 @@ def test_dipolar_kernels_equal_oracle(lat, n, kw, cutoff):
                                       ("IP_Square", 14, {}), ("IP_Square", 12, dict(PBC=True))])
 def test_dimonopolar_kernels_equal_oracle(lat, n, kw):
     """hs_build_dimonopolar_kernels against the oracle's restatement of energies.py:416-641. The
-    plain kernel rounds squared distances to FP32 (as the reference does), so its entries may
-    differ by the last FP32 bit of a 1/r term: the tolerance is 2^-22 of the largest 1/r term."""
+    plain kernel rounds squared distances to FP32 and takes their FP32 power -0.5 (as the
+    reference does); both sides take the correctly rounded FP32 value, so each 1/r term is the
+    same FP32 number and the entries (FP64 sums of four terms) agree to FP64 rounding, also
+    where the four terms nearly cancel. The closest-charge bound (2^-22 of the largest 1/r term)
+    is what a single misrounded FP32 term would cost; the test asserts far below it."""
     import hotspice_amd as hs
     from oracle import model
     cls = getattr(hs.ASI, lat)
@@ -187,8 +190,9 @@ def test_dimonopolar_kernels_equal_oracle(lat, n, kw):
     owner = np.concatenate([np.arange(len(ys))] * 2)
     dist = np.hypot(qx[:, None] - qx[None, :], qy[:, None] - qy[None, :])
     rmin = dist[owner[:, None] != owner[None, :]].min()  # closest charges of two magnets
-    tol = 4 * 1e-7 * 2.0 ** -22 / rmin
-    np.testing.assert_allclose(dm.kernel_unitcell, K, rtol=0, atol=tol)
+    one_term_ulp = 1e-7 * 2.0 ** -22 / rmin
+    assert np.abs(dm.kernel_unitcell - K).max() < 1e-3 * one_term_ulp
+    np.testing.assert_allclose(dm.kernel_unitcell, K, rtol=0, atol=1e-13 * np.abs(K).max())
     scale = np.abs(Kps).max()
     np.testing.assert_allclose(dm.kernel_perpself_unitcell, Kps, rtol=0, atol=1e-12 * scale)
     np.testing.assert_allclose(dm.kernel_perpother_unitcell, Kpo, rtol=0, atol=1e-12 * scale)
@@ -213,8 +217,8 @@ def test_dimonopolar_fields_equal_oracle(lat, n, kw):
                             PBC=kw.get("PBC", False))
     om.update_energy()
     scale = np.abs(om.E).max()
-    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-6 * scale)
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-9 * scale)
     np.testing.assert_allclose(mm.E_perp, om.E_perp, rtol=0, atol=1e-9 * scale)
-    assert mm.E_tot == pytest.approx(om.E_tot, rel=1e-6, abs=1e-6 * scale)
+    assert mm.E_tot == pytest.approx(om.E_tot, rel=1e-9, abs=1e-9 * scale)
     d = mm.get_energy('dimonopolar')
-    np.testing.assert_allclose(d.E, om.get_energy('dimonopolar').E, rtol=0, atol=1e-6 * scale)
+    np.testing.assert_allclose(d.E, om.get_energy('dimonopolar').E, rtol=0, atol=1e-9 * scale)

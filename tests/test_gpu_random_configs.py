@@ -50,16 +50,48 @@ def _random_spec(rng, k):
                 params=params, update_kw=update_kw, nsteps=60, seed=1000 + k, kw=dict(PBC=pbc) if pbc else {})
 
 
+def _random_presample_spec(rng, k):
+    """Grids of 150-220 cells a side sampled at 1.0-1.5 lattice spacings: more samples per step
+    than one selection workgroup has threads, so every step takes the presampled route (wide
+    sampling launch, batched selection, tile marking). Windows stay small (REDUCED_KERNEL_SIZE
+    2-6), so both update branches (sums below the switch cutoff, truncated convolutions above)
+    are reached."""
+    lat = str(rng.choice(LATTICES))
+    n = 2 * int(rng.integers(75, 111))
+    pbc = bool(rng.random() < 0.4) and lat != "IP_Kagome"
+    a = 1e-6
+    energies = []
+    r = rng.random()
+    if r < 0.5 or not lat.startswith("IP"):
+        energies.append(("dipolar", dict(cutoff_range=float(rng.choice([2.1e-6, 3.1e-6, 4.1e-6])))))
+    else:
+        energies.append(("dimonopolar", dict(d=float(rng.uniform(150e-9, 350e-9)),
+                                             small_d=float(rng.uniform(80e-9, 200e-9)))))
+    if rng.random() < 0.6:
+        energies.append(("zeeman", dict(magnitude=float(rng.uniform(0, 3e-3)), angle=float(rng.uniform(0, 6.28)))))
+    if rng.random() < 0.5:
+        energies.append(("exchange", dict(J_kT=float(rng.uniform(0.05, 0.6)))))
+    params = dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=int(rng.choice([2, 3, 4, 6])),
+                  SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=int(rng.choice([0, 20, 100000])))
+    moment = 1.1e-15 if lat == "OOP_Square" else 2.37e-16
+    return dict(lattice=lat, a=a, n=n, T=float(rng.uniform(200, 900)), moment=moment, energies=energies,
+                params=params, update_kw=dict(r=float(rng.uniform(1.0, 1.5)) * a), nsteps=16, seed=2000 + k,
+                kw=dict(PBC=pbc) if pbc else {})
+
+
 N_CONFIGS = 48
+N_PRESAMPLE = 12
 _rng = np.random.default_rng(20261203)
 RANDOM = {f"random_{k:02d}": _random_spec(_rng, k) for k in range(N_CONFIGS)}
+_rng = np.random.default_rng(20261204)
+RANDOM.update({f"random_presample_{k:02d}": _random_presample_spec(_rng, k) for k in range(N_PRESAMPLE)})
 
 
 @pytest.mark.parametrize("name", sorted(RANDOM))
 def test_random_config_lockstep(name):
     cases.SYNTH[name] = RANDOM[name]
     try:
-        done, tie = lockstep(name, check_every=20)
+        done, tie = lockstep(name, check_every=8 if "presample" in name else 20)
     finally:
         del cases.SYNTH[name]
     assert done == RANDOM[name]["nsteps"] or tie is not None
