@@ -98,9 +98,14 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None):
     for m in mags:
         if not m._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-    teams = _team_array(mags, [it[1] for it in items], [it[2] for it in items], [it[3] for it in items],
-                        attempt_freq, begin, nblocks=[it[4] for it in items])
-    dev.run_teams(teams, "neel")
+    # one launch per E_perp setting: the team kernel is compiled for either, and instances are
+    # independent, so two launches in stream order step each exactly as one would
+    perp = [bool(m.USE_PERP_ENERGY) for m in mags]
+    for flag in sorted(set(perp)):
+        g = [i for i, p in enumerate(perp) if p == flag]
+        teams = _team_array([mags[i] for i in g], [items[i][1] for i in g], [items[i][2] for i in g],
+                            [items[i][3] for i in g], attempt_freq, begin, nblocks=[items[i][4] for i in g])
+        dev.run_teams(teams, "neel")
     queued = []
     if full is not None:
         queued = [m for (m, steps, *_), f in zip(items, full)

@@ -95,3 +95,38 @@ def test_random_config_lockstep(name):
     finally:
         del cases.SYNTH[name]
     assert done == RANDOM[name]["nsteps"] or tie is not None
+
+
+def _random_batch(scheme, seed, count=6):
+    rng = np.random.default_rng(seed)
+    specs, k = {}, 0
+    while len(specs) < count:
+        sp = _random_spec(rng, 500 + k)
+        k += 1
+        if sp["params"]["UPDATE_SCHEME"] == scheme:
+            specs[f"batch_{seed}_{len(specs)}"] = sp
+    return specs
+
+
+@pytest.mark.parametrize("scheme,seed", [("Néel", 1), ("Néel", 2), ("Metropolis", 3), ("Metropolis", 4)])
+def test_random_batch_equals_alone(scheme, seed):
+    """batch.progress_many over a random mix of instances (lattices, sizes, energies, PBC, in-plane
+    next to out-of-plane) ends every instance bit-identical to stepping it alone with
+    Magnets.progress: same spins, fields and compensations, time and counters."""
+    from hotspice_amd import batch
+    from test_gpu_team import _same, _snapshot
+    specs = _random_batch(scheme, seed)
+    cases.SYNTH.update(specs)
+    try:
+        kws = [sp["update_kw"] if scheme == "Metropolis" else {} for sp in specs.values()]
+        alone = [cases.product_magnets(n, cases.load(n)) for n in specs]
+        for m, kw in zip(alone, kws):
+            m.progress(t_max=math.inf, MCsteps_max=1.5, **kw)
+        mixed = [cases.product_magnets(n, cases.load(n)) for n in specs]
+        res = batch.progress_many(mixed, t_max=math.inf, MCsteps_max=1.5, per_instance=kws)
+        for m1, m2, r in zip(alone, mixed, res):
+            _same(_snapshot(m1), _snapshot(m2))
+            assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
+    finally:
+        for n in specs:
+            del cases.SYNTH[n]
