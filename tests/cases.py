@@ -127,7 +127,7 @@ def oracle_magnets(name, fx=None):
         else:
             ens.append(model.Exchange(J=kw["J_kT"] * kT))
     params = dict(sp["params"])
-    params["UPDATE_SCHEME"] = "Neel" if params["UPDATE_SCHEME"] in ("Néel", "Neel") else "Metropolis"
+    params["UPDATE_SCHEME"] = "Neel" if params["UPDATE_SCHEME"] in ("Néel", "Neel") else params["UPDATE_SCHEME"]
     return model.make_magnets(sp["lattice"], sp["a"], n=sp["n"], T=sp["T"], E_B=fx["E_B"],
                               moment=sp["moment"], pattern=fx["pattern"], energies=ens, params=params,
                               seed=sp["seed"], PBC=sp.get("kw", {}).get("PBC", False))

@@ -130,3 +130,82 @@ def test_random_batch_equals_alone(scheme, seed):
     finally:
         for n in specs:
             del cases.SYNTH[n]
+
+
+def _relax_specs(seed, count):
+    """Random configurations with a pair energy: without one, exchange between perpendicular
+    neighbours (IP_Square, pinwheel) is a cancellation zero and the sign of a switch energy,
+    which decides relax and minimize, would be rounding noise on both sides."""
+    rng = np.random.default_rng(seed)
+    out, k = {}, 0
+    while len(out) < count:
+        sp = _random_spec(rng, 700 + k)
+        k += 1
+        if any(kind in ("dipolar", "dimonopolar") for kind, _ in sp["energies"]):
+            out[f"relax_{len(out):02d}"] = sp
+    return out
+
+
+RELAX_RANDOM = _relax_specs(31, 10)
+
+
+@pytest.mark.parametrize("method", ["relax", "minimize"])
+@pytest.mark.parametrize("name", sorted(RELAX_RANDOM))
+def test_random_relax_matches_oracle(name, method):
+    """Magnets.relax / minimize (core.py:763-845) on random configurations against the oracle
+    with the same numpy.random draws: the same outcome (including the reference's IndexError on
+    grids shorter than two unit cells), spins bit-exact, counters, fields at 1e-5 of their scale."""
+    sp = RELAX_RANDOM[name]
+    cases.SYNTH[name] = sp
+    try:
+        fx = cases.load(name)
+        mm, om = cases.product_magnets(name, fx), cases.oracle_magnets(name, fx)
+        outcome = []
+        for m in (mm, om):
+            np.random.seed(17)
+            try:
+                getattr(m, method)()
+                outcome.append("ok")
+            except IndexError:
+                outcome.append("IndexError")
+        assert outcome[0] == outcome[1]
+        np.testing.assert_array_equal(mm.m, om.m)
+        assert mm.switches == om.switches and mm.attempted_switches == om.attempted_switches
+        np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * cases.energy_scale(om))
+    finally:
+        del cases.SYNTH[name]
+
+
+def _wolff_spec(rng, k):
+    lat = str(rng.choice(["OOP_Square", "IP_Ising", "IP_Kagome", "OOP_Square"]))
+    n = int(rng.integers(20, 44)) if lat == "IP_Kagome" else int(rng.integers(8, 40))
+    pbc = bool(rng.random() < 0.5) and lat != "IP_Kagome"
+    energies = [("exchange", dict(J_kT=float(rng.uniform(0.1, 0.9))))]
+    if rng.random() < 0.5:
+        energies.append(("zeeman", dict(magnitude=float(rng.uniform(0, 3e-4)), angle=float(rng.uniform(0, 6.28)))))
+    moment = 1.1e-15 if lat == "OOP_Square" else 2.37e-16
+    return dict(lattice=lat, a=1e-6, n=n, T=float(rng.uniform(200, 900)), moment=moment, energies=energies,
+                params=dict(UPDATE_SCHEME="Wolff"), update_kw={}, nsteps=30, seed=3000 + k,
+                kw=dict(PBC=pbc) if pbc else {})
+
+
+_rng = np.random.default_rng(41)
+WOLFF_RANDOM = {f"wolff_{k:02d}": _wolff_spec(_rng, k) for k in range(8)}
+
+
+@pytest.mark.parametrize("name", sorted(WOLFF_RANDOM))
+def test_random_wolff_matches_oracle(name):
+    """Wolff cluster steps (core.py:588-602, 711-718) on random exchange(+Zeeman) configurations:
+    the same cluster every step, spins bit-exact, counters, fields at 1e-5 of their scale."""
+    sp = WOLFF_RANDOM[name]
+    cases.SYNTH[name] = sp
+    try:
+        fx = cases.load(name)
+        mm, om = cases.product_magnets(name, fx), cases.oracle_magnets(name, fx)
+        for s in range(sp["nsteps"]):
+            np.testing.assert_array_equal(mm.update(), om.update(), err_msg=f"{name} step {s}")
+        np.testing.assert_array_equal(mm.m, om.m)
+        assert mm.switches == om.switches and mm.attempted_switches == om.attempted_switches
+        np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * cases.energy_scale(om))
+    finally:
+        del cases.SYNTH[name]
