@@ -56,6 +56,8 @@ class DeviceState:
         self.min_cells = None
         self.min_result = None
         self.rec_scratch = None
+        self.gen = 0  # bumped whenever the instance's parameters are refilled
+        self._rec_need = None  # (gen, recompute scratch bytes)
         self.team_ws = None
         self.sel_cache = {}
         self.log_cap = 0
@@ -148,10 +150,14 @@ class DeviceState:
                                                 self.rec_scratch.numel(), self.stream()))
 
     def recompute_scratch(self):
-        """The recompute scratch tensor of this instance, sized for it."""
-        need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
-        if need < 0:
-            raise _lib.HotspiceError("bad instance for recompute")
+        """The recompute scratch tensor of this instance, sized for it (the size is asked of the
+        library once per refill of the instance's parameters)."""
+        if self._rec_need is None or self._rec_need[0] != self.gen:
+            need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
+            if need < 0:
+                raise _lib.HotspiceError("bad instance for recompute")
+            self._rec_need = (self.gen, need)
+        need = self._rec_need[1]
         if self.rec_scratch is None or self.rec_scratch.numel() < need:
             self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
         return self.rec_scratch

@@ -692,6 +692,7 @@ class Magnets(ABC):
     def _device_params_changed(self):
         """(Re)fills the hs_instance_t the kernels read."""
         d = self._dev
+        d.gen += 1  # sizes derived from the instance (recompute scratch) are asked for again
         I = d.inst
         P = d.ptr
         I.m, I.flags, I.E, I.Ec = P(d.m), P(d.flags), P(d.E), P(d.Ec)

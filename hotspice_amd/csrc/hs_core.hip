@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "hs_device.h"
@@ -24,6 +25,61 @@ int hs_set_error(int code, const char *fmt, ...) {
 }
 
 void hs_clear_error() { g_err.clear(); }
+
+// ---- descriptor uploads ----------------------------------------------------------------------
+// Launch descriptors are built on the host and copied into the caller's workspace ahead of the
+// kernels that read them. A copy from pageable memory would have to be followed by a stream
+// synchronisation (the buffer must outlive the copy), which makes the host wait for every kernel
+// queued before it. Instead the bytes go through a ring of pinned buffers: a slot is refilled
+// only after the event recorded behind its last copy has completed, which for a ring of 8 is
+// long past in any ordinary sequence of launches.
+namespace {
+struct StageSlot {
+  void *host = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+  int device = -1;
+  bool pending = false;
+};
+constexpr int STAGE_SLOTS = 8;
+std::mutex g_stage_mu;
+StageSlot g_stage[STAGE_SLOTS];
+int g_stage_next = 0;
+}  // namespace
+
+int hs_upload_async(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return HS_OK;
+  std::lock_guard<std::mutex> lock(g_stage_mu);
+  StageSlot &slot = g_stage[g_stage_next];
+  g_stage_next = (g_stage_next + 1) % STAGE_SLOTS;
+  if (slot.pending) {
+    HS_HIP_TRY(hipEventSynchronize(slot.done));
+    slot.pending = false;
+  }
+  int dev = 0;
+  HS_HIP_TRY(hipGetDevice(&dev));
+  if (slot.done && slot.device != dev) {  // an event is recorded on streams of its own device
+    HS_HIP_TRY(hipEventDestroy(slot.done));
+    slot.done = nullptr;
+  }
+  if (!slot.done) {
+    HS_HIP_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
+    slot.device = dev;
+  }
+  if (slot.cap < bytes) {
+    if (slot.host) HS_HIP_TRY(hipHostFree(slot.host));
+    slot.host = nullptr;
+    slot.cap = 0;
+    const size_t cap = bytes < 65536 ? 65536 : bytes + bytes / 2;
+    HS_HIP_TRY(hipHostMalloc(&slot.host, cap, hipHostMallocDefault));
+    slot.cap = cap;
+  }
+  std::memcpy(slot.host, src, bytes);
+  HS_HIP_TRY(hipMemcpyAsync(dst, slot.host, bytes, hipMemcpyHostToDevice, s));
+  HS_HIP_TRY(hipEventRecord(slot.done, s));
+  slot.pending = true;
+  return HS_OK;
+}
 
 extern "C" int hs_version(void) { return 100; }
 
@@ -895,8 +951,7 @@ extern "C" int hs_recompute_fields_many(const hs_instance_t *const *insts, const
   }
   hipStream_t s = (hipStream_t)stream;
   char *base = (char *)ws;
-  HS_HIP_TRY(hipMemcpyAsync(base, host.data(), L.total, hipMemcpyHostToDevice, s));
-  HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  if (int rc = hs_upload_async(base, host.data(), L.total, s)) return rc;
   if (nl > 0) {
     k_recompute_small_lds_many<<<nl, SMALL_LDS_THREADS, 0, s>>>((const RecJob *)(base + L.dip_off),
                                                                 (const int32_t *)(base + L.lmap_off));

@@ -19,6 +19,11 @@ void hs_clear_error();
     if (e_ != hipSuccess) return hs_set_error(HS_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e_)); \
   } while (0)
 
+// Copies `bytes` of host memory to device memory `dst` in stream order, without waiting for the
+// work queued before it: the bytes go through a pinned staging buffer that is refilled only once
+// its previous copy has completed. `src` may be released as soon as the call returns.
+int hs_upload_async(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // Validates the fields every stepping kernel relies on (sizes, pointers, table bounds).
 int hs_check_instance(const hs_instance_t *I);
 

@@ -1816,8 +1816,7 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   int2 *tmap = (int2 *)(host.data() + L.tmap_off);
   for (size_t k = 0; k < L.tmap.size(); ++k) tmap[k] = L.tmap[k];
   hipStream_t s = (hipStream_t)stream;
-  HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
-  HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  if (int rc = hs_upload_async(base, host.data(), host.size(), s)) return rc;
   const MetroJob *dj = (const MetroJob *)(base + L.jobs_off);
   k_state_begin_many<<<(nteams + 255) / 256, 256, 0, s>>>(dj, nteams);
   HS_LAUNCH_CHECK("k_state_begin_many");

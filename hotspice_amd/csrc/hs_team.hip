@@ -627,8 +627,7 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     td[i] = d;
     for (int b = 0; b < L.G[i]; ++b) map[L.first[i] + b] = i;
   }
-  HS_HIP_TRY(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s));
-  HS_HIP_TRY(hipStreamSynchronize(s));  // the host descriptor buffer must outlive the copy
+  if (int rc = hs_upload_async(base, host.data(), host.size(), s)) return rc;
   int32_t *err = (int32_t *)(base + L.err_off);
   k_team_reset<<<nteams, 256, 0, s>>>((const TeamDev *)base, err);
   HS_LAUNCH_CHECK("k_team_reset");
