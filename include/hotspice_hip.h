@@ -253,7 +253,8 @@ int64_t hs_recompute_scratch_bytes(const hs_instance_t *inst);
  * all finishing passes): insts[i] (HOST pointers to the descriptors), which[i] as above,
  * scratch[i] / scratch_bytes[i] the recompute scratch of instance i. ws: device scratch of
  * hs_recompute_many_ws_bytes bytes for the descriptors. Same results as n single calls.
- * Blocking for the small descriptor upload, otherwise asynchronous.
+ * Asynchronous: the descriptors are copied through pinned staging and the host arrays may be
+ * released when the call returns.
  */
 int hs_recompute_fields_many(const hs_instance_t *const *insts, const int32_t *which, void *const *scratch,
                              const int64_t *scratch_bytes, int32_t n, void *ws, int64_t ws_bytes, void *stream);
@@ -305,7 +306,7 @@ int hs_zeeman_swap(const hs_instance_t *inst, double Bx0, double By0, const floa
  * state: device hs_state_t[2] (copy 0 holds the input state; the result is in copy
  * (steps_launched+1)&1, reported through *final_copy). partials: device scratch of
  * 32*nblocks*2 bytes. log_idx/log_dt: optional device arrays [max_steps] (flat winner index or
- * -1 when no switch; dt added to t). Non-blocking.
+ * -1 when no switch; dt added to t). Needs uniform_beta = 1 and beta64 > 0. Non-blocking.
  */
 int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_state_t *state, void *partials,
                 int32_t *log_idx, double *log_dt, int32_t *final_copy, void *stream);
@@ -383,8 +384,9 @@ int64_t hs_wolff_ws_bytes(const hs_instance_t *inst);
  * Philox streams; a team of several workgroups exchanges one tagged record per workgroup per
  * step (its best candidate and that cell's waiting time) and polls for the round's records
  * (bounded spin, err = 3 on timeout); teams never wait for each other. ws: device scratch of
- * hs_neel_teams_ws_bytes bytes. Blocking for the small descriptor upload, otherwise
- * asynchronous. Grids need ny * 2^ceil(log2 nx) <= 2^24.
+ * hs_neel_teams_ws_bytes bytes. Asynchronous (descriptors copied through pinned staging; the
+ * team array may be released when the call returns). Grids need ny * 2^ceil(log2 nx) <= 2^24.
+ * Every instance needs uniform_beta = 1 and beta64 > 0; teams of one call agree on use_perp.
  */
 int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
 int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
@@ -397,8 +399,8 @@ int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
  * the output tiles of all instances; each instance stops on its own predicate. team.sel is
  * required; team.log_cells / team.log_idx are the optional accepted-cell / count logs of
  * hs_metropolis_run; nblocks and log_dt are unused. ws: device scratch of
- * hs_metro_teams_ws_bytes bytes. Blocking for the small descriptor upload, otherwise
- * asynchronous.
+ * hs_metro_teams_ws_bytes bytes. Asynchronous (descriptors copied through pinned staging; the
+ * team array may be released when the call returns).
  */
 int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
 int64_t hs_metro_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
