@@ -585,7 +585,9 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
         mmu = (float)(-mv[u]) * muv[u];  // (m mu) after a flip
         if (SEL.metropolis_time) {
           const double ut = hs_u53(hs_bits53(I.seed, step, 5u, (uint64_t)c), 1);
-          tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * (double)btv[u]);
+          // FP64 beta when T is uniform: barrier*beta reaches a few hundred here as in Neel times
+          const double b64 = I.uniform_beta ? I.beta64 : (double)btv[u];
+          tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * b64);
         }
         acc = metro_accepts(I, step, c, ev[u], ecv[u], btv[u]);
       }

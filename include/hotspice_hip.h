@@ -71,8 +71,9 @@ typedef struct hs_instance {
   double dip_pf;        /* DipolarEnergy.prefactor (0 = no dipolar energy) */
   double J;             /* ExchangeEnergy.J (0 = no exchange energy) */
   double m_perp_factor; /* Magnets.m_perp_factor */
-  double beta64;        /* 1/(kB T) in FP64 when uniform_beta: Neel waiting times exp(b*beta)
-                           need it at ~1e-9 relative, since b*beta reaches a few hundred */
+  double beta64;        /* 1/(kB T) in FP64 when uniform_beta: Neel waiting times and
+                           METROPOLIS_TIME step times exp(b*beta) need it at ~1e-9 relative,
+                           since b*beta reaches a few hundred */
   /* 4-byte members */
   int32_t nx, ny, ux, uy;
   int32_t n_sub;        /* number of dipolar kernels (occupied unit-cell sites) */

@@ -12,6 +12,8 @@ Bars (written in the tests):
     exp(beta*barrier_w)*(-ln u)/f (the global minimum barrier cancels), and a barrier moves by at
     most 2*max|dE| (+ max|dE_perp|), so each step's time may differ by expm1(2*beta*that)
     relative; this grows with beta*|E|, which reaches ~100 for closely spaced dumbbell charges.
+    E_B is held in FP32 (2^-24 relative), which adds beta*E_B*2^-24 relative per step: 1.7e-5 at
+    a 290 kT barrier. Metropolis steps with METROPOLIS_TIME get the same bound.
     A step time that underflows into the subnormals (~1e-320 s in a deep minimum) has no
     relative precision left: the device clears the low 6 bits of every step time (so that all
     stepping paths add the same value), which there is up to 63 subnormal units, so each step
@@ -44,18 +46,20 @@ def lockstep(name, nsteps=None, check_every=25):
     om = oracle_magnets(name, fx)
     scale = energy_scale(om)
     neel = spec["params"]["UPDATE_SCHEME"] in ("Néel", "Neel")
+    timed = neel or bool(spec["params"].get("METROPOLIS_TIME", False))  # times exp(barrier * beta)
     nsteps = spec["nsteps"] if nsteps is None else nsteps
     t_slack = 0.0
+    beta_EB = float(np.max(np.abs(om.E_B) * om.beta))  # E_B is held in FP32 on the device
     for s in range(nsteps):
-        if neel:
+        if timed:
             dE = float(np.max(np.abs(np.asarray(mm.E, dtype=float) - om.E)))
             if om.USE_PERP_ENERGY:
                 dE += float(np.max(np.abs(np.asarray(mm.E_perp, dtype=float) - om.E_perp)))
             t_0 = om.t
         a = mm.update(**spec["update_kw"])
         b = om.update(**spec["update_kw"])
-        if neel and math.isfinite(om.t - t_0):
-            t_slack += abs(om.t - t_0) * math.expm1(2 * float(np.max(om.beta)) * dE)
+        if timed and math.isfinite(om.t - t_0):
+            t_slack += abs(om.t - t_0) * math.expm1(2 * float(np.max(om.beta)) * dE + beta_EB * 2.0 ** -24)
         if neel:
             ga = -1 if a is None else int(a[0]) * om.nx + int(a[1])
             gb = -1 if b is None else int(b[0]) * om.nx + int(b[1])

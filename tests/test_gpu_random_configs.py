@@ -79,12 +79,29 @@ def _random_presample_spec(rng, k):
                 kw=dict(PBC=pbc) if pbc else {})
 
 
+def _random_metro_spec(rng, k):
+    """Metropolis with the other selection options the oracle replays exactly: the 'single'
+    scheme, METROPOLIS_TIME (step time from the samples' barriers), and a radius derived from Q
+    (calc_r) instead of an explicit r."""
+    sp = _random_spec(rng, 4000 + k)
+    while sp["params"]["UPDATE_SCHEME"] != "Metropolis":
+        sp = _random_spec(rng, 4000 + k)
+    sp["params"]["MULTISAMPLING_SCHEME"] = "single" if rng.random() < 0.3 else "grid"
+    sp["params"]["METROPOLIS_TIME"] = bool(rng.random() < 0.5)
+    if rng.random() < 0.5:
+        sp["update_kw"] = dict(Q=float(rng.uniform(0.01, 0.3)))
+    return sp
+
+
 N_CONFIGS = 48
 N_PRESAMPLE = 12
+N_METRO = 16
 _rng = np.random.default_rng(20261203)
 RANDOM = {f"random_{k:02d}": _random_spec(_rng, k) for k in range(N_CONFIGS)}
 _rng = np.random.default_rng(20261204)
 RANDOM.update({f"random_presample_{k:02d}": _random_presample_spec(_rng, k) for k in range(N_PRESAMPLE)})
+_rng = np.random.default_rng(20261205)
+RANDOM.update({f"random_metro_{k:02d}": _random_metro_spec(_rng, k) for k in range(N_METRO)})
 
 
 @pytest.mark.parametrize("name", sorted(RANDOM))
