@@ -21,17 +21,30 @@ def _serve(jobs, **kw):
     return srv.run(timeout=300), srv
 
 
-def test_served_job_equals_alone():
-    mixed, srv = _serve(_jobs())
-    assert len(mixed) == 8 and srv.iterations > 0
+def _check_served_equals_alone(make_jobs):
+    mixed, srv = _serve(make_jobs())
+    assert len(mixed) == len(make_jobs()) and srv.iterations > 0
     by_name = {(j.name, j.arrival): j.result for j in mixed}
-    for job in _jobs():
+    for job in make_jobs():
         job.arrival = 0.0
         (alone,), _ = _serve([job])
         match = [r for (name, arr), r in by_name.items() if name == job.name]
         assert any(np.array_equal(r["m"], alone.result["m"]) and r["t"] == alone.result["t"]
                    and r["attempts"] == alone.result["attempts"] and r["switches"] == alone.result["switches"]
                    and r["E_tot"] == alone.result["E_tot"] for r in match), job.name
+
+
+def test_served_job_equals_alone():
+    _check_served_equals_alone(_jobs)
+
+
+@pytest.mark.parametrize("seed,cutoff,n_range", [(11, 5, (24, 96)), (13, None, (16, 48))])
+def test_served_mix_equals_alone(seed, cutoff, n_range):
+    """Other mixes: windowed kernels with grids large enough for the presampled Metropolis steps,
+    and full kernels at another seed (lattices, schemes, sizes and arrivals drawn from it)."""
+    from hotspice_amd.workloads import served_jobs
+    _check_served_equals_alone(lambda: served_jobs(count=6, rate=800.0, seed=seed, n_range=n_range, field_steps=3,
+                                                   neel_work=200, metro_sweeps=1.0, cutoff_cells=cutoff))
 
 
 def test_latency_stats_and_budget():
