@@ -42,7 +42,7 @@ def lockstep(name, nsteps=None, check_every=25):
     """Steps product and oracle side by side; returns (steps matched, near-tie step or None)."""
     spec = case_spec(name)
     fx = load(name)
-    mm = product_magnets(name, fx)
+    mm = product_magnets(name, fx, **spec.get("product_kw", {}))
     om = oracle_magnets(name, fx)
     scale = energy_scale(om)
     neel = spec["params"]["UPDATE_SCHEME"] in ("Néel", "Neel")
