@@ -66,7 +66,7 @@ class Run(C.Structure):
     """hs_run_t"""
     _fields_ = [("t_max", C.c_double), ("mc_max", C.c_double), ("attempt_freq", C.c_double),
                 ("max_steps", C.c_int64), ("log", C.c_int32), ("nblocks", C.c_int32), ("begin", C.c_int32),
-                ("pad", C.c_int32)]
+                ("first_free", C.c_int32)]
 
 
 class State(C.Structure):

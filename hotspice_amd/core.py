@@ -815,7 +815,7 @@ class Magnets(ABC):
         self._pre_step()
         self._maybe_recompute()
         run = _lib.Run(t_max=float(t_max), mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1,
-                       nblocks=0, begin=1)
+                       nblocks=0, begin=1, first_free=1)
         if not self._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
         self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self._neel_nblocks())
@@ -840,7 +840,7 @@ class Magnets(ABC):
         cells = t.zeros(cap, dtype=t.int32, device=self._dev.device)
         count = t.zeros(1, dtype=t.int32, device=self._dev.device)
         run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1, nblocks=0,
-                       begin=1)
+                       begin=1, first_free=1)
         self._dev.metropolis_run(sel, run, cells, count)
         self._after_device_step()
         a = int(count.item())
@@ -866,7 +866,8 @@ class Magnets(ABC):
         t = self._dev.torch
         cells = t.zeros(self.nx * self.ny, dtype=t.int32, device=self._dev.device)
         count = t.zeros(1, dtype=t.int32, device=self._dev.device)
-        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=1e10, max_steps=1, log=1, nblocks=0, begin=1)
+        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=1e10, max_steps=1, log=1, nblocks=0, begin=1,
+                       first_free=1)
         p = self.params
         self._dev.wolff_run(run, p.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF, p.REDUCED_KERNEL_SIZE, cells, count)
         self._after_device_step()

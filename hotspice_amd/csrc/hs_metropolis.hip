@@ -401,8 +401,9 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
   if (tid < HS_MAX_UC) s_site[tid] = 0;
-  bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
-            hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  bool go = !S.done && S.steps_done < R.max_steps &&
+            ((R.first_free && S.steps_done == 0) ||
+             (hs_lower_than(S.t - S.t0, R.t_max) && hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) {
     __syncthreads();
     if (tid == 0) {
@@ -1003,8 +1004,10 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   __shared__ long long s_l[PRESAMPLE_THREADS / 64];
   const hs_state_t S = J.st[0];
   const double n = (double)I.n_occ;
-  const bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
-                  hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  const bool go = !S.done && S.steps_done < R.max_steps &&
+                  ((R.first_free && S.steps_done == 0) ||
+                   (hs_lower_than(S.t - S.t0, R.t_max) &&
+                    hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) return;
   const uint64_t step = (uint64_t)S.step;
   int ox, oy;
@@ -1377,8 +1380,10 @@ __global__ void __launch_bounds__(MS_THREADS) k_wolff_select(MetroJob J, WolffWS
   const double n = (double)I.n_occ;
   if (tid < HS_MAX_UC) s_site[tid] = 0;
   if (tid < I.nn_w * I.nn_h) s_nn[tid] = I.nn_mask[tid];
-  const bool go = !S.done && S.steps_done < R.max_steps && hs_lower_than(S.t - S.t0, R.t_max) &&
-                  hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max);
+  const bool go = !S.done && S.steps_done < R.max_steps &&
+                  ((R.first_free && S.steps_done == 0) ||
+                   (hs_lower_than(S.t - S.t0, R.t_max) &&
+                    hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) {
     if (tid == 0) {
       *ws.acc_n = 0;

@@ -135,8 +135,9 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
   bool compute = false;
   if (!done) {
     const double n = (double)I.n_occ;
-    compute = !last && sd < R.max_steps && hs_lower_than(t - S.t0, R.t_max) &&
-              hs_lower_than((double)att / n - (double)S.attempts0 / n, R.mc_max);
+    compute = !last && sd < R.max_steps &&
+              ((R.first_free && sd == 0) || (hs_lower_than(t - S.t0, R.t_max) &&
+                                              hs_lower_than((double)att / n - (double)S.attempts0 / n, R.mc_max)));
     if (!compute) done = 1;
   }
   if (blockIdx.x == 0 && tid == 0) {

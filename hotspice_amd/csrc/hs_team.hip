@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     }
   };
 
-  bool go = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
+  bool go = sd < R.max_steps && ((R.first_free && sd == 0) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
   if (go && !owner) draw(step, (step & 1) == 0);
   __syncthreads();
   while (go) {
