@@ -232,3 +232,23 @@ def test_random_wolff_matches_oracle(name):
         np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * cases.energy_scale(om))
     finally:
         del cases.SYNTH[name]
+
+
+def test_update_steps_at_zero_t_max():
+    """Magnets.update takes its step with no stop test (core.py:679-709): with t_max = 0 each Neel
+    step counts an attempt, adds t_max = 0 to t and switches nothing, on the device as in the
+    oracle; a following update(t_max=inf) switches the same magnet on both."""
+    name = "random_00"
+    assert RANDOM[name]["params"]["UPDATE_SCHEME"] == "Néel"
+    cases.SYNTH[name] = RANDOM[name]
+    try:
+        fx = cases.load(name)
+        mm, om = cases.product_magnets(name, fx), cases.oracle_magnets(name, fx)
+        for _ in range(3):
+            assert mm.update(t_max=0.0) is None and om.update(t_max=0.0) is None
+        assert mm.attempted_switches == om.attempted_switches == 3
+        assert mm.switches == om.switches == 0 and mm.t == om.t == 0.0
+        np.testing.assert_array_equal(np.asarray(mm.update(t_max=math.inf)), np.asarray(om.update(t_max=math.inf)))
+        np.testing.assert_array_equal(mm.m, om.m)
+    finally:
+        del cases.SYNTH[name]
