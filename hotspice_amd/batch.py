@@ -82,10 +82,12 @@ def neel_team_size(m, nblocks=0):
     return max(1, min(cus, -(-m.n // 512)))
 
 
-def launch_neel(items, attempt_freq=1e10, begin=False, full=None):
+def launch_neel(items, attempt_freq=1e10, begin=False, full=None, updates=False):
     """ONE persistent launch stepping every (mm, max_steps, t_max, mc_max, nblocks) item. With
     begin=True the launch starts each instance's progress call (its origin t0, attempts0 := the
-    current t, attempts); otherwise the origin must have been set. Afterwards checks each
+    current t, attempts); otherwise the origin must have been set. updates=True runs max_steps
+    update(t_max) calls instead of a progress() call: every step runs, whatever t has reached
+    (also +inf), and compares its waiting time with t_max itself (core.py:702). Afterwards checks each
     instance's error word, counts its steps towards the periodic FP64 recompute and runs that
     when due. `full[i]`: item i is known to run all its max_steps (no stop predicate can end it
     earlier); when those steps complete its recompute period, the recompute is queued right
@@ -104,7 +106,8 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None):
     for flag in sorted(set(perp)):
         g = [i for i, p in enumerate(perp) if p == flag]
         teams = _team_array([mags[i] for i in g], [items[i][1] for i in g], [items[i][2] for i in g],
-                            [items[i][3] for i in g], attempt_freq, begin, nblocks=[items[i][4] for i in g])
+                            [items[i][3] for i in g], attempt_freq, begin, nblocks=[items[i][4] for i in g],
+                            first_free=2 if updates else 0)
         dev.run_teams(teams, "neel")
     queued = []
     if full is not None:
@@ -118,7 +121,7 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None):
 _TEAM_DTYPE = np.dtype(_lib.Team)
 
 
-def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=None):
+def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=None, first_free=0):
     """The hs_team_t descriptors of one launch, filled column by column."""
     arr = np.zeros(len(mags), dtype=_TEAM_DTYPE)
     arr['inst'] = [_lib.C.addressof(m._dev.inst) for m in mags]
@@ -131,6 +134,7 @@ def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=
     run['attempt_freq'] = float(attempt_freq)
     run['max_steps'] = np.asarray(steps, dtype=np.int64)
     run['begin'] = int(begin)
+    run['first_free'] = int(first_free)
     if nblocks is not None:
         arr['nblocks'] = np.asarray(nblocks, dtype=np.int32)
     return arr
@@ -164,6 +168,7 @@ def _after_launch(mags, what, before=None, recomputed=()):
         m._since_recompute += int(st.steps_done)
         if id(m) in recomputed:
             m._since_recompute = 0
+            m.recomputes += 1
             continue
         if m.recompute_every and m._since_recompute >= m.recompute_every and m._incremental_exact():
             due.append(m)
@@ -171,6 +176,7 @@ def _after_launch(mags, what, before=None, recomputed=()):
         due[0]._dev.recompute_many([m._dev for m in due], [3 if m.USE_PERP_ENERGY else 1 for m in due])
         for m in due:
             m._since_recompute = 0
+            m.recomputes += 1
     return sts
 
 

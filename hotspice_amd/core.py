@@ -139,6 +139,7 @@ class Magnets(ABC):
         self._m_perp_factor = 0.0
         self._calc_r_cache = {}
         self._since_recompute = 0
+        self.recomputes = 0  # FP64 full recomputes run so far
         self._sel_key = None
         self.initialize_m(self._get_groundstate() if pattern is None else pattern, update_energy=False)
         self.T = T
@@ -764,6 +765,7 @@ class Magnets(ABC):
             return
         self._dev.recompute(3 if self.USE_PERP_ENERGY else 1)
         self._since_recompute = 0
+        self.recomputes += 1
 
     def _device_zeeman_changed(self, z: ZeemanEnergy, old):
         """ZeemanEnergy.set_field: swap the Zeeman term of the combined device field."""
@@ -848,6 +850,9 @@ class Magnets(ABC):
         if st.err:
             raise _lib.HotspiceError(f"device Metropolis step failed (err={st.err})")
         c = cells[:a].cpu().numpy().astype(np.int64)
+        # the periodic FP64 recompute counts update() steps as progress() counts its steps
+        self._since_recompute += 1
+        self._maybe_recompute()
         return np.asarray([c // self.nx, c % self.nx])
 
     def _wolff_check(self):

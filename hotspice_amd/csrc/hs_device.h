@@ -131,6 +131,12 @@ HS_INLINE bool hs_lower_than(double x, double y) {
   return !close;
 }
 
+// hs_run_t.first_free: 0 = the steps of one progress() call (core.py:753-760), each behind its
+// stop predicate; 1 = one update() call (core.py:604-622), whose step runs unconditionally;
+// 2 = a sequence of update() calls: every step runs unconditionally and Neel compares each
+// step's time with t_max itself. True when step `sd` of the call is such a free step.
+HS_INLINE bool hs_step_free(int32_t first_free, int64_t sd) { return first_free == 2 || (first_free == 1 && sd == 0); }
+
 // Python's modulo (result has the sign of the divisor).
 HS_INLINE int hs_pmod(int a, int b) {
   const int r = a % b;

@@ -402,7 +402,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   const double n = (double)I.n_occ;
   if (tid < HS_MAX_UC) s_site[tid] = 0;
   bool go = !S.done && S.steps_done < R.max_steps &&
-            ((R.first_free && S.steps_done == 0) ||
+            (hs_step_free(R.first_free, S.steps_done) ||
              (hs_lower_than(S.t - S.t0, R.t_max) && hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) {
     __syncthreads();
@@ -1005,7 +1005,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   const hs_state_t S = J.st[0];
   const double n = (double)I.n_occ;
   const bool go = !S.done && S.steps_done < R.max_steps &&
-                  ((R.first_free && S.steps_done == 0) ||
+                  (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
                     hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) return;
@@ -1381,7 +1381,7 @@ __global__ void __launch_bounds__(MS_THREADS) k_wolff_select(MetroJob J, WolffWS
   if (tid < HS_MAX_UC) s_site[tid] = 0;
   if (tid < I.nn_w * I.nn_h) s_nn[tid] = I.nn_mask[tid];
   const bool go = !S.done && S.steps_done < R.max_steps &&
-                  ((R.first_free && S.steps_done == 0) ||
+                  (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
                     hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
   if (!go) {

@@ -110,7 +110,9 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
       err = 1;  // nanargmin over an all-NaN grid
       done = 1;
     } else {
-      const double tmax_step = R.t_max - (t - S.t0);  // progress passes t_max - dt (core.py:757)
+      // progress passes t_max - dt (core.py:757); update() passes t_max itself, also when t has
+      // overflowed to +inf (where t - t0 would be NaN)
+      const double tmax_step = hs_step_free(R.first_free, sd) ? R.t_max : R.t_max - (t - S.t0);
       att += 1;
       step += 1;
       sd += 1;
@@ -136,7 +138,7 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
   if (!done) {
     const double n = (double)I.n_occ;
     compute = !last && sd < R.max_steps &&
-              ((R.first_free && sd == 0) || (hs_lower_than(t - S.t0, R.t_max) &&
+              (hs_step_free(R.first_free, sd) || (hs_lower_than(t - S.t0, R.t_max) &&
                                               hs_lower_than((double)att / n - (double)S.attempts0 / n, R.mc_max)));
     if (!compute) done = 1;
   }

@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     }
   };
 
-  bool go = sd < R.max_steps && ((R.first_free && sd == 0) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
+  bool go = sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
   if (go && !owner) draw(step, (step & 1) == 0);
   __syncthreads();
   while (go) {
@@ -337,7 +337,9 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         err = 1;  // nanargmin over an all-NaN grid
       } else {
         const double dt = __longlong_as_double((long long)pd);
-        const double tmax_step = R.t_max - (t - t0);
+        // progress passes t_max - dt (core.py:757); update() passes t_max itself, also when t
+        // has overflowed to +inf (where t - t0 would be NaN)
+        const double tmax_step = hs_step_free(R.first_free, sd) ? R.t_max : R.t_max - (t - t0);
         att += 1;
         sd += 1;
         const bool flip = !(dt > tmax_step || isnan(dt));
@@ -348,7 +350,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           sT.log_idx[sd - 1] = flip ? (int)(wp >> XB) * nx + (int)(wp & XM) : -1;
           sT.log_dt[sd - 1] = flip ? dt : tmax_step;
         }
-        const bool more = sd < R.max_steps && att < att_stop && hs_lower_than(t - t0, R.t_max);
+        const bool more =
+            sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
         ctl = (flip ? 1u : 0u) | (more ? 2u : 0u);
       }
       if (lane == 0) {

@@ -87,10 +87,15 @@ class _Active:
         self.mm._set_progress_origin(*self.start)
 
     def step_done(self, st=None):
+        """Neel: `work` update(t_max=inf) calls have run (each steps unconditionally, also once t
+        has overflowed to +inf). Metropolis: progress(t_max=inf, MCsteps_max=work) has returned,
+        i.e. its predicate (core.py:756) fails, on the MC steps or on an infinite (or NaN) elapsed
+        time from METROPOLIS_TIME."""
         st = self.mm._state() if st is None else st
         if self.scheme == Scheme.NEEL:
             return st.attempts - self.start[1] >= int(self.job.work)
-        return not lower_than(st.attempts / self.mm.n - self.start[1] / self.mm.n, self.job.work)
+        return not (lower_than(st.t - self.start[0], math.inf)
+                    and lower_than(st.attempts / self.mm.n - self.start[1] / self.mm.n, self.job.work))
 
     def advance(self):
         """Moves to the next field step; returns True when the job is complete."""
@@ -159,7 +164,7 @@ class Server:
                 items.append((a.mm, steps, math.inf, math.inf, g))
                 used += g
                 self.attempts -= st.attempts
-            self.attempts += sum(st.attempts for st in batch.launch_neel(items, 1e10))
+            self.attempts += sum(st.attempts for st in batch.launch_neel(items, 1e10, updates=True))
         if metro:
             items = []
             for a, st in zip(metro, batch.read_states([a.mm for a in metro])):

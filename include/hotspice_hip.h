@@ -130,7 +130,9 @@ typedef struct hs_run {
   int32_t nblocks;      /* grid size for grid-wide kernels (0 = automatic) */
   int32_t begin;        /* 1: this call starts a progress() call (t0, attempts0 := t, attempts) */
   int32_t first_free;   /* 1: the call's first step runs whatever t and MCsteps are, as
-                           Magnets.update steps unconditionally (core.py:659-718) */
+                           Magnets.update steps unconditionally (core.py:659-718);
+                           2: every step does (a sequence of update() calls, each Neel step
+                           compared with t_max itself, core.py:702); 0: progress() */
 } hs_run_t;
 
 /* Device-resident run state; the library keeps two copies (double buffer). */

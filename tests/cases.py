@@ -36,6 +36,35 @@ CASES = {
     "metro_pinwheel": dict(lattice="IP_Pinwheel", a=1e-6, n=24, T=300, moment=2.37e-16,
                            energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=math.pi / 8))],
                            params=dict(UPDATE_SCHEME="Metropolis"), update_kw=dict(r=2.5e-6), nsteps=150, seed=24),
+    "metro_kagome_single": dict(lattice="IP_Kagome", a=1e-6, n=20, T=300, moment=2.37e-16,
+                                energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=0.5))],
+                                params=dict(UPDATE_SCHEME="Metropolis", MULTISAMPLING_SCHEME="single"),
+                                update_kw={}, nsteps=300, seed=25),
+    "metro_oop_fallback": dict(lattice="OOP_Square", a=1e-6, n=10, T=300, moment=1.1e-15,
+                               energies=[("dipolar", {}), ("exchange", dict(J_kT=0.2))],
+                               params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0),
+                               update_kw=dict(r=8.5e-6), nsteps=150, seed=26),
+}
+
+# Reference fixtures of progress() calls (tests/golden/make_golden.py, "calls"): one entry per call.
+PROGRESS = {
+    "prog_neel_kagome": dict(lattice="IP_Kagome", a=1e-6, n=20, T=300, moment=2.37e-16,
+                             energies=[("dipolar", {}), ("zeeman", dict(magnitude=5e-4, angle=0.3)), ("exchange", dict(J_kT=0.3))],
+                             params=dict(UPDATE_SCHEME="Néel"),
+                             calls=[dict(t_max=np.inf, MCsteps_max=0.5), dict(t_max=5e-8, MCsteps_max=np.inf),
+                                    dict(t_max=2e-8, MCsteps_max=1.0), dict(t_max=1e-12, MCsteps_max=4.0),
+                                    dict(t_max=1e-9, MCsteps_max=0.25), dict(t_max=3e-8, MCsteps_max=0.4)], seed=41),
+    "prog_metro_oop": dict(lattice="OOP_Square", a=1e-6, n=32, T=300, moment=1.1e-15,
+                           energies=[("dipolar", dict(cutoff_range=5e-6)), ("exchange", dict(J_kT=0.2))],
+                           params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=5,
+                                       SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF=3),
+                           calls=[dict(MCsteps_max=0.5, r=4e-6), dict(MCsteps_max=1.25, r=4e-6),
+                                  dict(t_max=np.inf, MCsteps_max=0.1, r=4e-6), dict(MCsteps_max=2.0, r=4e-6)], seed=42),
+    "prog_metro_time": dict(lattice="IP_Square", a=1e-6, n=24, T=300, moment=2.37e-16,
+                            energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=0.8))],
+                            params=dict(UPDATE_SCHEME="Metropolis", METROPOLIS_TIME=True),
+                            calls=[dict(t_max=2e-9, MCsteps_max=np.inf, r=3e-6), dict(t_max=np.inf, MCsteps_max=0.5, r=3e-6),
+                                   dict(t_max=1e-8, MCsteps_max=1.0, r=3e-6)], seed=43),
 }
 
 
@@ -94,7 +123,10 @@ SYNTH = {
 
 
 def spec(name):
-    return CASES[name] if name in CASES else SYNTH[name]
+    for table in (CASES, PROGRESS, SYNTH):
+        if name in table:
+            return table[name]
+    raise KeyError(name)
 
 
 def load(name):

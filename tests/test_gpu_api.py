@@ -348,3 +348,51 @@ def test_wolff_needs_exchange_and_no_dipolar():
                            params=hs.SimParams(UPDATE_SCHEME='Wolff'))
     with pytest.raises(NotImplementedError):
         mm.progress()
+
+
+def test_update_after_time_overflow():
+    """Barriers of 2000 kT overflow exp(beta * barrier) to +inf, so update(t_max=inf) switches a
+    magnet and sets t = +inf. update(t_max=1) then compares the step's time with t_max itself
+    (core.py:702-704): inf > 1, so t += 1 and nothing switches -- where t - t0 = inf - inf is NaN.
+    Later update(t_max=inf) calls switch the same magnets as the oracle's."""
+    import hotspice_amd as hs
+    from oracle import model
+    n = 12
+    rng = np.random.default_rng(21)
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    kT = hs.kB * 300
+    EB = 2000 * kT * rng.normal(1, 0.01, size=(n, n))
+    mm = hs.ASI.OOP_Square(1e-6, n, T=300, E_B=EB, moment=1.1e-15, pattern=pat, seed=21,
+                           energies=[hs.DipolarEnergy()], params=hs.SimParams(UPDATE_SCHEME='Néel'))
+    om = model.make_magnets("OOP_Square", 1e-6, n=n, T=300, E_B=EB, moment=1.1e-15, pattern=pat, seed=21,
+                            energies=[model.Dipolar()], params=dict(UPDATE_SCHEME="Neel"))
+    for t_max in (math.inf, 1.0, 1.0, math.inf, 1e-3, math.inf, math.inf):
+        a = mm.update(t_max=t_max)
+        b = om.update(t_max=t_max)
+        assert (a is None) == (b is None), (t_max, a, b)
+        if b is not None:
+            assert tuple(map(int, a)) == tuple(map(int, b))
+        assert mm.t == om.t == math.inf
+    assert mm.attempted_switches == om.attempted_switches == 7
+    assert mm.switches == om.switches == 4
+    np.testing.assert_array_equal(mm.m, om.m)
+
+
+def test_metropolis_update_runs_the_periodic_recompute():
+    """update() steps under Metropolis count towards the FP64 recompute period as progress() steps
+    do: with recompute_every = 5, ten update() calls run two recomputes, and the fields stay at
+    the FP64 oracle's."""
+    import hotspice_amd as hs
+    from oracle import model
+    n = 24
+    rng = np.random.default_rng(4)
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    mm = hs.ASI.OOP_Square(1e-6, n, T=300, moment=1.1e-15, pattern=pat, seed=4, energies=[hs.DipolarEnergy()],
+                           params=hs.SimParams(UPDATE_SCHEME='Metropolis'), recompute_every=5)
+    om = model.make_magnets("OOP_Square", 1e-6, n=n, T=300, moment=1.1e-15, pattern=pat, seed=4,
+                            energies=[model.Dipolar()], params=dict(UPDATE_SCHEME="Metropolis"))
+    r0 = mm.recomputes
+    for _ in range(10):
+        np.testing.assert_array_equal(mm.update(r=4e-6), om.update(r=4e-6))
+    assert mm.recomputes - r0 == 2
+    np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-9 * np.abs(om.E).max())

@@ -110,14 +110,24 @@ def test_random_config_lockstep(name):
     periodic recompute lands inside the lock-step trajectory."""
     spec = dict(RANDOM[name])
     k = sorted(RANDOM).index(name)
-    if k % 2:
-        spec["product_kw"] = dict(recompute_every=7 + 9 * (k % 5))
+    every = 7 + 9 * (k % 5) if k % 2 else None
+    if every:
+        spec["product_kw"] = dict(recompute_every=every)
+    neel = spec["params"]["UPDATE_SCHEME"] == "Néel"
+
+    def recomputed(mm, n):
+        # update() counts switches (Neel) or steps (Metropolis) towards the period; a recompute
+        # is due once the count passes it, unless truncated windows are part of the result
+        counted = mm.switches if neel else spec["nsteps"]
+        if every and mm._incremental_exact() and counted > every:
+            assert n >= 1, f"{name}: no FP64 recompute after {counted} counted steps (period {every})"
+
     cases.SYNTH[name] = spec
     try:
-        done, tie = lockstep(name, check_every=8 if "presample" in name else 20)
+        compared, ties = lockstep(name, check_every=8 if "presample" in name else 20, check=recomputed)
     finally:
         del cases.SYNTH[name]
-    assert done == RANDOM[name]["nsteps"] or tie is not None
+    assert compared + len(ties) == RANDOM[name]["nsteps"]
 
 
 def _random_batch(scheme, seed, count=6):
