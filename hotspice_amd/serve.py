@@ -45,6 +45,7 @@ class Job:
     footprint: int = 0
     name: str = ""
     kwargs: dict = dataclasses.field(default_factory=dict)  # Metropolis r / Q, attempt_freq
+    spec: Optional[dict] = None  # what make() builds (for the caller's records; unused here)
     # filled in by the server
     admitted: Optional[float] = None
     finished: Optional[float] = None

@@ -156,29 +156,41 @@ def served_jobs(count=32, rate=200.0, seed=4, n_range=(64, 1024), field_steps=10
         n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
         lattice = "IP_Kagome" if rng.random() < 0.5 else "IP_Pinwheel"
         scheme = "Néel" if rng.random() < 0.5 else "Metropolis"
-        jseed = int(seed * 100000 + i)
-
-        def make(n=n, lattice=lattice, scheme=scheme, jseed=jseed):
-            import hotspice_amd as hs
-            r = np.random.default_rng(jseed)
-            kT = kB * 300
-            cls = getattr(hs.ASI, lattice)
-            shape = _lattice_shape(lattice, n)
-            dx = 1e-6 / 4 if lattice == "IP_Kagome" else 1e-6 / 2
-            dip = hs.DipolarEnergy() if cutoff_cells is None else \
-                hs.DipolarEnergy(cutoff_range=edge_cutoff(cutoff_cells * dx))
-            params = hs.SimParams(UPDATE_SCHEME=scheme) if cutoff_cells is None else \
-                hs.SimParams(UPDATE_SCHEME=scheme, REDUCED_KERNEL_SIZE=cutoff_cells)
-            return cls(1e-6, n, T=300, E_B=60 * kT * np.exp(0.05 * r.normal(0, 1, size=shape)), moment=2.37e-16,
-                       pattern=np.where(r.random(shape) < 0.5, -1.0, 1.0), seed=jseed,
-                       energies=[dip, hs.ZeemanEnergy(0.0, 0.0)], params=params)
+        spec = dict(lattice=lattice, n=n, scheme=scheme, seed=int(seed * 100000 + i), cutoff_cells=cutoff_cells)
         nx, ny = _lattice_shape(lattice, n)[::-1]
         n_sub = 6 if lattice == "IP_Kagome" else 2
         work = float(neel_work) if scheme == "Néel" else float(metro_sweeps)
         fp = footprint_bytes(nx, ny, n_sub) if cutoff_cells is None else \
             footprint_bytes(nx, ny, n_sub, kernel_full=False, kwx=cutoff_cells, kwy=cutoff_cells)
-        jobs.append(Job(make=make, fields=fields, work=work, arrival=t, footprint=fp, name=f"{lattice} n={n} {scheme}"))
+        jobs.append(Job(make=lambda spec=spec: served_instance(spec), fields=fields, work=work, arrival=t,
+                        footprint=fp, name=f"{lattice} n={n} {scheme}", spec=spec))
     return jobs
+
+
+def served_inputs(spec):
+    """Arrays and parameters of one served job's instance (spec from served_jobs): E_B lognormal
+    (median 60 kT, sigma 5 %) and a random +-1 start from the job's seed, a = 1 um, moment
+    2.37e-16, T = 300 K, the dipolar cutoff (None = full kernels) in cell pitches."""
+    r = np.random.default_rng(spec["seed"])
+    kT = kB * 300
+    shape = _lattice_shape(spec["lattice"], spec["n"])
+    dx = 1e-6 / 4 if spec["lattice"] == "IP_Kagome" else 1e-6 / 2
+    cut = spec.get("cutoff_cells")
+    return dict(a=1e-6, T=300, moment=2.37e-16, E_B=60 * kT * np.exp(0.05 * r.normal(0, 1, size=shape)),
+                pattern=np.where(r.random(shape) < 0.5, -1.0, 1.0),
+                cutoff=None if cut is None else edge_cutoff(cut * dx), reduced_k=cut)
+
+
+def served_instance(spec):
+    """The Magnets of one served job (a Zeeman energy whose field the job's steps set)."""
+    import hotspice_amd as hs
+    p = served_inputs(spec)
+    dip = hs.DipolarEnergy() if p["cutoff"] is None else hs.DipolarEnergy(cutoff_range=p["cutoff"])
+    params = hs.SimParams(UPDATE_SCHEME=spec["scheme"]) if p["reduced_k"] is None else \
+        hs.SimParams(UPDATE_SCHEME=spec["scheme"], REDUCED_KERNEL_SIZE=p["reduced_k"])
+    return getattr(hs.ASI, spec["lattice"])(p["a"], spec["n"], T=p["T"], E_B=p["E_B"], moment=p["moment"],
+                                            pattern=p["pattern"], seed=spec["seed"],
+                                            energies=[dip, hs.ZeemanEnergy(0.0, 0.0)], params=params)
 
 
 def _lattice_shape(lattice, n):

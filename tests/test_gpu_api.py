@@ -387,10 +387,12 @@ def test_metropolis_update_runs_the_periodic_recompute():
     n = 24
     rng = np.random.default_rng(4)
     pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    # the full kernel in both update_multiple branches (REDUCED_KERNEL_SIZE = 0), so incremental
+    # updates equal a recompute and the periodic one is due
     mm = hs.ASI.OOP_Square(1e-6, n, T=300, moment=1.1e-15, pattern=pat, seed=4, energies=[hs.DipolarEnergy()],
-                           params=hs.SimParams(UPDATE_SCHEME='Metropolis'), recompute_every=5)
+                           params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=0), recompute_every=5)
     om = model.make_magnets("OOP_Square", 1e-6, n=n, T=300, moment=1.1e-15, pattern=pat, seed=4,
-                            energies=[model.Dipolar()], params=dict(UPDATE_SCHEME="Metropolis"))
+                            energies=[model.Dipolar()], params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0))
     r0 = mm.recomputes
     for _ in range(10):
         np.testing.assert_array_equal(mm.update(r=4e-6), om.update(r=4e-6))
