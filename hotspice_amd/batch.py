@@ -87,19 +87,30 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None, updates=False)
     begin=True the launch starts each instance's progress call (its origin t0, attempts0 := the
     current t, attempts); otherwise the origin must have been set. updates=True runs max_steps
     update(t_max) calls instead of a progress() call: every step runs, whatever t has reached
-    (also +inf), and compares its waiting time with t_max itself (core.py:702). Afterwards checks each
-    instance's error word, counts its steps towards the periodic FP64 recompute and runs that
-    when due. `full[i]`: item i is known to run all its max_steps (no stop predicate can end it
-    earlier); when those steps complete its recompute period, the recompute is queued right
-    behind the launch instead of after reading the states back (the same recompute at the same
-    point of the trajectory)."""
+    (also +inf), and compares its waiting time with t_max itself (core.py:702). Afterwards checks
+    each instance's error word (raising on the first), counts its steps towards the periodic FP64
+    recompute and runs that when due. `full[i]`: item i is known to run all its max_steps (no
+    stop predicate can end it earlier); when those steps complete its recompute period, the
+    recompute is queued right behind the launch instead of after reading the states back (the
+    same recompute at the same point of the trajectory)."""
     if not items:
         return
+    queued = enqueue_neel(items, attempt_freq, begin, free=[2 if updates else 0] * len(items), full=full)
+    sts, errs = collect([it[0] for it in items], recomputed={id(m) for m in queued})
+    _raise_first(errs, "Neel")
+    return sts
+
+
+def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
+    """The launches of launch_neel on the current stream, without reading anything back.
+    free[i]: hs_run_t.first_free of item i (0: a progress() call, 2: update() calls). Returns the
+    instances whose recompute was queued behind the launch (see launch_neel's `full`)."""
     mags = [it[0] for it in items]
     dev = mags[0]._dev
     for m in mags:
         if not m._uniform_beta:
             raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
+    free = [0] * len(items) if free is None else list(free)
     # one launch per E_perp setting: the team kernel is compiled for either, and instances are
     # independent, so two launches in stream order step each exactly as one would
     perp = [bool(m.USE_PERP_ENERGY) for m in mags]
@@ -107,7 +118,7 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None, updates=False)
         g = [i for i, p in enumerate(perp) if p == flag]
         teams = _team_array([mags[i] for i in g], [items[i][1] for i in g], [items[i][2] for i in g],
                             [items[i][3] for i in g], attempt_freq, begin, nblocks=[items[i][4] for i in g],
-                            first_free=2 if updates else 0)
+                            first_free=[free[i] for i in g])
         dev.run_teams(teams, "neel")
     queued = []
     if full is not None:
@@ -115,7 +126,7 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None, updates=False)
                   if f and m.recompute_every and m._since_recompute + steps >= m.recompute_every]
         if queued:
             dev.recompute_many([m._dev for m in queued], [3 if m.USE_PERP_ENERGY else 1 for m in queued])
-    return _after_launch(mags, "Neel", recomputed={id(m) for m in queued})
+    return queued
 
 
 _TEAM_DTYPE = np.dtype(_lib.Team)
@@ -134,7 +145,7 @@ def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=
     run['attempt_freq'] = float(attempt_freq)
     run['max_steps'] = np.asarray(steps, dtype=np.int64)
     run['begin'] = int(begin)
-    run['first_free'] = int(first_free)
+    run['first_free'] = np.asarray(first_free, dtype=np.int32)
     if nblocks is not None:
         arr['nblocks'] = np.asarray(nblocks, dtype=np.int32)
     return arr
@@ -147,22 +158,37 @@ def launch_metropolis(items, attempt_freq=1e10, begin=False):
     if not items:
         return
     mags = [it[0] for it in items]
-    dev = mags[0]._dev
+    before = [st.attempts for st in read_states(mags)]
+    enqueue_metropolis(items, attempt_freq, begin)
+    sts, errs = collect(mags, before)
+    _raise_first(errs, "Metropolis")
+    return sts
+
+
+def enqueue_metropolis(items, attempt_freq=1e10, begin=False):
+    """The launches of launch_metropolis on the current stream, without reading anything back."""
+    mags = [it[0] for it in items]
     teams = _team_array(mags, [it[2] for it in items], [it[3] for it in items], [it[4] for it in items],
                         attempt_freq, begin, sels=[it[1] for it in items])
-    before = [st.attempts for st in read_states(mags)]
-    dev.run_teams(teams, "metropolis")
-    return _after_launch(mags, "Metropolis", before)
+    mags[0]._dev.run_teams(teams, "metropolis")
 
 
-def _after_launch(mags, what, before=None, recomputed=()):
-    """Checks errors, records samples per step and runs due recomputes (except those already
-    queued behind the launch, `recomputed`: ids of instances); returns the states."""
+def _raise_first(errs, what):
+    for i, e in errs.items():
+        raise _lib.HotspiceError(f"device {what} step failed (err={e})")
+
+
+def collect(mags, before=None, recomputed=()):
+    """After a launch: reads the states back, records samples per step (`before`: attempts before
+    the launch) and runs the recomputes that are due (except those already queued behind the
+    launch, `recomputed`: ids of instances). Returns (states, {index: device error word}) -- an
+    instance with an error is left as it is (no recompute), so its caller can drop it."""
     sts = read_states(mags)
-    due = []
+    due, errs = [], {}
     for i, (m, st) in enumerate(zip(mags, sts)):
         if st.err:
-            raise _lib.HotspiceError(f"device {what} step failed (err={st.err})")
+            errs[i] = int(st.err)
+            continue
         if before is not None and st.steps_done > 0:  # samples per step, to size the next launch
             m._metro_rate = max(1.0, (st.attempts - before[i]) / st.steps_done)
         m._since_recompute += int(st.steps_done)
@@ -177,7 +203,7 @@ def _after_launch(mags, what, before=None, recomputed=()):
         for m in due:
             m._since_recompute = 0
             m.recomputes += 1
-    return sts
+    return sts, errs
 
 
 def neel_chunk(m, start, t_max, mc, chunk=None, st=None):

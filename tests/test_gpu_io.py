@@ -167,3 +167,56 @@ def test_perp_inputter_relax_minimizes_under_each_field():
             b.minimize()
     _same(_state(a), _state(b))
     assert a.switches > 0
+
+
+def _oracle_twin(mm, scheme):
+    """The FP64 oracle instance of a test lattice built above (same pattern, E_B, seed)."""
+    from oracle import model
+    lat = type(mm).__name__
+    return model.make_magnets(lat, 1e-6, n=mm.nx, T=300, E_B=np.asarray(mm.E_B), moment=float(np.max(mm.moment)),
+                              pattern=np.asarray(mm.m), seed=mm.seed,
+                              energies=[model.Dipolar(), model.Zeeman(0.0, 0.0)],
+                              params=dict(UPDATE_SCHEME="Neel" if scheme == "Néel" else "Metropolis",
+                                          REDUCED_KERNEL_SIZE=mm.params.REDUCED_KERNEL_SIZE))
+
+
+def _play_on_oracle(om, steps):
+    """io.run_program's semantics on the oracle: set_field, then the step's advance."""
+    z = om.get_energy('zeeman')
+    for st in steps:
+        z.set_field(st.magnitude, st.angle)
+        if st.advance == 'progress':
+            om.progress(t_max=st.t_max, MCsteps_max=st.MCsteps_max)
+        elif st.advance == 'update':
+            for _ in range(st.count):
+                om.update(t_max=st.t_max)
+
+
+@pytest.mark.parametrize("which", ["clocking", "field_full", "chess_steps"])
+def test_inputter_programs_match_oracle(which):
+    """Field programs of the inputters (io.py:221-239, 468-481; examples/Clocking_IP_Pinwheel.py:
+    43-55) played on the device (io.run_program) and on the FP64 oracle: after every input the
+    same spins and counters, fields within 1e-5 of their scale, time within 1e-4 relative."""
+    from hotspice_amd import io
+    if which == "chess_steps":
+        mm = _oop(n=24, seed=12)
+        inp = io.OOPSquareChessStepsInputter(io.RandomBinaryDatastream(), magnitude=2e-3, n=0.5)
+        scheme = "Metropolis"
+    else:
+        mm = _pinwheel(n=20, seed=11)
+        scheme = "Néel"
+        inp = io.ClockingFieldInputter(io.RandomBinaryDatastream(), magnitude=1.5e-3, angle=math.pi / 4, n=0.5,
+                                       frequency=1e6) if which == "clocking" else \
+            io.FieldInputter(io.ConstantDatastream(1.0), magnitude=2e-3, angle=0.3, n=0.3, frequency=1e6,
+                             half_period=False)
+    om = _oracle_twin(mm, scheme)
+    scale = float(np.abs(om.E).max())
+    for v in [0, 1, 1, 0, 1]:
+        steps = inp.program(mm, v)
+        io.run_program(mm, steps)
+        _play_on_oracle(om, steps)
+        np.testing.assert_array_equal(mm.m, om.m)
+        assert mm.attempted_switches == om.attempted_switches and mm.switches == om.switches
+        np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-5 * scale)
+        assert mm.t == pytest.approx(om.t, rel=1e-4)
+    assert mm.switches > 0

@@ -104,3 +104,80 @@ def test_served_jobs_complete_after_time_overflow():
     # progress() takes no step (t - t0 is NaN)
     m = done["metro"].result
     assert math.isinf(m["t"]) and 0 < m["attempts"]
+
+
+def _corrupted(spec, field, value):
+    """make() of a served job whose instance descriptor gets `field` = `value` after every refill:
+    a host-side error the library's descriptor check rejects before it launches anything."""
+    from hotspice_amd.workloads import served_instance
+
+    def make():
+        mm = served_instance(spec)
+        refill = mm._device_params_changed
+
+        def corrupt_refill():
+            refill()
+            setattr(mm._dev.inst, field, value)
+
+        mm._device_params_changed = corrupt_refill
+        corrupt_refill()
+        return mm
+    return make
+
+
+def test_failing_jobs_are_isolated():
+    """A Neel job whose descriptor has beta64 = 0 (rejected by the batched Neel launch) and a
+    Metropolis job with n_occ = 0 (rejected at its first field swap) fail with their errors; the
+    other jobs of the mix end exactly as each does served alone."""
+    from hotspice_amd.workloads import served_jobs
+
+    def make_jobs():
+        jobs = served_jobs(count=6, rate=800.0, seed=7, n_range=(16, 40), field_steps=3, neel_work=100,
+                           metro_sweeps=1.0)
+        neel = next(j for j in jobs if j.spec["scheme"] == "Néel")
+        metro = next(j for j in jobs if j.spec["scheme"] == "Metropolis")
+        neel.make, neel.name = _corrupted(neel.spec, "beta64", 0.0), "bad beta64"
+        metro.make, metro.name = _corrupted(metro.spec, "n_occ", 0), "bad n_occ"
+        return jobs
+
+    done, srv = _serve(make_jobs())
+    assert sorted(j.name for j in srv.failed) == ["bad beta64", "bad n_occ"]
+    errors = {j.name: j.error for j in srv.failed}
+    assert "beta64" in errors["bad beta64"] and "n_occ" in errors["bad n_occ"]
+    assert len(done) == 4 and srv.in_use == 0
+    for job in make_jobs():
+        if job.name.startswith("bad"):
+            continue
+        job.arrival = 0.0
+        (alone,), _ = _serve([job])
+        (mixed,) = [j for j in done if j.name == job.name and j.spec == job.spec]
+        r, a = mixed.result, alone.result
+        np.testing.assert_array_equal(r["m"], a["m"])
+        assert (r["t"], r["attempts"], r["switches"], r["E_tot"]) == (a["t"], a["attempts"], a["switches"], a["E_tot"])
+
+
+def test_served_inputter_program_equals_run_alone():
+    """A clocking inputter's field program (io.ClockingFieldInputter.schedule: progress() steps
+    with a finite t_max) served next to other jobs equals io.run_program of the same steps on
+    the instance alone, bit-exact."""
+    import math
+
+    from hotspice_amd import io
+    from hotspice_amd.serve import Job, Server
+    from hotspice_amd.workloads import served_jobs, served_instance
+    spec = dict(lattice="IP_Pinwheel", n=24, scheme="Néel", seed=99, cutoff_cells=None)
+    inp = io.ClockingFieldInputter(io.RandomBinaryDatastream(), magnitude=1.5e-3, angle=math.pi / 4, n=0.5,
+                                   frequency=1e6)
+    program = inp.schedule(served_instance(spec), [0, 1, 1, 0])
+    assert len(program) == 8 and all(st.advance == 'progress' for st in program)
+    others = served_jobs(count=4, rate=800.0, seed=8, n_range=(16, 32), field_steps=3, neel_work=80, metro_sweeps=0.5)
+    srv = Server()
+    for j in others + [Job(make=lambda: served_instance(spec), program=program, name="clocking")]:
+        srv.submit(j)
+    done = {j.name: j for j in srv.run(timeout=300)}
+    ref = served_instance(spec)
+    io.run_program(ref, program)
+    r = done["clocking"].result
+    np.testing.assert_array_equal(r["m"], ref.m)
+    assert (r["t"], r["attempts"], r["switches"]) == (ref.t, ref.attempted_switches, ref.switches)
+    assert r["switches"] > 0
