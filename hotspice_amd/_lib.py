@@ -107,6 +107,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise HotspiceError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # PyTorch-ROCm brings its own HIP runtime under the system runtime's soname (libamdhip64.so.7).
+    # Loaded after torch, the library binds torch's copy. Loaded before it, the library would pull
+    # in the system HIP and HSA runtimes and torch a second pair: two runtimes in one process, and
+    # the one that initialises second finds no device.
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     vp, i32, i64, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
     sig = {
