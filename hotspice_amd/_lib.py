@@ -23,7 +23,10 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_metropolis_run", "hs_metropolis_ws_bytes", "hs_neel_run_teams", "hs_neel_teams_ws_bytes",
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
            "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
-           "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes")
+           "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes",
+           "hs_metropolis_run_static")
+
+HS_METRO_STATIC_SCRATCH_BYTES = 4 << 20
 
 
 class Instance(C.Structure):
@@ -130,6 +133,7 @@ def load():
         "hs_neel_run": ([C.POINTER(Instance), C.POINTER(Run), vp, vp, vp, vp, C.POINTER(i32), vp], C.c_int),
         "hs_metropolis_run": ([C.POINTER(Instance), C.POINTER(Select), C.POINTER(Run), vp, vp, vp, vp, vp], C.c_int),
         "hs_metropolis_ws_bytes": ([C.POINTER(Instance), C.POINTER(Select)], C.c_int64),
+        "hs_metropolis_run_static": ([C.POINTER(Instance), C.POINTER(Select), C.POINTER(Run), vp, vp, vp, vp], C.c_int),
         "hs_neel_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),
         "hs_neel_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
         "hs_metro_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),

@@ -26,6 +26,7 @@
 
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "hs_device.h"
@@ -1711,6 +1712,69 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
     HS_LAUNCH_CHECK("k_metro_update");
   }
   return HS_OK;
+}
+
+// ---- the allocation-free variant -----------------------------------------------------------
+// hs_metropolis_run_static takes its scratch (the MetroWS arrays and the worklist) from this
+// file-scope array instead of a caller's workspace. Every HIP device the library runs on holds
+// its own copy at its own address; host code reaches it only through the address that
+// hipGetSymbolAddress reports on that device (the array's name in host code is not a device
+// address).
+static __device__ __align__(256) unsigned char scratch[HS_METRO_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up on the first call there.
+int static_scratch(void **out) {
+  int dev = 0;
+  HS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return hs_set_error(HS_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return hs_set_error(HS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return HS_OK;
+}
+
+}  // namespace
+
+// Zeroes n16 16-byte words (a kernel rather than hipMemsetAsync, so the call can be captured).
+__global__ void k_zero16(uint4 *p, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+extern "C" int hs_metropolis_run_static(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
+                                        hs_state_t *state, int32_t *log_cells, int32_t *log_count, void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!sel) return hs_set_error(HS_ERR_ARG, "missing sel");
+  const int64_t need = hs_metropolis_ws_bytes(inst, sel);
+  if (need < 0) return hs_set_error(HS_ERR_ARG, "bad selection geometry");
+  if (need > (int64_t)HS_METRO_STATIC_SCRATCH_BYTES)
+    return hs_set_error(HS_ERR_ARG,
+                        "this instance and selection need %lld scratch bytes; the static scratch holds %lld "
+                        "(use hs_metropolis_run with a workspace)",
+                        (long long)need, (long long)HS_METRO_STATIC_SCRATCH_BYTES);
+  void *ws = nullptr;
+  rc = static_scratch(&ws);
+  if (rc) return rc;
+  // the scratch keeps the last call's contents: start from zeros, as a fresh workspace does
+  const int64_t n16 = (need + 15) / 16;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n16 + 255) / 256, 1024);
+  k_zero16<<<blocks, 256, 0, (hipStream_t)stream>>>((uint4 *)ws, n16);
+  HS_LAUNCH_CHECK("k_zero16");
+  return hs_metropolis_run(inst, sel, run, state, ws, log_cells, log_count, stream);
 }
 
 namespace {

@@ -330,6 +330,23 @@ int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const h
 /* Scratch bytes hs_metropolis_run needs for this instance and selection geometry. */
 int64_t hs_metropolis_ws_bytes(const hs_instance_t *inst, const hs_select_t *sel);
 
+/* Size of the static scratch of hs_metropolis_run_static (per device). */
+#define HS_METRO_STATIC_SCRATCH_BYTES (4 << 20)
+
+/*
+ * hs_metropolis_run without a workspace argument and without any allocation: the scratch comes
+ * from a file-scope device array of HS_METRO_STATIC_SCRATCH_BYTES bytes that each device holds
+ * until the process ends (its address is looked up once per device, on the first call there).
+ * Same arguments otherwise and bit-identical results. Calls whose hs_metropolis_ws_bytes exceeds
+ * that size return HS_ERR_ARG. Every call on a device shares that device's scratch and clears
+ * what it uses first, so calls that overlap on one device (on concurrent streams, or from
+ * several threads) race; serialise them. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Non-blocking.
+ */
+int hs_metropolis_run_static(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
+                             hs_state_t *state, int32_t *log_cells, int32_t *log_count, void *stream);
+
 /*
  * One zero-temperature sweep: Magnets._minimize_all (core.py:785-806), the step of
  * Magnets.relax (core.py:808-845). For each subset index i of order[0..n_order) (host array, the
