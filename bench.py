@@ -17,6 +17,11 @@ Other workloads (one JSON line each, same keys):
   served      continuous batching of clocking jobs (Poisson arrivals, Kagome / pinwheel,
               Neel / Metropolis); reports jobs/s and p50/p99 latency beside attempts/s
 
+The default run (`--workload all`) measures the plain Neel line above as `value` and adds, timed
+in the same run, a `secondary` object with the other configurations of the metric: metro2048
+(plain, large), batched (with `vs_plain`) and served, at the offered 200 jobs/s and at a rate
+below what the device sustains -- each with its own `roofline` and `cpu_baseline`.
+
 Prints ONE JSON line. `roofline` is for the dominant kernel: algorithmic bytes per launch /
 average launch duration, measured with HIP events recorded on the launch stream around every
 launch in the timed region. `cpu_baseline` times the FP64 NumPy restatement of the reference on
@@ -43,18 +48,21 @@ PERIOD = 1024
 # commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
 NEEL_TEAM_TRAFFIC_PER_LAUNCH = 96.3e6
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
-WORKLOADS = ("neel", "metro64", "metro2048", "batched", "neel_batched", "served")
+WORKLOADS = ("all", "neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=WORKLOADS, default="neel")
+    p.add_argument("--workload", choices=WORKLOADS, default="all")
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--jobs", type=int, default=32, help="served: number of jobs")
+    p.add_argument("--window", type=int, default=None,
+                   help="served: dipolar cutoff in cell pitches with the matching REDUCED_KERNEL_SIZE (default: the "
+                        "full kernels and default SimParams of the served configuration)")
     p.add_argument("--field-steps", type=int, default=100, help="served: field steps per job")
     p.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -79,7 +87,7 @@ def base_line(args, value, steps, warmup, wall, dtype, config, unit="switch atte
 
 
 def roofline(kernel, launches, bytes_total, extra=None):
-    us = [e0.elapsed_time(e1) * 1e3 for e0, e1, _ in launches]
+    us = [e0.elapsed_time(e1) * 1e3 for e0, e1, *_ in launches]
     avg = sum(us) / len(us)
     per_launch = bytes_total / len(us)
     achieved = per_launch / (avg * 1e-6) / 1e9
@@ -201,6 +209,36 @@ def cpu_baseline_metro(make, kw, seconds, label):
                 sample=f"{n} Metropolis attempts of {label} in {dt:.1f} s, FP64 NumPy, 1 thread ({cores_note()})")
 
 
+def cpu_baseline_metro2048(seconds):
+    """The 2048x2048 workload in the FP64 NumPy restatement, one core: full (4095^2) kernels as the
+    reference builds them, and its own update_multiple branch rule, so a step with more than 20
+    switches per sublattice convolves the whole grid's switch mask with the 41x41 window
+    (energies.py:343-361) -- about 13 s per step on one core. At least one step, then steps until
+    `seconds` have passed."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import model
+    from hotspice_amd.workloads import edge_cutoff, kB, oop_large_inputs
+    p = oop_large_inputs()
+    kT = kB * p["T"]
+    t_build = time.perf_counter()
+    om = model.make_magnets("OOP_Square", p["a"], n=p["n"], T=p["T"], E_B=p["E_B"], moment=p["moment"],
+                            pattern=p["pattern"], energies=[model.Dipolar(cutoff_range=edge_cutoff(20 * p["a"])),
+                                                            model.Exchange(J=0.5 * kT)],
+                            params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=20), seed=1, rng="numpy")
+    t_build = time.perf_counter() - t_build
+    a0 = om.attempted_switches
+    t0 = time.perf_counter()
+    steps = 0
+    while steps == 0 or time.perf_counter() - t0 < seconds:
+        om.update(r=20 * p["a"])
+        steps += 1
+    dt = time.perf_counter() - t0
+    n = om.attempted_switches - a0
+    return dict(value=n / dt, unit="switch attempts/s", cores=1, kind="port",
+                sample=f"{steps} Metropolis steps ({n} attempts) of the same 2048x2048 workload in {dt:.1f} s, FP64 "
+                       f"NumPy with full kernels (built in {t_build:.0f} s), 1 thread ({cores_note()})")
+
+
 def bench_metro(args, large):
     import numpy as np
     import torch
@@ -252,7 +290,7 @@ def bench_metro(args, large):
     line["roofline"] = roofline(kname, launches, bytes_total)
     if not args.no_cpu:
         if large:
-            line["cpu_baseline"] = None
+            line["cpu_baseline"] = cpu_baseline_metro2048(args.cpu_seconds)
         else:
             def make():
                 p = oop_metropolis_inputs()
@@ -399,28 +437,177 @@ def bench_neel_batched(args):
     return line
 
 
-def bench_served(args):
+def _cpu_served_worker(specs, fields, works, counter, ready):
+    """One CPU worker of the served baseline: the FP64 NumPy restatement of its jobs (at most two),
+    their field steps played round-robin -- set_field, then 1000 update(t_max=inf) calls (Neel) or
+    progress(t_max=inf, MCsteps_max=4) as its update() steps (Metropolis) -- until the parent
+    stops it. After every step it adds the step's attempts to `counter`, so the parent reads the
+    attempts completed within its window (a step cut off by the stop counts nothing)."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import model
+    from hotspice_amd.workloads import served_inputs
+    mags = []
+    for sp in specs:
+        p = served_inputs(sp)
+        dip = model.Dipolar(recompute="fft") if p["cutoff"] is None else model.Dipolar(cutoff_range=p["cutoff"])
+        params = dict(UPDATE_SCHEME="Neel" if sp["scheme"] == "Néel" else "Metropolis")
+        if p["reduced_k"] is not None:
+            params["REDUCED_KERNEL_SIZE"] = p["reduced_k"]
+        mags.append(model.make_magnets(sp["lattice"], p["a"], n=sp["n"], T=p["T"], E_B=p["E_B"], moment=p["moment"],
+                                       pattern=p["pattern"], energies=[dip, model.Zeeman(0.0, 0.0)], params=params,
+                                       seed=sp["seed"], rng="numpy"))
+    ready.release()
+    k = 0
+    while True:
+        for m, sp, work in zip(mags, specs, works):
+            mag, ang = fields[k % len(fields)]
+            m.get_energy("zeeman").set_field(mag, ang)
+            mc0 = m.MCsteps
+            steps = 0
+            while (steps < work) if sp["scheme"] == "Néel" else model.lower_than(m.MCsteps - mc0, work):
+                a = m.attempted_switches
+                m.update(t_max=math.inf) if sp["scheme"] == "Néel" else m.update()
+                steps += 1
+                with counter.get_lock():
+                    counter.value += m.attempted_switches - a
+        k += 1
+
+
+def cpu_baseline_served(jobs, seconds):
+    """The served jobs on the host the way the reference runs jobs (ParallelJobs.py:64): P = physical
+    cores (at most 16) independent processes, each with at most two of the jobs. Once every worker
+    has built its instances, the attempts completed in the next `seconds` are counted and the
+    workers are stopped; aggregate attempts/s."""
+    import multiprocessing as mp
+    import psutil
+    ctx = mp.get_context("spawn")
+    P = max(1, min(16, psutil.cpu_count(logical=False) or 1, len(jobs)))
+    shares = [jobs[i::P][:2] for i in range(P)]
+    counter, ready = ctx.Value("q", 0), ctx.Semaphore(0)
+    procs = [ctx.Process(target=_cpu_served_worker, daemon=True,
+                         args=([j.spec for j in sh], sh[0].fields, [j.work for j in sh], counter, ready))
+             for sh in shares]
+    for p in procs:
+        p.start()
+    try:
+        for _ in procs:
+            if not ready.acquire(timeout=600):
+                raise RuntimeError("a served CPU-baseline worker did not build its instances within 600 s")
+        with counter.get_lock():
+            c0 = counter.value
+        t0 = time.perf_counter()
+        time.sleep(seconds)
+        with counter.get_lock():
+            n = counter.value - c0
+        dt = time.perf_counter() - t0
+    finally:
+        for p in procs:
+            p.terminate()
+        for p in procs:
+            p.join(timeout=30)
+    return dict(value=n / dt, unit="switch attempts/s", cores=P, kind="port",
+                sample=f"{n} attempts completed in {dt:.0f} s by {P} processes x 1 thread, each stepping up to two of "
+                       f"the jobs ({sum(len(sh) for sh in shares)} in all) through their field steps, FP64 NumPy "
+                       f"({cores_note()})")
+
+
+def serve_once(jobs):
+    """Serves `jobs` on a fresh server; returns (done jobs, server, wall seconds, launch events)."""
     import torch
-    from hotspice_amd.serve import Server, latency_stats
-    from hotspice_amd.workloads import served_jobs
-    jobs = served_jobs(count=args.jobs, rate=args.rate, field_steps=args.field_steps, cutoff_cells=10)
+    from hotspice_amd._device import DeviceState
+    from hotspice_amd.serve import Server
     srv = Server()
     for j in jobs:
         srv.submit(j)
+    DeviceState.launch_events = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = srv.run(timeout=900)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    launches, DeviceState.launch_events = DeviceState.launch_events, None
+    return done, srv, wall, launches
+
+
+def served_roofline(done, launches):
+    """Roofline of the served Neel launches (k_neel_team, every Neel job's team in them): the
+    algorithmic bytes of the Neel jobs' attempts (full kernels: the update reads the whole grid)
+    over the summed launch durations."""
+    from hotspice_amd.workloads import _lattice_shape, neel_bytes_per_attempt
+    neel = [l for l in launches if l[3] == "neel"]
+    if not neel:
+        return None
+    total = 0.0
+    for j in done:
+        if j.spec["scheme"] != "Néel":
+            continue
+        ny, nx = _lattice_shape(j.spec["lattice"], j.spec["n"])
+        n_occ = j.result["n"]
+        w = j.spec.get("cutoff_cells")
+        win = n_occ if w is None else min(n_occ, int((2 * w + 1) ** 2 * n_occ / (nx * ny)))
+        total += j.result["attempts"] * neel_bytes_per_attempt(n_occ, win, perp=True)
+    return roofline("k_neel_team (served Neel jobs)", neel, total)
+
+
+def bench_served(args, rate=None, count=None, cpu=True):
+    from hotspice_amd.serve import latency_stats
+    from hotspice_amd.workloads import served_jobs
+    rate = args.rate if rate is None else rate
+    count = args.jobs if count is None else count
+    jobs = served_jobs(count=count, rate=rate, field_steps=args.field_steps, cutoff_cells=args.window)
+    done, srv, wall, launches = serve_once(jobs)
     st = latency_stats(done)
+    win = "full kernels (default SimParams)" if args.window is None else \
+        f"dipolar window {2 * args.window + 1}x{2 * args.window + 1} (cutoff {args.window} cells)"
     line = base_line(args, srv.attempts / wall, srv.iterations, 0, wall, "f32+kahan (fields), f64 (recompute, time)",
-                     {"workload": f"served: {args.jobs} clocking jobs, Poisson arrivals at {args.rate:g}/s, 50% IP_Kagome "
+                     {"workload": f"served: {count} clocking jobs, Poisson arrivals at {rate:g}/s, 50% IP_Kagome "
                                   f"/ 50% IP_Pinwheel, n log-uniform 64..1024, 50% Neel (1000 attempts per field step) / "
                                   f"50% Metropolis (4 MC sweeps per field step), {args.field_steps} field steps "
-                                  "(1.5 mT (1-cos), 0/45 deg), dipolar window 21x21 (cutoff 10 cells)",
-                      "jobs": len(done), "jobs_per_s": len(done) / wall, "latency_s": st,
-                      "attempts": srv.attempts, "parallelism": "single GPU"})
-    line["roofline"] = None
+                                  f"(1.5 mT (1-cos), 0/45 deg), {win}",
+                      "jobs": len(done), "failed": len(srv.failed), "jobs_per_s": len(done) / wall,
+                      "offered_jobs_per_s": rate, "latency_s": st, "attempts": srv.attempts, "parallelism": "single GPU"})
+    line["roofline"] = served_roofline(done, launches)
+    line["cpu_baseline"] = cpu_baseline_served(jobs, args.cpu_seconds / 2) if cpu and not args.no_cpu else None
+    return line
+
+
+def _secondary(line):
+    """The part of a workload's line that the combined line carries for it."""
+    keep = ("value", "unit", "ms_per_step", "steps", "roofline", "cpu_baseline")
+    out = {k: line.get(k) for k in keep}
+    out.update(line["config"])
+    return out
+
+
+def bench_all(args):
+    """The plain Neel line (`value`), then the other configurations of the metric, each timed in
+    this run with its own defaults: plain 2048x2048 Metropolis, the 64-instance batch with its
+    plain-path comparison, and the served mix at the offered 200 jobs/s and at half the rate the
+    device sustained there."""
+    t0 = time.perf_counter()
+
+    def progress(what):
+        print(f"bench: {what} done at {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+
+    line = bench_neel(args)
+    progress("plain Neel")
+    sub = argparse.Namespace(**vars(args))
+    sub.steps = None
+    sec = {}
+    sec["metro2048"] = _secondary(bench_metro(sub, True))
+    progress("plain Metropolis 2048x2048")
+    sec["batched"] = _secondary(bench_batched(sub))
+    progress("batched")
+    served = bench_served(sub, rate=args.rate, count=args.jobs)
+    sec["served"] = _secondary(served)
+    progress("served")
+    sustain = served["config"]["jobs_per_s"]
+    low = max(0.05, round(0.5 * sustain, 3))
+    sec["served_below_saturation"] = _secondary(bench_served(sub, rate=low, count=max(8, args.jobs // 2), cpu=False))
+    progress("served below saturation")
+    sec["served_below_saturation"]["note"] = (f"offered {low:g} jobs/s, half of the {sustain:.3g} jobs/s the device "
+                                              f"sustained at {args.rate:g}/s offered")
+    line["secondary"] = sec
     return line
 
 
@@ -437,8 +624,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    fn = {"neel": bench_neel, "metro64": lambda a: bench_metro(a, False), "metro2048": lambda a: bench_metro(a, True),
-          "batched": bench_batched, "neel_batched": bench_neel_batched, "served": bench_served}[args.workload]
+    fn = {"all": bench_all, "neel": bench_neel, "metro64": lambda a: bench_metro(a, False),
+          "metro2048": lambda a: bench_metro(a, True), "batched": bench_batched, "neel_batched": bench_neel_batched,
+          "served": bench_served}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -253,23 +253,24 @@ class DeviceState:
             self.team_ws = self.torch.zeros(max(need, 256), dtype=self.torch.uint8, device=self.device)
         ev = self._event_start()
         _lib.check(run_fn(arr, len(teams), self.ptr(self.team_ws), self.team_ws.numel(), self.stream()))
-        self._event_end(ev, states)
+        self._event_end(ev, states, kind)
 
     def _event_start(self):
         """Timing hook: when launch_events is a list, HIP events on the launch stream bracket the
-        library call and (start, end, state pointers) is appended to it."""
+        library call and (start, end, state pointers, kind) is appended to it. Setting
+        DeviceState.launch_events (the class attribute) collects the launches of every instance."""
         if self.launch_events is None:
             return None
         e0 = self.torch.cuda.Event(enable_timing=True)
         e0.record(self.torch.cuda.current_stream(self.device))
         return e0
 
-    def _event_end(self, e0, states):
+    def _event_end(self, e0, states, kind="metropolis"):
         if e0 is None:
             return
         e1 = self.torch.cuda.Event(enable_timing=True)
         e1.record(self.torch.cuda.current_stream(self.device))
-        self.launch_events.append((e0, e1, states))
+        self.launch_events.append((e0, e1, states, kind))
 
     def metropolis_run(self, sel: _lib.Select, run: _lib.Run, log_cells=None, log_count=None):
         need = int(self.lib.hs_metropolis_ws_bytes(C.byref(self.inst), C.byref(sel)))

@@ -244,7 +244,7 @@ class Server:
         a.job.finished = now
         a.job.status = "done"
         a.job.result = dict(m=a.mm.m.copy(), t=a.mm.t, attempts=a.mm.attempted_switches, switches=a.mm.switches,
-                            E_tot=a.mm.E_tot)
+                            E_tot=a.mm.E_tot, n=a.mm.n)
         self.in_use -= a.job.footprint
         self.done.append(a.job)
         a.mm.release_device()

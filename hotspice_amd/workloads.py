@@ -67,15 +67,22 @@ def oop_metropolis(n=64, seed=0):
                                                  REDUCED_KERNEL_SIZE=0))
 
 
-def oop_large(n=2048, seed=0):
-    import hotspice_amd as hs
+def oop_large_inputs(n=2048, seed=0):
     rng = np.random.default_rng(seed)
     a, T = 200e-9, 350
     kT = kB * T
     shape = (n, n)
-    return hs.ASI.OOP_Square(a, n, T=T, E_B=30 * kT * rng.normal(1, 0.05, size=shape), moment=1.1e-15,
-                             pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0), seed=seed,
-                             energies=[hs.DipolarEnergy(cutoff_range=edge_cutoff(20 * a)), hs.ExchangeEnergy(J=0.5 * kT)],
+    return dict(a=a, n=n, T=T, moment=1.1e-15, seed=seed, E_B=30 * kT * rng.normal(1, 0.05, size=shape),
+                pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0))
+
+
+def oop_large(n=2048, seed=0):
+    import hotspice_amd as hs
+    p = oop_large_inputs(n, seed)
+    kT = kB * p["T"]
+    return hs.ASI.OOP_Square(p["a"], n, T=p["T"], E_B=p["E_B"], moment=p["moment"], pattern=p["pattern"], seed=seed,
+                             energies=[hs.DipolarEnergy(cutoff_range=edge_cutoff(20 * p["a"])),
+                                       hs.ExchangeEnergy(J=0.5 * kT)],
                              params=hs.SimParams(UPDATE_SCHEME='Metropolis', REDUCED_KERNEL_SIZE=20))
 
 
