@@ -39,10 +39,18 @@ constexpr int MU_TX = 32, MU_TY = 8, MU_THREADS = MU_TX * MU_TY;
 constexpr int MAX_ATTEMPTS = 4096;
 // instances up to this many grid cells step in the persistent one-workgroup kernel
 constexpr int PERSIST_CELLS = 80 * 80;
-// workgroups of an update launch over the worklist (8 per CU of a 256-CU device), and the waves
-// per SIMD the update kernels are compiled for, so that all of them are resident at once
+// workgroups of an update launch over the worklist when the caller leaves it automatic
+// (hs_run_t.nblocks = 0; 8 per CU of a 256-CU device), and the waves per SIMD the update kernels
+// are compiled for, so that all of them are resident at once
 constexpr int UPDATE_WGS = 2048;
 constexpr int UPDATE_WAVES = 8;
+
+// Workgroups of an update launch over `ntiles` tiles: `requested` (hs_run_t.nblocks) when
+// positive, else UPDATE_WGS; never more than one per tile.
+inline unsigned update_grid(int64_t ntiles, int64_t requested) {
+  const int64_t g = requested > 0 ? requested : UPDATE_WGS;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, std::max<int64_t>(1, ntiles)));
+}
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
 // the selection marks tiles in an LDS bitmap of this many 32-bit words (jobs with more tiles list
@@ -1090,21 +1098,49 @@ __device__ __forceinline__ void metro_tiles(const MetroJob &J, WorkList WL, int 
     const int cs1 = min(SEL.ncols - 1, floor_div(x0 + MU_TX - 1 + Wx - ox, 2 * SEL.Rx));
     const int rs0 = max(0, -floor_div(-(y0 - Wy - oy - SEL.Ry + 1), 2 * SEL.Ry));
     const int rs1 = min(nrows - 1, floor_div(y0 + MU_TY - 1 + Wy - oy, 2 * SEL.Ry));
-    for (int rs = rs0; rs <= rs1 && !touched; ++rs)
-      for (int cs = cs0; cs <= cs1 && !touched; ++cs) {
-        const int j = rs * SEL.ncols + cs;
-        if (!ws.accf[j]) continue;
-        const int sc = ws.cand[j], sy = sc / nx, sx = sc % nx, ss = hs_site(I, sy, sx);
-        bool rel = sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1;  // the switch's own tile (its flip)
-        if (!rel && has_dip && I.sub_of[ss] >= 0) {
-          const bool tr = ws.trunc[ss] != 0;
-          rel = reaches(sx, tr ? tkx : I.kwx, nx, false, x0, x0 + MU_TX - 1) &&
-                reaches(sy, tr ? tky : I.kwy, ny, false, y0, y0 + MU_TY - 1);
-        }
-        if (!rel && has_ex)
-          rel = reaches(sx, hx, nx, false, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, false, y0, y0 + MU_TY - 1);
-        touched = rel;
+    const int ncs = max(0, cs1 - cs0 + 1), nq = ncs * max(0, rs1 - rs0 + 1);
+    // the accepted flags and cells of the first TILE_BATCH candidate supercells are loaded
+    // together (independent loads, not a chain) and then tested; any further ones in turn
+    constexpr int TILE_BATCH = 12;
+    int8_t af[TILE_BATCH];
+    int cc[TILE_BATCH];
+#pragma unroll
+    for (int q = 0; q < TILE_BATCH; ++q) {
+      af[q] = 0;
+      cc[q] = 0;
+      if (q < nq) {
+        const int j = (rs0 + q / ncs) * SEL.ncols + cs0 + q % ncs;
+        af[q] = ws.accf[j];
+        cc[q] = ws.cand[j];
       }
+    }
+    for (int q = 0; q < nq && !touched; ++q) {
+      int8_t acc_q = 0;
+      int sc = 0;
+      if (q < TILE_BATCH) {
+#pragma unroll
+        for (int u = 0; u < TILE_BATCH; ++u)
+          if (u == q) {
+            acc_q = af[u];
+            sc = cc[u];
+          }
+      } else {
+        const int j = (rs0 + q / ncs) * SEL.ncols + cs0 + q % ncs;
+        acc_q = ws.accf[j];
+        sc = ws.cand[j];
+      }
+      if (!acc_q) continue;
+      const int sy = sc / nx, sx = sc % nx, ss = hs_site(I, sy, sx);
+      bool rel = sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1;  // the switch's own tile (its flip)
+      if (!rel && has_dip && I.sub_of[ss] >= 0) {
+        const bool tr = ws.trunc[ss] != 0;
+        rel = reaches(sx, tr ? tkx : I.kwx, nx, false, x0, x0 + MU_TX - 1) &&
+              reaches(sy, tr ? tky : I.kwy, ny, false, y0, y0 + MU_TY - 1);
+      }
+      if (!rel && has_ex)
+        rel = reaches(sx, hx, nx, false, x0, x0 + MU_TX - 1) && reaches(sy, hy, ny, false, y0, y0 + MU_TY - 1);
+      touched = rel;
+    }
   }
   const unsigned long long ball = __ballot(touched);
   const int lane = threadIdx.x & 63;
@@ -1601,7 +1637,7 @@ extern "C" int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, in
   hipStream_t s = (hipStream_t)stream;
   k_flip_select<<<1, MS_THREADS, 0, s>>>(J, cells, k);
   HS_LAUNCH_CHECK("k_flip_select");
-  k_relax_update<<<(unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS), MU_THREADS, 0, s>>>(J);
+  k_relax_update<<<update_grid(nt, 0), MU_THREADS, 0, s>>>(J);
   HS_LAUNCH_CHECK("k_relax_update");
   return HS_OK;
 }
@@ -1652,7 +1688,7 @@ extern "C" int hs_wolff_run(const hs_instance_t *inst, const hs_run_t *run, int3
   for (int64_t j = 0; j < run->max_steps; ++j) {
     k_wolff_select<<<1, MS_THREADS, 0, s>>>(J, W);
     HS_LAUNCH_CHECK("k_wolff_select");
-    k_relax_update<<<(unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS), MU_THREADS, 0, s>>>(J);
+    k_relax_update<<<update_grid(nt, 0), MU_THREADS, 0, s>>>(J);
     HS_LAUNCH_CHECK("k_relax_update");
   }
   return HS_OK;
@@ -1688,7 +1724,7 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
   k_metro_clear<<<1, 64, 0, s>>>(WL);
   HS_LAUNCH_CHECK("k_metro_clear");
-  const unsigned wgs = (unsigned)(nt < UPDATE_WGS ? nt : UPDATE_WGS);
+  const unsigned wgs = update_grid(nt, run->nblocks);
   // grid selection without METROPOLIS_TIME and with more samples than one batch per selection
   // thread: the samples are drawn and accepted by a wide launch
   const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time && J.SEL.nsc > PRESAMPLE_MIN;
@@ -1901,7 +1937,9 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
     const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)L.ntiles))};
     k_metro_clear<<<1, 64, 0, s>>>(WL);
     HS_LAUNCH_CHECK("k_metro_clear");
-    const unsigned wgs = (unsigned)(L.ntiles < UPDATE_WGS ? L.ntiles : UPDATE_WGS);
+    int64_t req = 0;  // the largest update grid a team asks for (0: automatic)
+    for (int i : L.large) req = std::max<int64_t>(req, teams[i].run.nblocks);
+    const unsigned wgs = update_grid(L.ntiles, req);
     const int32_t *lj = (const int32_t *)(base + L.large_off);
     for (int64_t j = 0; j < steps; ++j) {
       const int par = (int)(j & 1);

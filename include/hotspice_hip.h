@@ -127,7 +127,8 @@ typedef struct hs_run {
   double attempt_freq;  /* f0 (core.py:659, 679) */
   int64_t max_steps;    /* step budget of this call */
   int32_t log;          /* 1: write per-step log arrays */
-  int32_t nblocks;      /* grid size for grid-wide kernels (0 = automatic) */
+  int32_t nblocks;      /* grid size for grid-wide kernels (0 = automatic); Metropolis: workgroups
+                           of the update launch over the step's tile worklist (0 = 2048) */
   int32_t begin;        /* 1: this call starts a progress() call (t0, attempts0 := t, attempts) */
   int32_t first_free;   /* 1: the call's first step runs whatever t and MCsteps are, as
                            Magnets.update steps unconditionally (core.py:659-718);
