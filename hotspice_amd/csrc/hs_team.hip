@@ -55,13 +55,28 @@ struct TeamDev {
   int32_t *log_idx;
   double *log_dt;
   TeamRec *rec;  // [3][G]: round k uses slot k % 3
+  // deferred waiting times (t_max = +inf launches; NULL: every step's time is decided in the
+  // step): adds[i] = what step i adds to t, written by the workgroup that holds the winner;
+  // dsl[3]: a step whose time must be known at once (see k_neel_team) publishes it there;
+  // done: workgroups that have finished the launch
+  double *adds;
+  TeamRec *dsl;
+  unsigned int *done;
   int32_t G, first;
 };
+
+// Launches with deferred waiting times hold at most this many steps.
+constexpr int64_t ADD_CAP = 1 << 16;
 
 HS_INLINE uint32_t key_bits(float k) {
   uint32_t u = __float_as_uint(k);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+
+// The key whose bits key_bits gave.
+HS_INLINE float key_value(uint32_t b) { return __uint_as_float((b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b); }
+
+
 
 // FP32 log-rate of a cell: -b*beta (one channel) or log(exp(-b1 beta) + exp(-b2 beta)); the
 // same expression as the per-step launches use, so both paths rank cells identically.
@@ -97,9 +112,12 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
   __shared__ unsigned long long s_pc[OW / 64];
-  __shared__ float s_in[OW / 64][8];  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each owner wave's best
+  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each owner wave's best, by step parity (a deferred
+  // waiting time reads step s's while the owners write step s+1's)
+  __shared__ float s_in[2][OW / 64][8];
   __shared__ unsigned long long s_win;
   __shared__ uint32_t s_ctl;
+  __shared__ int s_dsrc, s_dsd;  // deferred time: the owner wave of the winner, the step index
   __shared__ float s_x[NBUF][MAXC][OW];  // -ln(u) of the owned magnets, step s in buffer s % 4
   const int tid = threadIdx.x;
   const int team = block_team[blockIdx.x];
@@ -184,6 +202,11 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   int64_t sw = S0.switches, step = S0.step, sd = 0;
   int err = 0;
   uint32_t rounds = 0;
+  // deferred waiting times (t_max = +inf, t finite): see the step's decision below
+  const bool defer = sT.adds != nullptr && isfinite(t) && isfinite(t0);
+  const float key_safe = (float)(690.0 + log(R.attempt_freq / 2.0));  // exp(key)*2/f0 < 1e300 with margin
+  const double t_start = t;
+  bool t_inf = false;
   // lower_than(att/n - att0/n, MCsteps_max) is monotone in att: find the first att where it
   // fails once, so the per-step test is an integer compare (same FP64 expression as core.py:756)
   int64_t att_stop = INT64_MAX;
@@ -271,12 +294,13 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
         }
         s_pc[wv] = wbest;
-        s_in[wv][0] = se;
-        s_in[wv][1] = sec;
-        s_in[wv][2] = sep;
-        s_in[wv][3] = sepc;
-        s_in[wv][4] = seb;
-        s_in[wv][5] = s_x[buf][bk][ot];
+        float *sin = s_in[step & 1][wv];
+        sin[0] = se;
+        sin[1] = sec;
+        sin[2] = sep;
+        sin[3] = sepc;
+        sin[4] = seb;
+        sin[5] = s_x[buf][bk][ot];
       } else if (lane == 0 && wbest == ~0ull) {
         s_pc[wv] = ~0ull;
       }
@@ -289,31 +313,35 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     if (wv == 0) {
       unsigned long long src = (unsigned long long)lane, pd = 0ull;
       unsigned long long pc = hs_wave_min_pay(lane < OW / 64 ? s_pc[lane] : ~0ull, src);
+      const float *in = s_in[step & 1][src < OW / 64 ? src : 0];
+      // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
+      auto waiting_time = [&]() {
+        return hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0, (double)in[4],
+                          I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
+      };
       if (pc == ~0ull) {
         pc = NONE;
-      } else {
-        // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
-        const float *in = s_in[src];
-        const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
-                                     (double)in[4], I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
-        pd = (unsigned long long)__double_as_longlong(dt);
+      } else if (!defer) {
+        pd = (unsigned long long)__double_as_longlong(waiting_time());
       }
       bool bad = false;
+      unsigned long long wr = (unsigned long long)rank;  // rank of the workgroup that holds the winner
+      const uint32_t round = rounds;
       if (G > 1) {
-        const unsigned long long tg = rounds & 63u;
-        TeamRec *slot = sT.rec + (size_t)(rounds % 3u) * G;
+        const unsigned long long tg = round & 63u;
+        TeamRec *slot = sT.rec + (size_t)(round % 3u) * G;
         if (lane == 0) {
           __hip_atomic_store(&slot[rank].c, pc | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(&slot[rank].d, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (!defer) __hip_atomic_store(&slot[rank].d, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         unsigned long long c2 = ~0ull, d2 = 0ull;
         for (int r = lane; r < G; r += 64) {
-          unsigned long long cr, dr;
+          unsigned long long cr, dr = 0ull;
           uint64_t spins = 0;
           for (;;) {
             cr = __hip_atomic_load(&slot[r].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            dr = __hip_atomic_load(&slot[r].d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (((cr & TAG_MASK) >> 26) == tg && (dr & 63ull) == tg) break;
+            if (!defer) dr = __hip_atomic_load(&slot[r].d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (((cr & TAG_MASK) >> 26) == tg && (defer || (dr & 63ull) == tg)) break;
             if (++spins > SPIN_LIMIT) {
               bad = true;
               break;
@@ -322,11 +350,12 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
           if (cr < c2) {
             c2 = cr;
-            d2 = dr;
+            d2 = defer ? (unsigned long long)r : dr;
           }
         }
         pc = hs_wave_min_pay(c2, d2) & ~TAG_MASK;
-        pd = d2 & ~63ull;
+        if (defer) wr = d2;
+        else pd = d2 & ~63ull;
         bad = __ballot(bad) != 0ull;
       }
       ++rounds;
@@ -335,7 +364,7 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         err = 3;  // a record of the round never arrived
       } else if (pc == NONE) {
         err = 1;  // nanargmin over an all-NaN grid
-      } else {
+      } else if (!defer) {
         const double dt = __longlong_as_double((long long)pd);
         // progress passes t_max - dt (core.py:757); update() passes t_max itself, also when t
         // has overflowed to +inf (where t - t0 would be NaN)
@@ -353,6 +382,60 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         const bool more =
             sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
         ctl = (flip ? 1u : 0u) | (more ? 2u : 0u);
+      } else {
+        // t_max = +inf: the step flips unless its time is NaN (core.py:702-704), and t only
+        // decides later steps once it is +inf (lower_than(inf - t0, inf) fails). A winning key
+        // that is finite and below key_safe has a finite time below 1e300, so the step flips
+        // and t stays finite: its time is computed after the step, off the critical path, by
+        // the workgroup that holds the winner. Otherwise the time is needed now.
+        const float key = key_value((uint32_t)(pc >> 32));
+        const bool safe = key < key_safe && key > -INFINITY;
+        bool flip = true;
+        if (!safe) {
+          double dt = 0.0;
+          if (wr == (unsigned long long)rank) dt = waiting_time();
+          if (G > 1) {
+            // the winner's workgroup publishes the time (word c) and then, with release order,
+            // the round number (word d); the others wait for the round with acquire order
+            TeamRec *ds = sT.dsl + round % 3u;
+            if (wr == (unsigned long long)rank && lane == 0) {
+              __hip_atomic_store(&ds->c, (unsigned long long)__double_as_longlong(dt), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&ds->d, (unsigned long long)round, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            uint64_t spins = 0;
+            while (__hip_atomic_load(&ds->d, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != (unsigned long long)round) {
+              if (++spins > SPIN_LIMIT) {
+                err = 3;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+            }
+            dt = __longlong_as_double((long long)__hip_atomic_load(&ds->c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          }
+          flip = !isnan(dt);
+          const double add = flip ? dt : R.t_max;
+          t_inf = t_inf || !isfinite(add);
+          if (rank == 0 && lane == 0) {
+            __hip_atomic_store((unsigned long long *)&sT.adds[sd], (unsigned long long)__double_as_longlong(add),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (R.log) sT.log_dt[sd] = add;
+          }
+        }
+        att += 1;
+        sd += 1;
+        sw += flip ? 1 : 0;
+        if (rank == 0 && lane == 0 && R.log) {
+          const uint32_t wp = (uint32_t)(pc >> 2) & 0xFFFFFFu;
+          sT.log_idx[sd - 1] = flip ? (int)(wp >> XB) * nx + (int)(wp & XM) : -1;
+        }
+        const bool more = sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && !t_inf));
+        const bool mine = safe && wr == (unsigned long long)rank;
+        ctl = (flip ? 1u : 0u) | (more ? 2u : 0u) | (mine ? 8u : 0u);
+        if (lane == 0 && mine) {
+          s_dsrc = (int)src;
+          s_dsd = (int)(sd - 1);
+        }
       }
       if (lane == 0) {
         s_win = pc;
@@ -364,6 +447,17 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     }
     __syncthreads();
     const uint32_t ctl = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ctl);
+    if ((ctl & 8u) && wv == OW / 64 && lane == 0) {
+      // the deferred waiting time of this workgroup's winner, by the first drawer wave while the
+      // owners apply the step's field update
+      const float *in = s_in[step & 1][s_dsrc];
+      const int sidx = s_dsd;
+      const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
+                                   (double)in[4], I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
+      __hip_atomic_store((unsigned long long *)&sT.adds[sidx], (unsigned long long)__double_as_longlong(dt),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (R.log) sT.log_dt[sidx] = dt;
+    }
     if (ctl & 4u) break;
     step += 1;
     go = (ctl & 2u) != 0;
@@ -449,6 +543,37 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       }
     }
   }
+  if (defer) {
+    // every workgroup reports its deferred times done (the wave that wrote them, after a release
+    // fence); rank 0 then adds all of them to t in step order, as the steps would have
+    if (tid == OW || tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (tid == OW) __hip_atomic_fetch_add(sT.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (rank == 0 && wv == 0) {
+      uint64_t spins = 0;
+      while (__hip_atomic_load(sT.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (unsigned int)G) {
+        if (++spins > SPIN_LIMIT) {
+          err = 3;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      double tt = t_start;
+      for (int64_t b = 0; b < sd; b += 64) {
+        const int64_t i = b + lane;
+        const unsigned long long v =
+            i < sd ? __hip_atomic_load((unsigned long long *)&sT.adds[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const int nb = (int)(sd - b < 64 ? sd - b : 64);
+        for (int j = 0; j < nb; ++j) {
+          const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+          const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+          tt += __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        }
+      }
+      t = tt;
+    }
+  }
   // write back
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
@@ -487,6 +612,13 @@ __global__ void k_team_reset(const TeamDev *teams, int32_t *err) {
     d.rec[i].c = ~0ull;
     d.rec[i].d = ~0ull;
   }
+  if (d.adds) {  // no round is ~0: every slot starts unwritten
+    if (threadIdx.x < 3) {
+      d.dsl[threadIdx.x].c = 0ull;
+      d.dsl[threadIdx.x].d = ~0ull;
+    }
+    if (threadIdx.x == 0) *d.done = 0u;
+  }
 }
 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -494,6 +626,7 @@ inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Layout {
   size_t teams_off, inst_off, map_off, sync_off, err_off, total;
   std::vector<size_t> team_sync;  // offset of each team's sync block
+  std::vector<size_t> defer;      // offset of each team's deferred-time block (0: none)
   std::vector<int> G, first;
   int nblocks, maxc;
 };
@@ -536,9 +669,17 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
   L.sync_off = L.map_off + al256(sizeof(int32_t) * (size_t)L.nblocks);
   size_t off = L.sync_off;
   L.team_sync.resize(n);
+  L.defer.resize(n);
   for (int i = 0; i < n; ++i) {
     L.team_sync[i] = off;
     off += al256(sizeof(TeamRec) * 3 * (size_t)L.G[i]);
+    // deferred waiting times: t_max = +inf and a step budget the adds array can hold
+    const hs_run_t &r = teams[i].run;
+    L.defer[i] = 0;
+    if (std::isinf(r.t_max) && r.t_max > 0 && r.max_steps >= 1 && r.max_steps <= ADD_CAP) {
+      L.defer[i] = off;
+      off += al256(sizeof(TeamRec) * 3 + 256 + sizeof(double) * (size_t)r.max_steps);
+    }
   }
   L.err_off = off;
   L.total = off + 256;
@@ -625,6 +766,9 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     d.log_idx = teams[i].log_idx;
     d.log_dt = teams[i].log_dt;
     d.rec = (TeamRec *)(base + L.team_sync[i]);
+    d.dsl = L.defer[i] ? (TeamRec *)(base + L.defer[i]) : nullptr;
+    d.done = L.defer[i] ? (unsigned int *)(base + L.defer[i] + sizeof(TeamRec) * 3) : nullptr;
+    d.adds = L.defer[i] ? (double *)(base + L.defer[i] + sizeof(TeamRec) * 3 + 256) : nullptr;
     d.G = L.G[i];
     d.first = L.first[i];
     td[i] = d;

@@ -55,6 +55,43 @@ def test_team_equals_per_step_launches(make, n, nblocks):
     _same(_snapshot(a), _snapshot(b))
 
 
+def _oop_hot(n, seed, EB_kT, **kw):
+    import hotspice_amd as hs
+    rng = np.random.default_rng(seed)
+    kT = hs.kB * 300
+    pat = np.where(rng.random((n, n)) < .5, -1., 1.)
+    return hs.ASI.OOP_Square(1e-6, n, T=300, E_B=EB_kT * kT * rng.normal(1, 0.002, size=(n, n)), moment=1.1e-15,
+                             pattern=pat, seed=seed, energies=[hs.DipolarEnergy(), hs.ExchangeEnergy(J=0.3 * kT)],
+                             params=hs.SimParams(UPDATE_SCHEME='Néel'), **kw)
+
+
+@pytest.mark.parametrize("EB_kT", [2000, 705])
+@pytest.mark.parametrize("nblocks", [1, 3])
+def test_team_extreme_barriers_equal_per_step(nblocks, EB_kT):
+    """With t_max = +inf a team defers each step's waiting time unless the winning key says it
+    could be huge, and then decides the time at once (its workgroup publishes it to the others).
+    Barriers of ~2000 kT overflow every time to +inf (the first step ends a progress() call);
+    ~705 kT give times near 1e290 that stay finite. Teams of one and of three workgroups equal
+    the per-step launches bit for bit: spins, fields, t and counters."""
+    a = _oop_hot(32, 5, EB_kT, recompute_every=None)
+    b = _oop_hot(32, 5, EB_kT, recompute_every=None)
+    a.neel_persistent = False
+    b.neel_blocks = nblocks
+    a.progress(t_max=math.inf, MCsteps_max=0.2)
+    b.progress(t_max=math.inf, MCsteps_max=0.2)
+    _same(_snapshot(a), _snapshot(b))
+    if EB_kT == 2000:
+        assert math.isinf(a.t) and a.attempted_switches == 1
+    else:
+        assert 1e250 < a.t < 1e305 and a.attempted_switches > 100
+    for t_max in (math.inf, 1.0, math.inf, 1e-3, math.inf):
+        ra, rb = a.update(t_max=t_max), b.update(t_max=t_max)
+        assert (ra is None) == (rb is None)
+        if ra is not None:
+            assert tuple(map(int, ra)) == tuple(map(int, rb))
+    _same(_snapshot(a), _snapshot(b))
+
+
 def test_team_update_returns_index():
     a = _pinwheel(16, 4)
     b = _pinwheel(16, 4)
