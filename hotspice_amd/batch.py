@@ -132,8 +132,9 @@ def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
 _TEAM_DTYPE = np.dtype(_lib.Team)
 
 
-def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=None, first_free=0):
-    """The hs_team_t descriptors of one launch, filled column by column."""
+def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=None, first_free=0, grid=None):
+    """The hs_team_t descriptors of one launch, filled column by column (`nblocks`: Neel team
+    sizes; `grid`: Metropolis update grids, hs_run_t.nblocks)."""
     arr = np.zeros(len(mags), dtype=_TEAM_DTYPE)
     arr['inst'] = [_lib.C.addressof(m._dev.inst) for m in mags]
     arr['state'] = [m._dev.state.data_ptr() for m in mags]
@@ -148,6 +149,8 @@ def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=
     run['first_free'] = np.asarray(first_free, dtype=np.int32)
     if nblocks is not None:
         arr['nblocks'] = np.asarray(nblocks, dtype=np.int32)
+    if grid is not None:
+        run['nblocks'] = np.asarray(grid, dtype=np.int32)
     return arr
 
 
@@ -169,7 +172,7 @@ def enqueue_metropolis(items, attempt_freq=1e10, begin=False):
     """The launches of launch_metropolis on the current stream, without reading anything back."""
     mags = [it[0] for it in items]
     teams = _team_array(mags, [it[2] for it in items], [it[3] for it in items], [it[4] for it in items],
-                        attempt_freq, begin, sels=[it[1] for it in items])
+                        attempt_freq, begin, sels=[it[1] for it in items], grid=[m._metro_grid() for m in mags])
     mags[0]._dev.run_teams(teams, "metropolis")
 
 
@@ -292,8 +295,9 @@ def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws, sts):
         if kw.get('idx') is not None:
             raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
         r = kw.get('r', 0)
-        Q = kw.get('Q', 0.05)
-        sels.append(m._select_geometry(m.calc_r(Q) if r == 0 else r))
+        r = m.calc_r(kw.get('Q', 0.05)) if r == 0 else r
+        m._metro_nblocks(r)  # 'auto' instances are tuned first
+        sels.append(m._select_geometry(r))
     first = True
     while True:
         items = [(m, sel, metropolis_chunk(m, sel, s0, mc, chunk, st), t_max, mc)

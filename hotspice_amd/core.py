@@ -109,6 +109,8 @@ class Magnets(ABC):
         self.seed = int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed) & 0xFFFFFFFFFFFFFFFF
         self.neel_persistent = True  # one persistent launch per progress() chunk (False: one launch per step)
         self.neel_blocks = 0         # workgroups in the instance's team (0: default, 'auto': autotuned)
+        self.metro_blocks = 0        # Metropolis update grid (0: default, 'auto': autotuned per geometry)
+        self._metro_tuned = {}       # selection geometry -> autotuned update grid
         self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
         self.params = SimParams() if params is None else params
         self.recompute_every = recompute_every
@@ -812,6 +814,23 @@ class Magnets(ABC):
             tune_neel_blocks(self)
         return self.neel_blocks
 
+    def _metro_nblocks(self, r):
+        """Update grid (hs_run_t.nblocks) of the next Metropolis launch at sampling distance r
+        ('auto' runs the autotuner once per selection geometry)."""
+        if self.metro_blocks == 'auto':
+            self._select_geometry(r)
+            if self._sel_key not in self._metro_tuned:
+                from .tune import tune_metro_blocks
+                tune_metro_blocks(self, r)
+        return self._metro_grid()
+
+    def _metro_grid(self):
+        """Update grid for the current selection geometry (0, the library's default, while an
+        'auto' instance has not been tuned for it)."""
+        if self.metro_blocks == 'auto':
+            return self._metro_tuned.get(self._sel_key, 0)
+        return int(self.metro_blocks or 0)
+
     def _update_Néel(self, t_max=1, attempt_freq=1e10):
         """core.py:679-709: returns the (y, x) divmod tuple of the switched magnet, or None."""
         self._pre_step()
@@ -836,13 +855,15 @@ class Magnets(ABC):
         if idx is not None:
             raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
         self._pre_step()
-        sel = self._select_geometry(self.calc_r(Q) if r == 0 else r)
+        r = self.calc_r(Q) if r == 0 else r
+        nblocks = self._metro_nblocks(r)
+        sel = self._select_geometry(r)
         cap = max(1, sel.nsc)
         t = self._dev.torch
         cells = t.zeros(cap, dtype=t.int32, device=self._dev.device)
         count = t.zeros(1, dtype=t.int32, device=self._dev.device)
-        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1, nblocks=0,
-                       begin=1, first_free=1)
+        run = _lib.Run(t_max=math.inf, mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1,
+                       nblocks=nblocks, begin=1, first_free=1)
         self._dev.metropolis_run(sel, run, cells, count)
         self._after_device_step()
         a = int(count.item())
@@ -1014,7 +1035,9 @@ class Magnets(ABC):
             Q = kwargs.get('Q', 0.05)
             if kwargs.get('idx') is not None:
                 raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
-            sel = self._select_geometry(self.calc_r(Q) if r == 0 else r)
+            r = self.calc_r(Q) if r == 0 else r
+            nblocks = self._metro_nblocks(r)
+            sel = self._select_geometry(r)
             while True:
                 per_step = self._metro_rate or max(1.0, 0.25 * sel.nsc)
                 st = self._state()
@@ -1028,7 +1051,7 @@ class Magnets(ABC):
                 if self.recompute_every and self._incremental_exact():
                     chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
                 run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
-                               max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
+                               max_steps=int(chunk), log=0, nblocks=nblocks, begin=int(begin))
                 if begin:
                     self._set_progress_origin(t0, a0)
                 a_before = st.attempts

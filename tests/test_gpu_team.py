@@ -161,7 +161,16 @@ def test_recompute_many_equals_single():
             np.testing.assert_array_equal(x._dev.Epc.cpu().numpy(), y._dev.Epc.cpu().numpy())
 
 
-def test_autotuner_keeps_state_and_results():
+@pytest.fixture
+def tune_cache(tmp_path, monkeypatch):
+    from hotspice_amd import tune
+    monkeypatch.setenv("HOTSPICE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(tune, "_CACHE", {})
+    monkeypatch.setattr(tune, "_LOADED", False)
+    return tmp_path / "tune.json"
+
+
+def test_autotuner_keeps_state_and_results(tune_cache):
     """tune_neel_blocks times team sizes on the instance itself, restores its state exactly and
     leaves a choice that gives the same trajectory as the default team."""
     from hotspice_amd import tune
@@ -174,6 +183,40 @@ def test_autotuner_keeps_state_and_results():
     a.progress(t_max=math.inf, MCsteps_max=0.5)
     b.progress(t_max=math.inf, MCsteps_max=0.5)
     _same(_snapshot(a), _snapshot(b))
+    assert tune_cache.exists()
+    c = _pinwheel(64, 10)
+    c.neel_blocks = 'auto'  # a new instance of the same shape takes the cached choice
+    c.progress(t_max=math.inf, MCsteps_max=0.1)
+    assert c.neel_blocks == a.neel_blocks
+
+
+def test_metropolis_autotuner_keeps_state_and_results(tune_cache):
+    """tune_metro_blocks times update grids on the instance itself and restores its state; an
+    instance with metro_blocks='auto' (alone and batched) steps bit-identically to the default
+    grid, and a fixed grid does too."""
+    from hotspice_amd import batch, tune
+    from cases import spec
+    kw = spec("metro_oop_presample")["update_kw"]
+    a = _metro("metro_oop_presample")
+    before = _snapshot(a)
+    r = kw.get("r") or a.calc_r(kw.get("Q", 0.05))
+    timing = tune.tune_metro_blocks(a, r, candidates=[1, 16, 175], steps=8, reuse=False)
+    assert set(timing) == {1, 16, 175} and set(a._metro_tuned.values()) <= {1, 16, 175}
+    _same(_snapshot(a), before)
+    ref = _metro("metro_oop_presample")
+    ref.progress(t_max=math.inf, MCsteps_max=0.5, **kw)
+    a.metro_blocks = 'auto'
+    a.progress(t_max=math.inf, MCsteps_max=0.5, **kw)
+    _same(_snapshot(a), _snapshot(ref))
+    fixed = _metro("metro_oop_presample")
+    fixed.metro_blocks = 1
+    fixed.progress(t_max=math.inf, MCsteps_max=0.5, **kw)
+    _same(_snapshot(fixed), _snapshot(ref))
+    many = [_metro("metro_oop_presample") for _ in range(2)]
+    many[0].metro_blocks = 'auto'
+    batch.progress_many(many, t_max=math.inf, MCsteps_max=0.5, **kw)
+    for m in many:
+        _same(_snapshot(m), _snapshot(ref))
 
 
 def test_presampled_metropolis_batch_equals_alone():
