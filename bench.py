@@ -369,6 +369,19 @@ def bench_batched(args):
     attempts = sum(m.attempted_switches for m in mags) - a0
     accepted = sum(m.switches for m in mags) - s0
     launches, dev.launch_events = dev.launch_events, None
+    # the same sweeps again with the step loop replayed from captured hipGraphs of 16 steps
+    from hotspice_amd import _lib
+    g0 = sum(m.attempted_switches for m in mags)
+    old = _lib.set_graph_steps(16)
+    try:
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for _ in range(sweeps):
+            batch.progress_many(mags, t_max=math.inf, MCsteps_max=1.0)
+        torch.cuda.synchronize()
+        graph_rate = (sum(m.attempted_switches for m in mags) - g0) / (time.perf_counter() - tg)
+    finally:
+        _lib.set_graph_steps(old)
     err = max(rel_field_error(m) for m in mags[::8])
     # plain path: the same instances (fresh copies) stepped one at a time, one sweep each
     plain = [ipsq_batch_instance(s) for s in specs]
@@ -391,7 +404,8 @@ def bench_batched(args):
                                   "10 cells + REDUCED_KERNEL_SIZE=10, cutoff 0, grid multisampling Q=0.05, FP32+Kahan",
                       "instances": len(mags), "magnets": int(sum(m.n for m in mags)), "attempts": attempts,
                       "accepted": accepted, "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
-                      "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
+                      "vs_plain": attempts / wall / plain_rate, "graph_replay_attempts_per_s": graph_rate,
+                      "parallelism": "single GPU"})
     line["roofline"] = roofline("k_metro_select_many + k_metro_update_many (per batched call)", launches, bytes_total)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batched(specs, args.cpu_seconds)
     return line
@@ -432,7 +446,8 @@ def bench_neel_batched(args):
                                   "the 256x256 pinwheel workload, one persistent launch, FP64 recompute every 1024 "
                                   "switches per instance", "instances": count, "attempts": attempts,
                       "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
-                      "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
+                      "vs_plain": attempts / wall / plain_rate, "graph_replay_attempts_per_s": graph_rate,
+                      "parallelism": "single GPU"})
     line["roofline"] = roofline("k_neel_team (256 teams of 1 workgroup)", launches, bytes_total)
     return line
 

@@ -24,7 +24,7 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
            "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
            "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes",
-           "hs_metropolis_run_static")
+           "hs_metropolis_run_static", "hs_set_graph_steps")
 
 HS_METRO_STATIC_SCRATCH_BYTES = 4 << 20
 
@@ -137,6 +137,7 @@ def load():
         "hs_neel_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),
         "hs_neel_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
         "hs_metro_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),
+        "hs_set_graph_steps": ([i32], C.c_int),
         "hs_metro_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
         "hs_recompute_fields_many": ([C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), C.POINTER(i64), i32, vp, i64, vp],
                                      C.c_int),
@@ -186,3 +187,9 @@ def abi_layout_library():
     out = (C.c_int64 * 20)()
     n = lib.hs_abi_layout(out, 20)
     return list(out)[:n]
+
+
+def set_graph_steps(steps):
+    """Steps of the batched Metropolis loop captured per hipGraph (0: plain launches); returns the
+    previous setting. Results are bit-identical either way."""
+    return int(load().hs_set_graph_steps(int(steps)))

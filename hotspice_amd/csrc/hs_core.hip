@@ -6,8 +6,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "hs_device.h"
 #include "hs_internal.h"
@@ -78,6 +80,74 @@ int hs_upload_async(void *dst, const void *src, size_t bytes, hipStream_t s) {
   HS_HIP_TRY(hipMemcpyAsync(dst, slot.host, bytes, hipMemcpyHostToDevice, s));
   HS_HIP_TRY(hipEventRecord(slot.done, s));
   slot.pending = true;
+  return HS_OK;
+}
+
+// ---- captured step graphs -------------------------------------------------------------------
+// A stepping loop enqueues the same few kernels for every step, with arguments that depend only
+// on the workspace layout and the step's parity. hs_graph_launch captures such a run of launches
+// once, on a private stream of the device (capture records the launches without running them),
+// instantiates it, and replays the executable graph on the caller's stream for every later call
+// with the same key. The cache holds the GRAPH_CACHE most recently used graphs over all devices.
+namespace {
+struct GraphEntry {
+  std::vector<int64_t> key;
+  int device;
+  hipGraphExec_t exec;
+  uint64_t used;
+};
+constexpr size_t GRAPH_CACHE = 16;
+constexpr int GRAPH_DEVICES = 64;
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;
+uint64_t g_graph_clock = 0;
+hipStream_t g_capture_stream[GRAPH_DEVICES] = {};
+std::atomic<int> g_graph_steps{HS_GRAPH_STEPS_DEFAULT};
+}  // namespace
+
+int hs_graph_steps() { return g_graph_steps.load(std::memory_order_relaxed); }
+
+extern "C" int hs_set_graph_steps(int32_t steps) { return g_graph_steps.exchange(steps > 0 ? steps : 0); }
+
+int hs_graph_launch(const std::vector<int64_t> &key, hipStream_t s, const std::function<int(hipStream_t)> &record) {
+  int dev = 0;
+  HS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= GRAPH_DEVICES) return hs_set_error(HS_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  GraphEntry *hit = nullptr;
+  for (GraphEntry &e : g_graphs)
+    if (e.device == dev && e.key == key) hit = &e;
+  if (!hit) {
+    if (!g_capture_stream[dev]) HS_HIP_TRY(hipStreamCreateWithFlags(&g_capture_stream[dev], hipStreamNonBlocking));
+    hipStream_t cs = g_capture_stream[dev];
+    HS_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int rc = record(cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc ? rc : hs_set_error(HS_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return hs_set_error(HS_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    if (g_graphs.size() >= GRAPH_CACHE) {  // drop the least recently used graph, once it has run
+      size_t lru = 0;
+      for (size_t i = 1; i < g_graphs.size(); ++i)
+        if (g_graphs[i].used < g_graphs[lru].used) lru = i;
+      HS_HIP_TRY(hipSetDevice(g_graphs[lru].device));
+      const hipError_t es = hipDeviceSynchronize();
+      HS_HIP_TRY(hipSetDevice(dev));
+      if (es != hipSuccess) return hs_set_error(HS_ERR_HIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(es));
+      HS_HIP_TRY(hipGraphExecDestroy(g_graphs[lru].exec));
+      g_graphs.erase(g_graphs.begin() + (std::ptrdiff_t)lru);
+    }
+    g_graphs.push_back(GraphEntry{key, dev, exec, 0});
+    hit = &g_graphs.back();
+  }
+  hit->used = ++g_graph_clock;
+  HS_HIP_TRY(hipGraphLaunch(hit->exec, s));
   return HS_OK;
 }
 

@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <functional>
+#include <vector>
+
 #include "../../include/hotspice_hip.h"
 
 int hs_set_error(int code, const char *fmt, ...);
@@ -23,6 +27,15 @@ void hs_clear_error();
 // work queued before it: the bytes go through a pinned staging buffer that is refilled only once
 // its previous copy has completed. `src` may be released as soon as the call returns.
 int hs_upload_async(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+// Replays the kernel launches that `record(stream)` enqueues as one captured hipGraph on `s`
+// (hs_core.hip). The first call with a given key on a device captures them on a private stream
+// and instantiates the graph; later calls with that key replay it. The key must determine every
+// launch `record` makes and every argument of those launches.
+int hs_graph_launch(const std::vector<int64_t> &key, hipStream_t s, const std::function<int(hipStream_t)> &record);
+
+// Steps per captured graph of the stepping loops (hs_set_graph_steps; 0 = plain launches).
+int hs_graph_steps();
 
 // Validates the fields every stepping kernel relies on (sizes, pointers, table bounds).
 int hs_check_instance(const hs_instance_t *I);

@@ -1941,24 +1941,44 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
     for (int i : L.large) req = std::max<int64_t>(req, teams[i].run.nblocks);
     const unsigned wgs = update_grid(L.ntiles, req);
     const int32_t *lj = (const int32_t *)(base + L.large_off);
-    for (int64_t j = 0; j < steps; ++j) {
-      const int par = (int)(j & 1);
-      if (L.pre) {  // as hs_metropolis_run's large steps, for every job of the batch
-        k_metro_presample_many<<<(unsigned)L.pmap.size(), PRESAMPLE_THREADS, 0, s>>>(dj, (const int2 *)(base + L.pmap_off));
-        HS_LAUNCH_CHECK("k_metro_presample_many");
-        k_metro_select_pre_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, lj, WL, par);
-        HS_LAUNCH_CHECK("k_metro_select_pre_many");
-        if (!L.tmap.empty()) {
-          k_metro_tiles_many<<<(unsigned)L.tmap.size(), 256, 0, s>>>(dj, (const int2 *)(base + L.tmap_off), WL, par);
-          HS_LAUNCH_CHECK("k_metro_tiles_many");
+    // the launches of steps j0 .. j0+count-1 on stream `st`
+    auto record = [&](hipStream_t st, int64_t j0, int64_t count) -> int {
+      for (int64_t j = j0; j < j0 + count; ++j) {
+        const int par = (int)(j & 1);
+        if (L.pre) {  // as hs_metropolis_run's large steps, for every job of the batch
+          k_metro_presample_many<<<(unsigned)L.pmap.size(), PRESAMPLE_THREADS, 0, st>>>(
+              dj, (const int2 *)(base + L.pmap_off));
+          HS_LAUNCH_CHECK("k_metro_presample_many");
+          k_metro_select_pre_many<<<(unsigned)L.large.size(), MS_THREADS, 0, st>>>(dj, lj, WL, par);
+          HS_LAUNCH_CHECK("k_metro_select_pre_many");
+          if (!L.tmap.empty()) {
+            k_metro_tiles_many<<<(unsigned)L.tmap.size(), 256, 0, st>>>(dj, (const int2 *)(base + L.tmap_off), WL,
+                                                                         par);
+            HS_LAUNCH_CHECK("k_metro_tiles_many");
+          }
+        } else {
+          k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, st>>>(dj, lj, WL, par);
+          HS_LAUNCH_CHECK("k_metro_select_many");
         }
-      } else {
-        k_metro_select_many<<<(unsigned)L.large.size(), MS_THREADS, 0, s>>>(dj, lj, WL, par);
-        HS_LAUNCH_CHECK("k_metro_select_many");
+        k_metro_update_many<<<wgs, MU_THREADS, 0, st>>>(dj, WL, par);
+        HS_LAUNCH_CHECK("k_metro_update_many");
       }
-      k_metro_update_many<<<wgs, MU_THREADS, 0, s>>>(dj, WL, par);
-      HS_LAUNCH_CHECK("k_metro_update_many");
+      return HS_OK;
+    };
+    // whole groups of U steps (U even, so that every group starts at parity 0) replay one captured
+    // graph; the launches and arguments depend only on the workspace layout, which the key holds
+    const int64_t U = hs_graph_steps() & ~1;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HS_HIP_TRY(hipStreamIsCapturing(s, &cap));
+    int64_t j = 0;
+    if (U >= 2 && steps >= U && cap == hipStreamCaptureStatusNone) {
+      const std::vector<int64_t> key{1, (int64_t)(uintptr_t)base, nteams, (int64_t)L.large.size(),
+                                     (int64_t)L.pmap.size(), (int64_t)L.tmap.size(), L.ntiles, (int64_t)L.pre,
+                                     (int64_t)wgs, U};
+      for (; j + U <= steps; j += U)
+        if (int rc = hs_graph_launch(key, s, [&](hipStream_t cs) { return record(cs, 0, U); })) return rc;
     }
+    if (int rc = record(s, j, steps - j)) return rc;
   }
   return HS_OK;
 }

@@ -427,6 +427,22 @@ int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
 int64_t hs_metro_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 
+/* Steps per captured hipGraph in the step loop of hs_metro_run_teams by default: plain launches
+ * (the loop is bound by the GPU, not by the host's launch rate, and replaying graphs measured no
+ * faster on MI355X). */
+#define HS_GRAPH_STEPS_DEFAULT 0
+
+/*
+ * Sets how many steps of the hs_metro_run_teams step loop are captured into one hipGraph and
+ * replayed as a unit (rounded down to an even number; 0 or 1: plain launches, one per kernel).
+ * A graph is captured the first time a workspace layout and step count are seen and replayed on
+ * later calls; the step kernels, their order and their arguments are the same either way, so
+ * results are bit-identical. When the caller's stream is itself being captured the loop always
+ * launches plainly (the caller's capture records it). Returns the previous setting. No
+ * counterpart in the reference (launch mechanics of the device loop).
+ */
+int hs_set_graph_steps(int32_t steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -235,3 +235,39 @@ def test_presampled_metropolis_batch_equals_alone():
     for m1, m2, r in zip(alone, mixed, res):
         _same(_snapshot(m1), _snapshot(m2))
         assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)
+
+
+@pytest.mark.parametrize("names", [["metro_oop_presample", "metro_ipsq_presample_pbc", "metro_oop_window"],
+                                   ["metro_oop_presample", "metro_ipsq_pbc", "metro_pinwheel"]],
+                         ids=["presampled", "listed"])
+def test_metropolis_graph_replay_equals_plain_launches(names):
+    """The batched Metropolis loop replays a captured hipGraph per group of 4 steps (two replays
+    and two plain steps per call, the graph captured on the first call and reused after it, with
+    the host reading the states and running recomputes between calls); after every call each
+    instance is bit-identical to the same batch stepped with plain launches only."""
+    from cases import spec
+    from hotspice_amd import _lib, batch
+    kws = [spec(n)["update_kw"] for n in names]
+
+    def sels(mags):
+        return [m._select_geometry(kw.get("r") or m.calc_r(kw.get("Q", 0.05))) for m, kw in zip(mags, kws)]
+
+    plain = [_metro(n) for n in names]
+    graphed = [_metro(n) for n in names]
+    for m in plain + graphed:
+        m._pre_step()
+    sp, sg = sels(plain), sels(graphed)
+    old = _lib.set_graph_steps(4)
+    try:
+        for call in range(4):
+            _lib.set_graph_steps(0)
+            batch.launch_metropolis([(m, s, 10, math.inf, math.inf) for m, s in zip(plain, sp)], begin=call == 0)
+            _lib.set_graph_steps(4)
+            batch.launch_metropolis([(m, s, 10, math.inf, math.inf) for m, s in zip(graphed, sg)], begin=call == 0)
+            for a, b in zip(plain, graphed):
+                a._after_device_step()
+                b._after_device_step()
+                _same(_snapshot(a), _snapshot(b))
+            assert all(m.attempted_switches > 0 for m in graphed)
+    finally:
+        _lib.set_graph_steps(old)
