@@ -219,3 +219,20 @@ HS_INLINE unsigned long long hs_wave_min_pay(unsigned long long v, unsigned long
   pay = ((unsigned long long)hi << 32) | lo;
   return m;
 }
+
+// hs_wave_min_pay with two payloads taken from the lane that holds the minimum.
+HS_INLINE unsigned long long hs_wave_min_pay2(unsigned long long v, unsigned long long &pay, unsigned long long &pay2) {
+  const unsigned long long m = hs_wave_min_u64(v);
+  const int src = __ffsll((long long)__ballot(v == m)) - 1;
+  pay = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pay >> 32), src) << 32) |
+        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pay, src);
+  pay2 = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pay2 >> 32), src) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pay2, src);
+  return m;
+}
+
+// A 64-bit value of the wave's first active lane, in every lane.
+HS_INLINE unsigned long long hs_uniform_u64(unsigned long long v) {
+  return ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+}

@@ -44,8 +44,14 @@ constexpr unsigned long long NONE = ~0ull & ~TAG_MASK;  // "no candidate in this
 // tag in both words has the whole record even if the two words landed separately. Each record has
 // a 128-byte line of its own: records that share a line serialise their device-scope stores and
 // polls (64 workgroups exchange in about 1.0 us with one line each, 1.6 us at 16 bytes apart).
+// With deferred times (below) the candidate is published before its time, and the time goes to
+// a line of its own (dd), so storing it does not disturb the line that every workgroup polls.
+// Records sit REC_BYTES apart (measured on MI355X with 64 workgroups: 2.72 us per step at 256
+// bytes, 2.78 us with the deferred times in a separate array of 128-byte records).
+constexpr int REC_BYTES = 256;
 struct alignas(128) TeamRec {
-  unsigned long long c, d;
+  unsigned long long c, d, pad0[14];
+  unsigned long long dd, pad1[REC_BYTES / 8 - 17];
 };
 
 struct TeamDev {
@@ -55,18 +61,8 @@ struct TeamDev {
   int32_t *log_idx;
   double *log_dt;
   TeamRec *rec;  // [3][G]: round k uses slot k % 3
-  // deferred waiting times (t_max = +inf launches; NULL: every step's time is decided in the
-  // step): adds[i] = what step i adds to t, written by the workgroup that holds the winner;
-  // dsl[3]: a step whose time must be known at once (see k_neel_team) publishes it there;
-  // done: workgroups that have finished the launch
-  double *adds;
-  TeamRec *dsl;
-  unsigned int *done;
   int32_t G, first;
 };
-
-// Launches with deferred waiting times hold at most this many steps.
-constexpr int64_t ADD_CAP = 1 << 16;
 
 HS_INLINE uint32_t key_bits(float k) {
   uint32_t u = __float_as_uint(k);
@@ -75,8 +71,6 @@ HS_INLINE uint32_t key_bits(float k) {
 
 // The key whose bits key_bits gave.
 HS_INLINE float key_value(uint32_t b) { return __uint_as_float((b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b); }
-
-
 
 // FP32 log-rate of a cell: -b*beta (one channel) or log(exp(-b1 beta) + exp(-b2 beta)); the
 // same expression as the per-step launches use, so both paths rank cells identically.
@@ -107,17 +101,14 @@ HS_INLINE int mod_small(uint32_t x, uint32_t d, uint64_t M) {
   return (int)(x - q * d);
 }
 
-template <int MAXC, bool PERP>
+template <int MAXC, bool PERP, bool DEFER>
 __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const int32_t *block_team, int32_t *err_out) {
   __shared__ hs_instance_t sI;
   __shared__ TeamDev sT;
   __shared__ unsigned long long s_pc[OW / 64];
-  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each owner wave's best, by step parity (a deferred
-  // waiting time reads step s's while the owners write step s+1's)
-  __shared__ float s_in[2][OW / 64][8];
+  __shared__ float s_in[OW / 64][8];  // fields E, Ec, Ep, Epc, E_B and -ln(u) of each owner wave's best
   __shared__ unsigned long long s_win;
   __shared__ uint32_t s_ctl;
-  __shared__ int s_dsrc, s_dsd;  // deferred time: the owner wave of the winner, the step index
   __shared__ float s_x[NBUF][MAXC][OW];  // -ln(u) of the owned magnets, step s in buffer s % 4
   const int tid = threadIdx.x;
   const int team = block_team[blockIdx.x];
@@ -202,11 +193,6 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
   int64_t sw = S0.switches, step = S0.step, sd = 0;
   int err = 0;
   uint32_t rounds = 0;
-  // deferred waiting times (t_max = +inf, t finite): see the step's decision below
-  const bool defer = sT.adds != nullptr && isfinite(t) && isfinite(t0);
-  const float key_safe = (float)(690.0 + log(R.attempt_freq / 2.0));  // exp(key)*2/f0 < 1e300 with margin
-  const double t_start = t;
-  bool t_inf = false;
   // lower_than(att/n - att0/n, MCsteps_max) is monotone in att: find the first att where it
   // fails once, so the per-step test is an integer compare (same FP64 expression as core.py:756)
   int64_t att_stop = INT64_MAX;
@@ -253,7 +239,22 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     }
   };
 
-  bool go = sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
+  // hs_step_free(first_free, sd) without the step counter in the loop: the first step's flag,
+  // then every later step's (only first_free == 2 frees them)
+  const bool all_free = R.first_free == 2;
+  bool free_now = hs_step_free(R.first_free, 0);
+  bool go = sd < R.max_steps && (free_now || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
+  // Deferred waiting times (t_max = +inf, a team of G > 1, no step log): a step whose winning key
+  // is below key_safe flips whatever its exact time, so the workgroups go on without waiting for
+  // the winner's time; each adds it to t at the next round (read with that round's polls) or at
+  // the end of the launch, in step order, so t is bit-identical to waiting for every time. With
+  // |t| < 1e307 at the start and at most 2^20 steps of times below 1e300, t stays finite.
+  const bool defer = DEFER && G > 1 && !R.log && R.t_max == INFINITY && fabs(t) < 1e307 && isfinite(t0) &&
+                     R.max_steps <= (int64_t)1 << 20;
+  const float key_safe = (float)(690.0 + log(R.attempt_freq / 2.0));  // exp(key) * 2 / f0 < 1e300
+  int pend = -1;                   // rank whose record holds the pending time (-1: none)
+  uint32_t pend_slot = 0;          // its slot
+  unsigned long long pend_tag = 0; // its round tag
   if (go && !owner) draw(step, (step & 1) == 0);
   __syncthreads();
   while (go) {
@@ -294,53 +295,62 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
         }
         s_pc[wv] = wbest;
-        float *sin = s_in[step & 1][wv];
-        sin[0] = se;
-        sin[1] = sec;
-        sin[2] = sep;
-        sin[3] = sepc;
-        sin[4] = seb;
-        sin[5] = s_x[buf][bk][ot];
+        s_in[wv][0] = se;
+        s_in[wv][1] = sec;
+        s_in[wv][2] = sep;
+        s_in[wv][3] = sepc;
+        s_in[wv][4] = seb;
+        s_in[wv][5] = s_x[buf][bk][ot];
       } else if (lane == 0 && wbest == ~0ull) {
         s_pc[wv] = ~0ull;
       }
     }
     __syncthreads();
-    // 3. wave 0: the workgroup's best; with G > 1 it writes the workgroup's record for this round
-    //    and waits (bounded) until all G records of the round carry its tag, then reduces them.
-    //    Wave 0 then advances time and counters (core.py:700-707) and tells the others whether
-    //    the winner flips and whether another step follows.
+    // 3. wave 0: the workgroup's best; with G > 1 it publishes the workgroup's record for this
+    //    round (the candidate first, then its waiting time) and waits (bounded) until all G
+    //    candidates of the round carry its tag; only the winner's time is needed, and a deferred
+    //    step does not wait for it (below). Wave 0 then advances time and counters
+    //    (core.py:700-707) and tells the others whether the winner flips and whether another
+    //    step follows.
     if (wv == 0) {
       unsigned long long src = (unsigned long long)lane, pd = 0ull;
       unsigned long long pc = hs_wave_min_pay(lane < OW / 64 ? s_pc[lane] : ~0ull, src);
-      const float *in = s_in[step & 1][src < OW / 64 ? src : 0];
+      if (pc == ~0ull) pc = NONE;
       // FP64 waiting time of the workgroup's best from its compensated fields (core.py:686-700)
-      auto waiting_time = [&]() {
-        return hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0, (double)in[4],
-                          I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
+      auto own_dt = [&]() -> unsigned long long {
+        if (pc == NONE) return 0ull;
+        const float *in = s_in[src];
+        const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
+                                     (double)in[4], I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
+        return (unsigned long long)__double_as_longlong(dt);
       };
-      if (pc == ~0ull) {
-        pc = NONE;
-      } else if (!defer) {
-        pd = (unsigned long long)__double_as_longlong(waiting_time());
-      }
-      bool bad = false;
-      unsigned long long wr = (unsigned long long)rank;  // rank of the workgroup that holds the winner
-      const uint32_t round = rounds;
+      bool bad = false, deferred = false;
       if (G > 1) {
-        const unsigned long long tg = round & 63u;
-        TeamRec *slot = sT.rec + (size_t)(round % 3u) * G;
-        if (lane == 0) {
-          __hip_atomic_store(&slot[rank].c, pc | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (!defer) __hip_atomic_store(&slot[rank].d, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long tg = rounds & 63u;
+        const uint32_t sl = rounds % 3u;
+        TeamRec *slot = sT.rec + (size_t)sl * G;
+        if (defer) {  // the candidate at once, its time (needed only if it wins) after it
+          if (lane == 0) __hip_atomic_store(&slot[rank].c, pc | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pd = own_dt();
+          if (lane == 0) __hip_atomic_store(&slot[rank].dd, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          pd = own_dt();
+          if (lane == 0) {
+            __hip_atomic_store(&slot[rank].c, pc | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&slot[rank].d, pd | tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
         }
-        unsigned long long c2 = ~0ull, d2 = 0ull;
+        // the previous round's deferred time, loaded together with the first polls
+        unsigned long long dprev = 0ull;
+        if (pend >= 0 && lane == 0)
+          dprev = __hip_atomic_load(&sT.rec[(size_t)pend_slot * G + pend].dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long c2 = ~0ull, d2 = 0ull, r2 = 0ull;
         for (int r = lane; r < G; r += 64) {
-          unsigned long long cr, dr = 0ull;
+          unsigned long long cr, dr;
           uint64_t spins = 0;
           for (;;) {
             cr = __hip_atomic_load(&slot[r].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (!defer) dr = __hip_atomic_load(&slot[r].d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            dr = defer ? 0ull : __hip_atomic_load(&slot[r].d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (((cr & TAG_MASK) >> 26) == tg && (defer || (dr & 63ull) == tg)) break;
             if (++spins > SPIN_LIMIT) {
               bad = true;
@@ -350,13 +360,49 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           }
           if (cr < c2) {
             c2 = cr;
-            d2 = defer ? (unsigned long long)r : dr;
+            d2 = dr;
+            r2 = (unsigned long long)r;
           }
         }
-        pc = hs_wave_min_pay(c2, d2) & ~TAG_MASK;
-        if (defer) wr = d2;
-        else pd = d2 & ~63ull;
+        pc = hs_wave_min_pay2(c2, d2, r2) & ~TAG_MASK;
         bad = __ballot(bad) != 0ull;
+        if (pend >= 0) {  // settle the previous round's time first: t accumulates in step order
+          unsigned long long dp = hs_uniform_u64(dprev);
+          uint64_t spins = 0;
+          while ((dp & 63ull) != pend_tag && !bad) {
+            if (++spins > SPIN_LIMIT) bad = true;
+            dp = __hip_atomic_load(&sT.rec[(size_t)pend_slot * G + pend].dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          t += __longlong_as_double((long long)(dp & ~63ull));
+          pend = -1;
+        }
+        // a deferred step: t_max = +inf and a finite key below key_safe, whose waiting time is
+        // finite and below 1e300, so the step flips and the stop predicate does not depend on t
+        // (t stays below 1e307 over the launch); every other step waits for the winner's time
+        if (defer && pc != NONE && !bad) {
+          const float key = key_value((uint32_t)(pc >> 32));
+          deferred = key < key_safe && key > -INFINITY;
+        }
+        if (defer && !deferred && pc != NONE && !bad) {  // a step that needs the winner's time now
+          uint64_t spins = 0;
+          d2 = __hip_atomic_load(&slot[r2].dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          while ((d2 & 63ull) != tg) {
+            if (++spins > SPIN_LIMIT) {
+              bad = true;
+              break;
+            }
+            d2 = __hip_atomic_load(&slot[r2].dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          bad = __ballot(bad) != 0ull;
+        }
+        if (deferred) {
+          pend = (int)r2;
+          pend_slot = sl;
+          pend_tag = tg;
+        }
+        pd = d2 & ~63ull;
+      } else {
+        pd = own_dt();
       }
       ++rounds;
       uint32_t ctl = 4u;  // stop without a step
@@ -364,11 +410,19 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
         err = 3;  // a record of the round never arrived
       } else if (pc == NONE) {
         err = 1;  // nanargmin over an all-NaN grid
-      } else if (!defer) {
+      } else if (deferred) {
+        att += 1;
+        sd += 1;
+        sw += 1;
+        free_now = all_free;
+        const bool more = sd < R.max_steps && (all_free || att < att_stop);
+        ctl = 1u | (more ? 2u : 0u);
+      } else {
         const double dt = __longlong_as_double((long long)pd);
         // progress passes t_max - dt (core.py:757); update() passes t_max itself, also when t
         // has overflowed to +inf (where t - t0 would be NaN)
-        const double tmax_step = hs_step_free(R.first_free, sd) ? R.t_max : R.t_max - (t - t0);
+        const double tmax_step = free_now ? R.t_max : R.t_max - (t - t0);
+        free_now = all_free;
         att += 1;
         sd += 1;
         const bool flip = !(dt > tmax_step || isnan(dt));
@@ -379,63 +433,8 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
           sT.log_idx[sd - 1] = flip ? (int)(wp >> XB) * nx + (int)(wp & XM) : -1;
           sT.log_dt[sd - 1] = flip ? dt : tmax_step;
         }
-        const bool more =
-            sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
+        const bool more = sd < R.max_steps && (all_free || (att < att_stop && hs_lower_than(t - t0, R.t_max)));
         ctl = (flip ? 1u : 0u) | (more ? 2u : 0u);
-      } else {
-        // t_max = +inf: the step flips unless its time is NaN (core.py:702-704), and t only
-        // decides later steps once it is +inf (lower_than(inf - t0, inf) fails). A winning key
-        // that is finite and below key_safe has a finite time below 1e300, so the step flips
-        // and t stays finite: its time is computed after the step, off the critical path, by
-        // the workgroup that holds the winner. Otherwise the time is needed now.
-        const float key = key_value((uint32_t)(pc >> 32));
-        const bool safe = key < key_safe && key > -INFINITY;
-        bool flip = true;
-        if (!safe) {
-          double dt = 0.0;
-          if (wr == (unsigned long long)rank) dt = waiting_time();
-          if (G > 1) {
-            // the winner's workgroup publishes the time (word c) and then, with release order,
-            // the round number (word d); the others wait for the round with acquire order
-            TeamRec *ds = sT.dsl + round % 3u;
-            if (wr == (unsigned long long)rank && lane == 0) {
-              __hip_atomic_store(&ds->c, (unsigned long long)__double_as_longlong(dt), __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-              __hip_atomic_store(&ds->d, (unsigned long long)round, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            uint64_t spins = 0;
-            while (__hip_atomic_load(&ds->d, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != (unsigned long long)round) {
-              if (++spins > SPIN_LIMIT) {
-                err = 3;
-                break;
-              }
-              __builtin_amdgcn_s_sleep(1);
-            }
-            dt = __longlong_as_double((long long)__hip_atomic_load(&ds->c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-          }
-          flip = !isnan(dt);
-          const double add = flip ? dt : R.t_max;
-          t_inf = t_inf || !isfinite(add);
-          if (rank == 0 && lane == 0) {
-            __hip_atomic_store((unsigned long long *)&sT.adds[sd], (unsigned long long)__double_as_longlong(add),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (R.log) sT.log_dt[sd] = add;
-          }
-        }
-        att += 1;
-        sd += 1;
-        sw += flip ? 1 : 0;
-        if (rank == 0 && lane == 0 && R.log) {
-          const uint32_t wp = (uint32_t)(pc >> 2) & 0xFFFFFFu;
-          sT.log_idx[sd - 1] = flip ? (int)(wp >> XB) * nx + (int)(wp & XM) : -1;
-        }
-        const bool more = sd < R.max_steps && (hs_step_free(R.first_free, sd) || (att < att_stop && !t_inf));
-        const bool mine = safe && wr == (unsigned long long)rank;
-        ctl = (flip ? 1u : 0u) | (more ? 2u : 0u) | (mine ? 8u : 0u);
-        if (lane == 0 && mine) {
-          s_dsrc = (int)src;
-          s_dsd = (int)(sd - 1);
-        }
       }
       if (lane == 0) {
         s_win = pc;
@@ -447,17 +446,6 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
     }
     __syncthreads();
     const uint32_t ctl = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ctl);
-    if ((ctl & 8u) && wv == OW / 64 && lane == 0) {
-      // the deferred waiting time of this workgroup's winner, by the first drawer wave while the
-      // owners apply the step's field update
-      const float *in = s_in[step & 1][s_dsrc];
-      const int sidx = s_dsd;
-      const double dt = hs_neel_dt((double)in[0] - (double)in[1], PERP ? (double)in[2] - (double)in[3] : 0.0,
-                                   (double)in[4], I.beta64, mpf64, barrier, PERP, in[5], R.attempt_freq);
-      __hip_atomic_store((unsigned long long *)&sT.adds[sidx], (unsigned long long)__double_as_longlong(dt),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (R.log) sT.log_dt[sidx] = dt;
-    }
     if (ctl & 4u) break;
     step += 1;
     go = (ctl & 2u) != 0;
@@ -543,37 +531,6 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       }
     }
   }
-  if (defer) {
-    // every workgroup reports its deferred times done (the wave that wrote them, after a release
-    // fence); rank 0 then adds all of them to t in step order, as the steps would have
-    if (tid == OW || tid == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (tid == OW) __hip_atomic_fetch_add(sT.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (rank == 0 && wv == 0) {
-      uint64_t spins = 0;
-      while (__hip_atomic_load(sT.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (unsigned int)G) {
-        if (++spins > SPIN_LIMIT) {
-          err = 3;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      double tt = t_start;
-      for (int64_t b = 0; b < sd; b += 64) {
-        const int64_t i = b + lane;
-        const unsigned long long v =
-            i < sd ? __hip_atomic_load((unsigned long long *)&sT.adds[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-        const int nb = (int)(sd - b < 64 ? sd - b : 64);
-        for (int j = 0; j < nb; ++j) {
-          const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
-          const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
-          tt += __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-        }
-      }
-      t = tt;
-    }
-  }
   // write back
 #pragma unroll
   for (int k = 0; k < MAXC; ++k) {
@@ -586,6 +543,19 @@ __global__ void __launch_bounds__(TT) k_neel_team(const TeamDev *teams, const in
       I.Ep[c] = ep[k];
       I.Epc[c] = epc[k];
     }
+  }
+  if (rank == 0 && wv == 0 && pend >= 0) {  // the last deferred time
+    uint64_t spins = 0;
+    unsigned long long dp;
+    for (;;) {
+      dp = __hip_atomic_load(&sT.rec[(size_t)pend_slot * G + pend].dd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((dp & 63ull) == pend_tag) break;
+      if (++spins > SPIN_LIMIT) {
+        err = 3;
+        break;
+      }
+    }
+    if (err != 3) t += __longlong_as_double((long long)(dp & ~63ull));
   }
   if (rank == 0 && tid == 0) {
     hs_state_t o = S0;
@@ -611,13 +581,7 @@ __global__ void k_team_reset(const TeamDev *teams, int32_t *err) {
   for (int i = threadIdx.x; i < 3 * d.G; i += blockDim.x) {
     d.rec[i].c = ~0ull;
     d.rec[i].d = ~0ull;
-  }
-  if (d.adds) {  // no round is ~0: every slot starts unwritten
-    if (threadIdx.x < 3) {
-      d.dsl[threadIdx.x].c = 0ull;
-      d.dsl[threadIdx.x].d = ~0ull;
-    }
-    if (threadIdx.x == 0) *d.done = 0u;
+    d.rec[i].dd = ~0ull;
   }
 }
 
@@ -626,7 +590,6 @@ inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Layout {
   size_t teams_off, inst_off, map_off, sync_off, err_off, total;
   std::vector<size_t> team_sync;  // offset of each team's sync block
-  std::vector<size_t> defer;      // offset of each team's deferred-time block (0: none)
   std::vector<int> G, first;
   int nblocks, maxc;
 };
@@ -669,17 +632,9 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
   L.sync_off = L.map_off + al256(sizeof(int32_t) * (size_t)L.nblocks);
   size_t off = L.sync_off;
   L.team_sync.resize(n);
-  L.defer.resize(n);
   for (int i = 0; i < n; ++i) {
     L.team_sync[i] = off;
     off += al256(sizeof(TeamRec) * 3 * (size_t)L.G[i]);
-    // deferred waiting times: t_max = +inf and a step budget the adds array can hold
-    const hs_run_t &r = teams[i].run;
-    L.defer[i] = 0;
-    if (std::isinf(r.t_max) && r.t_max > 0 && r.max_steps >= 1 && r.max_steps <= ADD_CAP) {
-      L.defer[i] = off;
-      off += al256(sizeof(TeamRec) * 3 + 256 + sizeof(double) * (size_t)r.max_steps);
-    }
   }
   L.err_off = off;
   L.total = off + 256;
@@ -687,10 +642,22 @@ int make_layout(const hs_team_t *teams, int n, Layout &L) {
 }
 
 template <int MAXC>
-hipError_t launch_neel(bool perp, int nblocks, hipStream_t s, const TeamDev *td, const int32_t *map, int32_t *err) {
-  void *args[] = {(void *)&td, (void *)&map, (void *)&err};
-  const void *fn = perp ? (const void *)&k_neel_team<MAXC, true> : (const void *)&k_neel_team<MAXC, false>;
-  return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(TT), args, 0, s);
+const void *team_kernel(bool perp, bool defer) {
+  if (defer) return perp ? (const void *)&k_neel_team<MAXC, true, true> : (const void *)&k_neel_team<MAXC, false, true>;
+  return perp ? (const void *)&k_neel_team<MAXC, true, false> : (const void *)&k_neel_team<MAXC, false, false>;
+}
+
+// The k_neel_team instance for the largest owned-magnet count, the E_perp setting and whether
+// deferred waiting times are compiled in (only worth it when every team runs with t_max = +inf
+// and no step log; the kernel without them keeps the shorter step of the other calls).
+const void *team_kernel_of(int maxc, bool perp, bool defer) {
+  switch (maxc) {
+    case 1: return team_kernel<1>(perp, defer);
+    case 2: return team_kernel<2>(perp, defer);
+    case 4: return team_kernel<4>(perp, defer);
+    case 8: return team_kernel<8>(perp, defer);
+    default: return team_kernel<16>(perp, defer);
+  }
 }
 
 }  // namespace
@@ -739,14 +706,9 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   HS_HIP_TRY(hipGetDevice(&dev));
   int cus = 0, per = 0;
   HS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const void *fn = nullptr;
-  switch (L.maxc) {
-    case 1: fn = perp ? (const void *)&k_neel_team<1, true> : (const void *)&k_neel_team<1, false>; break;
-    case 2: fn = perp ? (const void *)&k_neel_team<2, true> : (const void *)&k_neel_team<2, false>; break;
-    case 4: fn = perp ? (const void *)&k_neel_team<4, true> : (const void *)&k_neel_team<4, false>; break;
-    case 8: fn = perp ? (const void *)&k_neel_team<8, true> : (const void *)&k_neel_team<8, false>; break;
-    default: fn = perp ? (const void *)&k_neel_team<16, true> : (const void *)&k_neel_team<16, false>; break;
-  }
+  bool defer = true;
+  for (int i = 0; i < nteams; ++i) defer = defer && teams[i].run.t_max == INFINITY && !teams[i].run.log;
+  const void *fn = team_kernel_of(L.maxc, perp, defer);
   HS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, TT, 0));
   if ((int64_t)per * cus < L.nblocks)
     return hs_set_error(HS_ERR_ARG, "%d workgroups requested but only %d can be co-resident (%d per CU x %d CUs)",
@@ -766,9 +728,6 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     d.log_idx = teams[i].log_idx;
     d.log_dt = teams[i].log_dt;
     d.rec = (TeamRec *)(base + L.team_sync[i]);
-    d.dsl = L.defer[i] ? (TeamRec *)(base + L.defer[i]) : nullptr;
-    d.done = L.defer[i] ? (unsigned int *)(base + L.defer[i] + sizeof(TeamRec) * 3) : nullptr;
-    d.adds = L.defer[i] ? (double *)(base + L.defer[i] + sizeof(TeamRec) * 3 + 256) : nullptr;
     d.G = L.G[i];
     d.first = L.first[i];
     td[i] = d;
@@ -778,16 +737,10 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   int32_t *err = (int32_t *)(base + L.err_off);
   k_team_reset<<<nteams, 256, 0, s>>>((const TeamDev *)base, err);
   HS_LAUNCH_CHECK("k_team_reset");
-  hipError_t e;
   const TeamDev *dtd = (const TeamDev *)(base + L.teams_off);
   const int32_t *dmap = (const int32_t *)(base + L.map_off);
-  switch (L.maxc) {
-    case 1: e = launch_neel<1>(perp, L.nblocks, s, dtd, dmap, err); break;
-    case 2: e = launch_neel<2>(perp, L.nblocks, s, dtd, dmap, err); break;
-    case 4: e = launch_neel<4>(perp, L.nblocks, s, dtd, dmap, err); break;
-    case 8: e = launch_neel<8>(perp, L.nblocks, s, dtd, dmap, err); break;
-    default: e = launch_neel<16>(perp, L.nblocks, s, dtd, dmap, err); break;
-  }
+  void *args[] = {(void *)&dtd, (void *)&dmap, (void *)&err};
+  const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(L.nblocks), dim3(TT), args, 0, s);
   if (e != hipSuccess) return hs_set_error(HS_ERR_HIP, "cooperative launch of k_neel_team failed: %s", hipGetErrorString(e));
   return HS_OK;
 }
