@@ -39,13 +39,14 @@ class DeviceState:
         self.Epc = torch.zeros(n, dtype=torch.float32, device=dev)
         self.EB = torch.zeros(n, dtype=torch.float32, device=dev)
         self.beta = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.beta64c = None  # FP64 per-cell beta, kept only while T is not uniform (Neel waiting times)
         self.mu = torch.zeros(n, dtype=torch.float32, device=dev)
         self.mu64 = torch.zeros(n, dtype=torch.float64, device=dev)
         self.pw64 = None  # FP64 pair weights when they are not the moments (DiMonopolarEnergy)
         self.src_ones = None  # source weights of DiMonopolarEnergy (ones)
         self.occ_list = torch.zeros(max(1, n), dtype=torch.int32, device=dev)
         self.state = torch.zeros(2 * _lib.STATE_BYTES, dtype=torch.uint8, device=dev)
-        self.partials = torch.zeros(2 * 1024 * 32, dtype=torch.uint8, device=dev)
+        self.partials = torch.zeros(2 * 1024 * 32 + 4 * 1024, dtype=torch.uint8, device=dev)
         self.zee_bx = None
         self.zee_by = None
         self.kern = None  # dict of kernel tensors + window

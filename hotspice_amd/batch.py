@@ -34,6 +34,22 @@ def progress_many(magnets, t_max=1.0, MCsteps_max=4.0, nblocks=None, chunk=None,
     if not mags:
         return []
     scheme = _check_same_scheme(mags)
+    if scheme == Scheme.NEEL and not all(m._uniform_beta for m in mags):
+        # a temperature that varies over the grid steps through the per-step launches (the
+        # persistent team kernel needs a uniform one): those instances go one at a time
+        nbl = nblocks if isinstance(nblocks, (list, tuple)) else [nblocks] * len(mags)
+        out = [None] * len(mags)
+        uni = [i for i, m in enumerate(mags) if m._uniform_beta]
+        if uni:
+            res = progress_many([mags[i] for i in uni], t_max, MCsteps_max, [nbl[i] for i in uni], chunk,
+                                None if per_instance is None else [per_instance[i] for i in uni], **kwargs)
+            for i, r in zip(uni, res):
+                out[i] = r
+        for i, m in enumerate(mags):
+            if out[i] is None:
+                kw = dict(kwargs, **(per_instance[i] if per_instance is not None else {}))
+                out[i] = m.progress(t_max=t_max, MCsteps_max=MCsteps_max, **kw)
+        return out
     dev = mags[0]._dev
     for m in mags:
         if m._dev.device != dev.device:
@@ -109,7 +125,8 @@ def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
     dev = mags[0]._dev
     for m in mags:
         if not m._uniform_beta:
-            raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
+            raise NotImplementedError("the batched Neel launch (persistent team kernel) needs a uniform temperature; "
+                                      "progress_many steps instances with a varying one through Magnets.progress")
     free = [0] * len(items) if free is None else list(free)
     # one launch per E_perp setting: the team kernel is compiled for either, and instances are
     # independent, so two launches in stream order step each exactly as one would

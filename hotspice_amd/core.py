@@ -355,6 +355,13 @@ class Magnets(ABC):
         self._T_any_zero = bool(np.any(self._T == 0))
         self._dev.upload(self._dev.beta, self.beta.astype(np.float32))
         self._uniform_beta = bool(np.all(self.beta == self.beta.reshape(-1)[0]))
+        if self._uniform_beta:
+            self._dev.beta64c = None
+        else:
+            if self._dev.beta64c is None:
+                self._dev.beta64c = self._dev.torch.zeros(self.nx * self.ny, dtype=self._dev.torch.float64,
+                                                          device=self._dev.device)
+            self._dev.upload(self._dev.beta64c, self.beta.astype(np.float64))
         if hasattr(self, "_energies"):
             self._device_params_changed()
 
@@ -701,6 +708,7 @@ class Magnets(ABC):
         I.m, I.flags, I.E, I.Ec = P(d.m), P(d.flags), P(d.E), P(d.Ec)
         I.Ep, I.Epc, I.EB, I.beta, I.mu, I.mu64 = P(d.Ep), P(d.Epc), P(d.EB), P(d.beta), P(d.mu), P(d.mu64)
         I.occ_list = P(d.occ_list)
+        I.beta64c = P(d.beta64c)
         I.seed = self.seed
         I.nx, I.ny, I.ux, I.uy = self.nx, self.ny, self.unitcell.x, self.unitcell.y
         I.pbc, I.in_plane, I.use_perp = int(self.PBC), int(self.in_plane), int(self.USE_PERP_ENERGY)
@@ -807,6 +815,12 @@ class Magnets(ABC):
             case other:
                 raise ValueError(f"The Magnets.params object has an invalid value for UPDATE_SCHEME='{other}'.")
 
+    def _neel_team_ok(self):
+        """The persistent team kernel steps uniform temperatures; a temperature that varies over
+        the grid takes the per-step launches (two per step: the grid's minimum barrier reorders
+        the waiting times, core.py:686-699)."""
+        return self.neel_persistent and self._uniform_beta
+
     def _neel_nblocks(self):
         """Team size of the next Neel launch ('auto' runs the autotuner once)."""
         if self.neel_blocks == 'auto':
@@ -837,9 +851,7 @@ class Magnets(ABC):
         self._maybe_recompute()
         run = _lib.Run(t_max=float(t_max), mc_max=math.inf, attempt_freq=float(attempt_freq), max_steps=1, log=1,
                        nblocks=0, begin=1, first_free=1)
-        if not self._uniform_beta:
-            raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
-        self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self._neel_nblocks())
+        self._dev.neel_run(run, persistent=self._neel_team_ok(), nblocks=self._neel_nblocks())
         self._after_device_step()
         idx = int(self._dev.log_idx[0].item())
         st = self._state()
@@ -1019,11 +1031,9 @@ class Magnets(ABC):
                     chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
                 run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
                                max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
-                if not self._uniform_beta:
-                    raise NotImplementedError("Neel stepping on the device needs a spatially uniform temperature.")
                 if begin:
                     self._set_progress_origin(t0, a0)
-                self._dev.neel_run(run, persistent=self.neel_persistent, nblocks=self._neel_nblocks())
+                self._dev.neel_run(run, persistent=self._neel_team_ok(), nblocks=self._neel_nblocks())
                 begin = False
                 st = self._state()
                 if st.err:

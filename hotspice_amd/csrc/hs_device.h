@@ -231,6 +231,19 @@ HS_INLINE unsigned long long hs_wave_min_pay2(unsigned long long v, unsigned lon
   return m;
 }
 
+// Minimum over the workgroup (every thread calls it; sh holds blockDim.x / 64 floats). fminf
+// drops NaN operands, as nanmin does.
+HS_INLINE float hs_block_min_f32(float v, float *sh) {
+  for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float m = sh[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fminf(m, sh[q]);
+  __syncthreads();
+  return m;
+}
+
 // A 64-bit value of the wave's first active lane, in every lane.
 HS_INLINE unsigned long long hs_uniform_u64(unsigned long long v) {
   return ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32) |

@@ -11,6 +11,10 @@
 //      log(tau) = log(-ln u) + b*beta (or the two-channel form), and writes the block's best.
 // The grid-wide minimum of b (core.py:688, 696) only rescales all taus when beta is uniform, so
 // it is not needed for the argmin; the winner's dt = -ln(u)*exp(b*beta)/f0 is evaluated in FP64.
+// With a temperature that varies over the grid the minimum does reorder the taus
+// (tau_i = -ln(u_i) exp((b_i - b_min) beta_i)): each step then takes two launches, the first
+// (update, as above) leaving per-block minima of the barriers, the second (k_neel_keys_nu)
+// reducing them to b_min and forming the keys. b_min cancels in the winner's time.
 // Blocks never wait for each other inside a launch, so there is no grid barrier to hang.
 #include <hip/hip_runtime.h>
 
@@ -52,10 +56,31 @@ HS_INLINE float log_rate_f32(const hs_instance_t &I, float e, float ep, float eb
   return -b * beta;
 }
 
+// FP32 barriers of a cell (core.py:641-655): lo = min(b1, b2) and hi = max(b1, b2) (E_perp),
+// else lo = hi = b.
+template <bool PERP>
+HS_INLINE void barriers_f32(const hs_instance_t &I, float e, float ep, float eb, float &lo, float &hi) {
+  const float dE = -2.f * e, El = e;
+  if (PERP) {
+    const float epf = (float)I.m_perp_factor * ep;
+    const float b1 = fmaxf(dE, (eb + epf) - El), b2 = fmaxf(dE, (eb - epf) - El);
+    lo = fminf(b1, b2);
+    hi = fmaxf(b1, b2);
+    return;
+  }
+  if (I.barrier == 0) {
+    lo = fmaxf(dE, eb - El);
+  } else {
+    const float q = dE * (0.25f / eb) + 1.f;
+    lo = dE > 4.f * eb ? dE : fminf(eb * (q * q), dE + 4.f * eb);
+  }
+  hi = lo;
+}
+
 template <bool PERP>
 __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_run_t R, hs_state_t *st,
                                                         NeelPartial *parts, int32_t *log_idx, double *log_dt,
-                                                        int j, int last) {
+                                                        int j, int last, float *bparts) {
   __shared__ float s_key[NB_THREADS / 64];
   __shared__ int s_idx[NB_THREADS / 64];
   __shared__ int s_slot[NB_THREADS / 64];
@@ -249,6 +274,12 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
       }
     }
     if (compute && (I.flags[c] & HS_FLAG_NEEL)) {
+      if (bparts) {  // varying temperature: this step's keys wait for the grid's minimum barrier
+        float lo, hi;
+        barriers_f32<PERP>(I, e, ep, I.EB[c], lo, hi);
+        bk = fminf(bk, lo);  // nanmin: NaN barriers drop out
+        continue;
+      }
       uint32_t wa, wb;
       hs_words(I.seed, (uint64_t)step, 1u, (uint64_t)c, wa, wb);
       const float x = hs_exp1_fast(wa, wb);
@@ -258,6 +289,11 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
     }
   }
   if (!compute) return;
+  if (bparts) {
+    bk = hs_block_min_f32(bk, s_key);
+    if (tid == 0) bparts[blockIdx.x] = bk;
+    return;
+  }
   hs_wave_argmin(bk, bi);
   __syncthreads();
   if ((tid & 63) == 0) {
@@ -280,6 +316,66 @@ __global__ void __launch_bounds__(NB_THREADS) k_neel_step(hs_instance_t I, hs_ru
       hs_words(I.seed, (uint64_t)step, 1u, (uint64_t)c, wa, wb);
       const double ep64 = PERP ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
       p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], I.beta64, I.m_perp_factor,
+                        I.barrier, PERP, hs_exp1_fast(wa, wb), R.attempt_freq);
+      p.mw = I.m[c];
+    }
+    parts[(size_t)(j & 1) * P + blockIdx.x] = p;
+  }
+}
+
+// Second launch of a step with a varying temperature: b_min from the first launch's per-block
+// minima, then the keys log(tau_i) = log(-ln u_i) + (b_i - b_min) beta_i (one channel) or
+// log(-ln u_i) + (lo_i - b_min) beta_i - log(1 + exp(-(hi_i - lo_i) beta_i)) (two channels,
+// core.py:686-699) and the block's argmin, with the candidate's FP64 time from its own beta.
+template <bool PERP>
+__global__ void __launch_bounds__(NB_THREADS) k_neel_keys_nu(hs_instance_t I, hs_run_t R, const hs_state_t *st,
+                                                           NeelPartial *parts, const float *bparts, int j) {
+  __shared__ float s_key[NB_THREADS / 64];
+  __shared__ int s_idx[NB_THREADS / 64];
+  const int P = gridDim.x, tid = threadIdx.x;
+  const hs_state_t S = st[(j + 1) & 1];
+  if (S.done) return;
+  float bmin = INFINITY;
+  for (int b = tid; b < P; b += NB_THREADS) bmin = fminf(bmin, bparts[b]);
+  bmin = hs_block_min_f32(bmin, s_key);
+  const uint64_t step = (uint64_t)S.step;
+  const int ncell = I.nx * I.ny;
+  float bk = INFINITY;
+  int bi = 0x7fffffff;
+  for (int c = blockIdx.x * NB_THREADS + tid; c < ncell; c += P * NB_THREADS) {
+    if (!(I.flags[c] & HS_FLAG_NEEL)) continue;
+    const float e = I.E[c], ep = PERP ? I.Ep[c] : 0.f, beta = I.beta[c];
+    float lo, hi;
+    barriers_f32<PERP>(I, e, ep, I.EB[c], lo, hi);
+    uint32_t wa, wb;
+    hs_words(I.seed, step, 1u, (uint64_t)c, wa, wb);
+    float key = hs_log_fast(hs_exp1_fast(wa, wb)) + (lo - bmin) * beta;
+    if (PERP) key -= hs_log_fast(1.f + __builtin_amdgcn_exp2f(-(hi - lo) * beta * 1.44269504088896f));
+    if (isnan(key)) key = INFINITY;
+    hs_argmin_pair(bk, bi, key, c);
+  }
+  hs_wave_argmin(bk, bi);
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    s_key[tid >> 6] = bk;
+    s_idx[tid >> 6] = bi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float k = s_key[0];
+    int i = s_idx[0];
+    for (int q = 1; q < NB_THREADS / 64; ++q) hs_argmin_pair(k, i, s_key[q], s_idx[q]);
+    NeelPartial p{};
+    p.key = k;
+    p.idx = (i == 0x7fffffff) ? -1 : i;
+    p.dt = NAN;
+    p.mw = 0;
+    if (p.idx >= 0) {
+      const int c = p.idx;
+      uint32_t wa, wb;
+      hs_words(I.seed, step, 1u, (uint64_t)c, wa, wb);
+      const double ep64 = PERP ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
+      p.dt = hs_neel_dt((double)I.E[c] - (double)I.Ec[c], ep64, (double)I.EB[c], I.beta64c[c], I.m_perp_factor,
                         I.barrier, PERP, hs_exp1_fast(wa, wb), R.attempt_freq);
       p.mw = I.m[c];
     }
@@ -320,11 +416,14 @@ extern "C" int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_st
   if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
   if (run->log && (!log_idx || !log_dt)) return hs_set_error(HS_ERR_ARG, "log requested without arrays");
   if (run->nblocks < 0 || run->nblocks > 1024) return hs_set_error(HS_ERR_ARG, "nblocks must be in [0, 1024]");
-  if (!inst->uniform_beta)
-    return hs_set_error(HS_ERR_ARG, "Neel stepping needs a uniform temperature (beta) over the grid");
-  if (!(inst->beta64 > 0.0)) return hs_set_error(HS_ERR_ARG, "Neel stepping needs beta64 > 0 (FP64 1/(kB T))");
+  const bool nu = !inst->uniform_beta;
+  if (nu && !inst->beta64c)
+    return hs_set_error(HS_ERR_ARG, "Neel stepping with a varying temperature needs beta64c (FP64 1/(kB T) per cell)");
+  if (!nu && !(inst->beta64 > 0.0)) return hs_set_error(HS_ERR_ARG, "Neel stepping needs beta64 > 0 (FP64 1/(kB T))");
   hipStream_t s = (hipStream_t)stream;
   const int P = hs_neel_blocks(inst, run->nblocks);
+  // the barrier minima of a varying-temperature step follow the two partial buffers
+  float *bparts = nu ? (float *)((NeelPartial *)partials + 2 * (size_t)P) : nullptr;
   k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
   HS_LAUNCH_CHECK("k_state_begin");
   const hs_run_t R = *run;
@@ -332,10 +431,19 @@ extern "C" int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_st
   for (int64_t j = 0; j <= N; ++j) {
     const int last = (j == N);
     if (inst->use_perp)
-      k_neel_step<true><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, log_idx, log_dt, (int)j, last);
+      k_neel_step<true><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, log_idx, log_dt, (int)j, last,
+                                                 bparts);
     else
-      k_neel_step<false><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, log_idx, log_dt, (int)j, last);
+      k_neel_step<false><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, log_idx, log_dt, (int)j, last,
+                                                  bparts);
     HS_LAUNCH_CHECK("k_neel_step");
+    if (nu && !last) {
+      if (inst->use_perp)
+        k_neel_keys_nu<true><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, bparts, (int)j);
+      else
+        k_neel_keys_nu<false><<<P, NB_THREADS, 0, s>>>(*inst, R, state, (NeelPartial *)partials, bparts, (int)j);
+      HS_LAUNCH_CHECK("k_neel_keys_nu");
+    }
   }
   const int from = (int)((N + 1) & 1);
   if (from != 0) {

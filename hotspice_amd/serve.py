@@ -109,7 +109,7 @@ class _Active:
         if self.scheme not in (Scheme.NEEL, Scheme.METROPOLIS):
             raise JobError(f"{self.scheme} jobs cannot be served")
         if self.scheme == Scheme.NEEL and not mm._uniform_beta:
-            raise JobError("Neel stepping on the device needs a spatially uniform temperature")
+            raise JobError("served Neel jobs run in the persistent team kernel, which needs a uniform temperature")
         self.sel = None
         if self.scheme == Scheme.METROPOLIS:
             r = job.kwargs.get('r', 0)

@@ -66,6 +66,8 @@ typedef struct hs_instance {
   const float *zee_bx;  /* optional (ny,nx) field x-component (OOP: magnitude); NULL = uniform */
   const float *zee_by;  /* optional (ny,nx) field y-component; NULL = uniform */
   const int32_t *occ_list; /* [n_occ] flat indices of occupied cells, row-major (core.py:97-98) */
+  const double *beta64c;  /* (ny,nx) 1/(kB T) in FP64: Neel stepping with uniform_beta = 0 needs it
+                             (the per-cell waiting times); NULL otherwise */
   uint64_t seed;        /* Philox key */
   double zee_Bx, zee_By; /* uniform field components when zee_bx/zee_by are NULL */
   double dip_pf;        /* DipolarEnergy.prefactor (0 = no dipolar energy) */
@@ -310,8 +312,11 @@ int hs_zeeman_swap(const hs_instance_t *inst, double Bx0, double By0, const floa
  * lower_than(t-t0, t_max) and lower_than(MCsteps-MCsteps0, MCsteps_max).
  * state: device hs_state_t[2] (copy 0 holds the input state; the result is in copy
  * (steps_launched+1)&1, reported through *final_copy). partials: device scratch of
- * 32*nblocks*2 bytes. log_idx/log_dt: optional device arrays [max_steps] (flat winner index or
- * -1 when no switch; dt added to t). Needs uniform_beta = 1 and beta64 > 0. Non-blocking.
+ * 32*nblocks*2 bytes, plus 4*nblocks when uniform_beta = 0. log_idx/log_dt: optional device
+ * arrays [max_steps] (flat winner index or -1 when no switch; dt added to t). Needs beta64 > 0
+ * with uniform_beta = 1; with a temperature that varies over the grid (uniform_beta = 0) it
+ * needs beta64c and takes two launches per step (the grid's minimum barrier reorders the
+ * waiting times, core.py:688-699). Non-blocking.
  */
 int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_state_t *state, void *partials,
                 int32_t *log_idx, double *log_dt, int32_t *final_copy, void *stream);

@@ -20,6 +20,16 @@ CASES = {
     "neel_kagome_tmax": dict(lattice="IP_Kagome", a=1e-6, n=20, T=300, moment=2.37e-16,
                              energies=[("dipolar", {}), ("zeeman", dict(magnitude=5e-4, angle=0.3)), ("exchange", dict(J_kT=0.3))],
                              params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=1e-3), nsteps=120, seed=13),
+    # a temperature that varies over the grid (T_ramp: T * (1 + T_ramp * (x / (nx-1) - 1/2)), the
+    # array stored in the fixture): the grid's minimum barrier reorders the Neel waiting times
+    # (core.py:686-699)
+    "neel_pinwheel_tramp": dict(lattice="IP_Pinwheel", a=1e-6, n=24, T=300, T_ramp=0.6, moment=2.37e-16,
+                                energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=0.4))],
+                                params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=np.inf), nsteps=200, seed=14),
+    "neel_oop_tramp_parabolic": dict(lattice="OOP_Square", a=1e-6, n=20, T=300, T_ramp=0.8, moment=1.1e-15,
+                                     energies=[("dipolar", {}), ("exchange", dict(J_kT=0.3))],
+                                     params=dict(UPDATE_SCHEME="Néel", ENERGY_BARRIER_METHOD="parabolic"),
+                                     update_kw=dict(t_max=1e-3), nsteps=200, seed=15),
     "metro_oop_sum": dict(lattice="OOP_Square", a=1e-6, n=32, T=300, moment=1.1e-15,
                           energies=[("dipolar", {})], params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0),
                           update_kw=dict(r=4e-6), nsteps=150, seed=21),
@@ -137,8 +147,12 @@ def load(name):
         g = lattices.make(sp["lattice"], sp["a"], n=sp["n"], PBC=sp.get("kw", {}).get("PBC", False))
         shape = (g.ny, g.nx)
         kT = kB * sp["T"]
-        return dict(pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0),
-                    E_B=20 * kT * rng.normal(1, 0.05, size=shape))
+        fx = dict(pattern=np.where(rng.random(shape) < 0.5, -1.0, 1.0),
+                  E_B=20 * kT * rng.normal(1, 0.05, size=shape))
+        if "T_ramp" in sp:
+            ramp = np.linspace(-0.5, 0.5, shape[1])[None, :] * np.ones((shape[0], 1))
+            fx["T"] = sp["T"] * (1 + sp["T_ramp"] * ramp)
+        return fx
     return dict(np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False))
 
 
@@ -160,7 +174,7 @@ def oracle_magnets(name, fx=None):
             ens.append(model.Exchange(J=kw["J_kT"] * kT))
     params = dict(sp["params"])
     params["UPDATE_SCHEME"] = "Neel" if params["UPDATE_SCHEME"] in ("Néel", "Neel") else params["UPDATE_SCHEME"]
-    return model.make_magnets(sp["lattice"], sp["a"], n=sp["n"], T=sp["T"], E_B=fx["E_B"],
+    return model.make_magnets(sp["lattice"], sp["a"], n=sp["n"], T=fx.get("T", sp["T"]), E_B=fx["E_B"],
                               moment=sp["moment"], pattern=fx["pattern"], energies=ens, params=params,
                               seed=sp["seed"], PBC=sp.get("kw", {}).get("PBC", False))
 
@@ -182,7 +196,7 @@ def product_magnets(name, fx=None, **extra):
         else:
             ens.append(hs.ExchangeEnergy(J=kw["J_kT"] * kT))
     cls = getattr(hs.ASI, sp["lattice"])
-    return cls(sp["a"], sp["n"], T=sp["T"], E_B=fx["E_B"], moment=sp["moment"], pattern=fx["pattern"],
+    return cls(sp["a"], sp["n"], T=fx.get("T", sp["T"]), E_B=fx["E_B"], moment=sp["moment"], pattern=fx["pattern"],
                energies=ens, params=hs.SimParams(**sp["params"]), seed=sp["seed"], **sp.get("kw", {}), **extra)
 
 

@@ -157,9 +157,13 @@ def run_case(core, energies, ASI, spec):
         elif e[0] == "exchange":
             ens.append(energies.ExchangeEnergy(J=e[1]["J_kT"] * kT))
     params = core.SimParams(**spec["params"])
+    # a temperature that varies over the grid: T * (1 + T_ramp * (x / (nx-1) - 1/2))
+    T = spec["T"]
+    if "T_ramp" in spec:
+        T = spec["T"] * (1 + spec["T_ramp"] * np.linspace(-0.5, 0.5, shape[1])[None, :] * np.ones((shape[0], 1)))
     # An array pattern cannot be passed to the constructor (core.py:172 reads self.shape before
     # self.m exists), so the state is set afterwards with initialize_m (core.py:163-180).
-    mm = cls(spec["a"], spec["n"], T=spec["T"], E_B=E_B, moment=spec["moment"], pattern="uniform",
+    mm = cls(spec["a"], spec["n"], T=T, E_B=E_B, moment=spec["moment"], pattern="uniform",
              energies=ens, params=params, **kw)
     mm.initialize_m(pattern)
     rep = Replay(mm, spec["seed"])
@@ -183,6 +187,7 @@ def run_case(core, energies, ASI, spec):
 
     mm.update = update
     out = dict(pattern=pattern, E_B=E_B, occupation=occ, m0=mm.m.copy(), E0=mm.E.copy(),
+               **({"T": np.asarray(T, dtype=float)} if "T_ramp" in spec else {}),
                Eperp0=(mm.E_perp.copy() if mm.USE_PERP_ENERGY else np.zeros(shape)),
                E_tot0=np.float64(mm.E_tot))
     E_dip0 = mm.get_energy("dipolar").E.copy() if mm.has_energy("dipolar") else np.zeros(shape)
@@ -238,6 +243,17 @@ CASES = {
     "neel_kagome_tmax": dict(lattice="IP_Kagome", a=1e-6, n=20, T=300, moment=2.37e-16, EB_kT=40, EB_dist="normal",
                              energies=[("dipolar", {}), ("zeeman", dict(magnitude=5e-4, angle=0.3)), ("exchange", dict(J_kT=0.3))],
                              params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=1e-3), nsteps=120, snap_every=40, seed=13),
+    # Neel with a temperature that varies over the grid: the grid's minimum barrier reorders the
+    # waiting times (core.py:686-699); pinwheel with perp channels, OOP square with parabolic
+    # barriers and a finite t_max
+    "neel_pinwheel_tramp": dict(lattice="IP_Pinwheel", a=1e-6, n=24, T=300, T_ramp=0.6, moment=2.37e-16, EB_kT=60,
+                                EB_dist="lognormal", energies=[("dipolar", {}), ("zeeman", dict(magnitude=1e-3, angle=0.4))],
+                                params=dict(UPDATE_SCHEME="Néel"), update_kw=dict(t_max=np.inf), nsteps=200, snap_every=50,
+                                seed=14),
+    "neel_oop_tramp_parabolic": dict(lattice="OOP_Square", a=1e-6, n=20, T=300, T_ramp=0.8, moment=1.1e-15, EB_kT=20,
+                                     EB_dist="normal", energies=[("dipolar", {}), ("exchange", dict(J_kT=0.3))],
+                                     params=dict(UPDATE_SCHEME="Néel", ENERGY_BARRIER_METHOD="parabolic"),
+                                     update_kw=dict(t_max=1e-3), nsteps=200, snap_every=50, seed=15),
     # Metropolis, OOP square, full kernel, grid selection r=4a, sum branch (<= 20 per sublattice)
     "metro_oop_sum": dict(lattice="OOP_Square", a=1e-6, n=32, T=300, moment=1.1e-15, EB_kT=20, EB_dist="normal",
                           energies=[("dipolar", {})], params=dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0),

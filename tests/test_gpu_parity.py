@@ -287,3 +287,21 @@ def test_dimonopolar_fields_equal_oracle(lat, n, kw):
     assert mm.E_tot == pytest.approx(om.E_tot, rel=1e-9, abs=1e-9 * scale)
     d = mm.get_energy('dimonopolar')
     np.testing.assert_allclose(d.E, om.get_energy('dimonopolar').E, rtol=0, atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("name", ["neel_pinwheel_tramp", "neel_oop_tramp_parabolic"])
+def test_progress_varying_temperature_matches_oracle(name):
+    """progress() with a temperature that varies over the grid (two launches per step on the
+    device, core.py:686-699 looped as core.py:736-760) against the oracle's progress()."""
+    fx = load(name)
+    mm = product_magnets(name, fx)
+    om = oracle_magnets(name, fx)
+    assert not mm._uniform_beta
+    for kw in (dict(t_max=math.inf, MCsteps_max=0.3), dict(t_max=1e-3, MCsteps_max=0.2)):
+        a = mm.progress(**kw)
+        b = om.progress(**kw)
+        assert mm.attempted_switches == om.attempted_switches
+        assert mm.switches == om.switches
+        np.testing.assert_array_equal(mm.m, om.m)
+        assert a[1] == pytest.approx(b[1], rel=1e-12)
+        assert mm.t == pytest.approx(om.t, rel=1e-4)

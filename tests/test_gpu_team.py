@@ -271,3 +271,22 @@ def test_metropolis_graph_replay_equals_plain_launches(names):
             assert all(m.attempted_switches > 0 for m in graphed)
     finally:
         _lib.set_graph_steps(old)
+
+
+def test_varying_temperature_instance_in_a_neel_batch():
+    """A Neel batch with an instance whose temperature varies over the grid: that instance steps
+    through the per-step launches, the others in the team launch; each ends as it does alone."""
+    from cases import load, product_magnets
+    from hotspice_amd import batch
+
+    def make():
+        return [_pinwheel(32, 3), product_magnets("neel_pinwheel_tramp", load("neel_pinwheel_tramp")), _oop(24, 4)]
+
+    alone = make()
+    for m in alone:
+        m.progress(t_max=math.inf, MCsteps_max=0.5)
+    mixed = make()
+    res = batch.progress_many(mixed, t_max=math.inf, MCsteps_max=0.5)
+    for m1, m2, r in zip(alone, mixed, res):
+        _same(_snapshot(m1), _snapshot(m2))
+        assert r[1] == pytest.approx(m1.MCsteps, rel=1e-12)

@@ -52,7 +52,7 @@ def test_golden_specs_match():
     for name, spec in list(CASES.items()) + list(PROGRESS.items()):
         fx = load(name)
         text = str(fx["spec"])
-        for key in ("lattice", "n", "T", "moment", "seed", "nsteps", "calls"):
+        for key in ("lattice", "n", "T", "T_ramp", "moment", "seed", "nsteps", "calls"):
             if key in spec:
                 assert repr(spec[key]) in text or str(spec[key]) in text, (name, key)
         assert f'"{name}"' in src
