@@ -309,8 +309,12 @@ def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk, sts):
 def _progress_many_metropolis(mags, starts, t_max, mc, f0, chunk, kws, sts):
     sels = []
     for m, kw in zip(mags, kws):
-        if kw.get('idx') is not None:
-            raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
+        if kw.get('idx') is not None:  # the instance's own sample list, every step
+            sel = m._explicit_selection(kw['idx'])
+            if sel is None:
+                raise ValueError("progress() with an empty idx never advances t or MCsteps")
+            sels.append(sel)
+            continue
         r = kw.get('r', 0)
         r = m.calc_r(kw.get('Q', 0.05)) if r == 0 else r
         m._metro_nblocks(r)  # 'auto' instances are tuned first

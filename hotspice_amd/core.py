@@ -862,14 +862,55 @@ class Magnets(ABC):
         self._since_recompute += 1
         return divmod(np.int64(idx), np.int64(self.nx))
 
+    def _explicit_selection(self, idx):
+        """The selection of update(idx=...) (core.py:660-661): the caller's (2, k) array of [y; x]
+        sample positions, in its order, as a device list (None when k = 0). The samples must be
+        distinct magnets of the grid."""
+        idx = np.asarray(idx)
+        if idx.ndim != 2 or idx.shape[0] != 2 or not np.issubdtype(idx.dtype, np.integer):
+            raise ValueError("idx must be an integer array of shape (2, k) holding [y; x] positions")
+        if idx.shape[1] == 0:
+            return None
+        y, x = idx[0].astype(np.int64), idx[1].astype(np.int64)
+        if np.any((y < 0) | (y >= self.ny) | (x < 0) | (x >= self.nx)):
+            raise ValueError("idx holds a position outside the grid")
+        if not np.all(np.asarray(self.occupation)[y, x] != 0):
+            raise ValueError("idx holds a position without a magnet")
+        flat = y * self.nx + x
+        if np.unique(flat).size != flat.size:
+            raise ValueError("idx holds a position more than once")
+        sel = _lib.Select()
+        sel.scheme = 3
+        sel.nsc = int(flat.size)
+        sel.cutoff = self.params.SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF
+        sel.reduced_k = self.params.REDUCED_KERNEL_SIZE
+        sel.metropolis_time = int(bool(self.params.METROPOLIS_TIME))
+        t = self._dev.torch
+        cells = t.from_numpy(flat.astype(np.int32)).to(self._dev.device)
+        sel.pat_cells = cells.data_ptr()
+        sel._keep = cells  # the device list lives as long as the selection
+        return sel
+
+    def _skip_step(self):
+        """A step with an empty sample list: nothing is attempted, but the step counter (the key
+        of the random streams) advances as for any update()."""
+        st = self._state()
+        st.step += 1
+        self._dev.write_state(st)
+
     def _update_Metropolis(self, idx=None, Q=0.05, r=0, attempt_freq=1e10):
         """core.py:659-677: returns idx_switch, a (2, a) array of [y; x]."""
-        if idx is not None:
-            raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
         self._pre_step()
-        r = self.calc_r(Q) if r == 0 else r
-        nblocks = self._metro_nblocks(r)
-        sel = self._select_geometry(r)
+        if idx is not None:
+            sel = self._explicit_selection(idx)
+            if sel is None:
+                self._skip_step()
+                return np.zeros((2, 0), dtype=np.int64)
+            nblocks = self._metro_grid() if self.metro_blocks != 'auto' else 0
+        else:
+            r = self.calc_r(Q) if r == 0 else r
+            nblocks = self._metro_nblocks(r)
+            sel = self._select_geometry(r)
         cap = max(1, sel.nsc)
         t = self._dev.torch
         cells = t.zeros(cap, dtype=t.int32, device=self._dev.device)
@@ -1043,11 +1084,15 @@ class Magnets(ABC):
         else:
             r = kwargs.get('r', 0)
             Q = kwargs.get('Q', 0.05)
-            if kwargs.get('idx') is not None:
-                raise NotImplementedError("Explicit sample lists (idx=...) are not supported by the device Metropolis step.")
-            r = self.calc_r(Q) if r == 0 else r
-            nblocks = self._metro_nblocks(r)
-            sel = self._select_geometry(r)
+            if kwargs.get('idx') is not None:  # every step samples the same list (core.py:757-758)
+                sel = self._explicit_selection(kwargs['idx'])
+                if sel is None:
+                    raise ValueError("progress() with an empty idx never advances t or MCsteps")
+                nblocks = self._metro_grid() if self.metro_blocks != 'auto' else 0
+            else:
+                r = self.calc_r(Q) if r == 0 else r
+                nblocks = self._metro_nblocks(r)
+                sel = self._select_geometry(r)
             while True:
                 per_step = self._metro_rate or max(1.0, 0.25 * sel.nsc)
                 st = self._state()

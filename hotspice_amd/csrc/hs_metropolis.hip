@@ -469,10 +469,23 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
       grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
     }
   }
+  bool bad_list = false;
   for (; !pre && attempt < MAX_ATTEMPTS; ++attempt) {
     const uint64_t b0 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0));
     const uint64_t b1 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1));
     const uint64_t b2 = hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 2));
+    if (SEL.scheme == 3) {  // the caller's samples (update(idx=...), core.py:660-661), in its order
+      int bad = 0;
+      for (int j = j0; j < j1; ++j) {
+        const int c = SEL.pat_cells[j];
+        const bool ok = c >= 0 && c < nx * ny && (I.flags[c] & HS_FLAG_OCC);
+        ws.cand[j] = ok ? c : -1;
+        bad += ok ? 0 : 1;
+      }
+      bad_list = block_sum_i64(bad, s_l) > 0;
+      nvalid = nsc;
+      break;
+    }
     if (SEL.scheme == 2) {
       const int tot_val = poisson_select(I, SEL, ws, step, attempt, s_i);
       if (tot_val > 0) {
@@ -546,9 +559,9 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
     }
   }
   __syncthreads();
-  if (attempt >= MAX_ATTEMPTS) {
+  if (attempt >= MAX_ATTEMPTS || bad_list) {
     if (tid == 0) {
-      S.err = 2;
+      S.err = bad_list ? 4 : 2;  // 4: a listed sample is outside the grid or not a magnet
       S.done = 1;
       st[0] = S;
       *ws.acc_n = 0;
@@ -1520,6 +1533,8 @@ int check_select(const hs_select_t *sel) {
         !(sel->r_y >= 1.5))
       return hs_set_error(HS_ERR_ARG, "bad Poisson geometry (background %dx%d, r %g x %g cells)", sel->snx, sel->sny,
                           sel->r_x, sel->r_y);
+  } else if (sel->scheme == 3) {
+    if (sel->nsc < 1 || !sel->pat_cells) return hs_set_error(HS_ERR_ARG, "an explicit sample list needs nsc >= 1 cells");
   } else if (sel->scheme != 1) {
     return hs_set_error(HS_ERR_ARG, "unsupported MULTISAMPLING_SCHEME");
   }

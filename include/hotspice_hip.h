@@ -152,14 +152,17 @@ typedef struct hs_state {
 } hs_state_t;
 
 /* Metropolis selection geometry for one r (Magnets._select_grid, core.py:496-558, or the
- * Poisson-disc scheme of Magnets._select_Poisson, core.py:560-570). */
+ * Poisson-disc scheme of Magnets._select_Poisson, core.py:560-570), or the caller's own samples
+ * (scheme 3: update(idx=...), core.py:660-661): pat_cells holds nsc flat cell indices y*nx+x in
+ * the caller's order, every one a magnet (a step whose list holds anything else ends with
+ * err = 4 before it changes anything). */
 typedef struct hs_select {
   const int32_t *pat_off;   /* [uy*ux+1] offsets into pat_cells for supercell pattern (dy,dx) */
   const int32_t *pat_cells; /* occupied (oy*Rx+ox) cells of occupation[dy:dy+Ry, dx:dx+Rx], row-major */
   int32_t Rx, Ry;           /* supercell size in cells */
   int32_t snx, sny;         /* supercells along x/y: n//R + 1 */
   int32_t ncols, nsc;       /* chosen (every other) supercells per row and in total; Poisson: background cells */
-  int32_t scheme;           /* 0 = 'grid', 1 = 'single', 2 = 'Poisson' */
+  int32_t scheme;           /* 0 = 'grid', 1 = 'single', 2 = 'Poisson', 3 = explicit sample list */
   int32_t cutoff;           /* SIMULTANEOUS_SWITCHES_CONVOLUTION_OR_SUM_CUTOFF */
   int32_t reduced_k;        /* REDUCED_KERNEL_SIZE (0 = full kernel) */
   int32_t metropolis_time;  /* SimParams.METROPOLIS_TIME */
