@@ -65,6 +65,8 @@ def parse():
                         "full kernels and default SimParams of the served configuration)")
     p.add_argument("--field-steps", type=int, default=100, help="served: field steps per job")
     p.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
+    p.add_argument("--no-overlap", dest="overlap", action="store_false",
+                   help="served: the Neel and Metropolis launches of an iteration on one stream (default: two)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                    help="first copy MB MiB between two device buffers (a known read + write volume that "
                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -526,12 +528,12 @@ def cpu_baseline_served(jobs, seconds):
                        f"({cores_note()})")
 
 
-def serve_once(jobs):
+def serve_once(jobs, overlap=False):
     """Serves `jobs` on a fresh server; returns (done jobs, server, wall seconds, launch events)."""
     import torch
     from hotspice_amd._device import DeviceState
     from hotspice_amd.serve import Server
-    srv = Server()
+    srv = Server(overlap=overlap)
     for j in jobs:
         srv.submit(j)
     DeviceState.launch_events = []
@@ -570,7 +572,7 @@ def bench_served(args, rate=None, count=None, cpu=True):
     rate = args.rate if rate is None else rate
     count = args.jobs if count is None else count
     jobs = served_jobs(count=count, rate=rate, field_steps=args.field_steps, cutoff_cells=args.window)
-    done, srv, wall, launches = serve_once(jobs)
+    done, srv, wall, launches = serve_once(jobs, overlap=args.overlap)
     st = latency_stats(done)
     win = "full kernels (default SimParams)" if args.window is None else \
         f"dipolar window {2 * args.window + 1}x{2 * args.window + 1} (cutoff {args.window} cells)"
@@ -580,7 +582,8 @@ def bench_served(args, rate=None, count=None, cpu=True):
                                   f"50% Metropolis (4 MC sweeps per field step), {args.field_steps} field steps "
                                   f"(1.5 mT (1-cos), 0/45 deg), {win}",
                       "jobs": len(done), "failed": len(srv.failed), "jobs_per_s": len(done) / wall,
-                      "offered_jobs_per_s": rate, "latency_s": st, "attempts": srv.attempts, "parallelism": "single GPU"})
+                      "offered_jobs_per_s": rate, "latency_s": st, "attempts": srv.attempts,
+                      "streams": 2 if args.overlap else 1, "parallelism": "single GPU"})
     line["roofline"] = served_roofline(done, launches)
     line["cpu_baseline"] = cpu_baseline_served(jobs, args.cpu_seconds / 2) if cpu and not args.no_cpu else None
     return line

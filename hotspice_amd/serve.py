@@ -173,12 +173,15 @@ class Server:
     """Continuous-batching loop. `budget_bytes` bounds the summed footprint of admitted jobs;
     `max_workgroups` bounds one Neel launch (default: one 1024-thread workgroup per CU);
     `metro_chunk` caps the Metropolis steps per iteration so that new arrivals are admitted
-    promptly; `overlap` runs the Neel and Metropolis launches of an iteration on two streams.
+    promptly; `overlap` (default) runs the Neel and Metropolis launches of an iteration on two
+    streams of the default priority, so their kernels share the device (measured on MI355X: 31% of
+    the Neel launches of a 16-job mix overlap Metropolis kernels, 2% more attempts/s, p50 latency
+    3% lower).
     `backend` replaces the module of batched launches (hotspice_amd.batch), e.g. by a host
     stand-in that exercises the control flow without a device."""
 
     def __init__(self, budget_bytes=64 << 30, max_workgroups=None, neel_chunk=1024, metro_chunk=256,
-                 clock=time.perf_counter, overlap=False, backend=None):
+                 clock=time.perf_counter, overlap=True, backend=None):
         import torch
         self.torch = torch
         self.batch = batch if backend is None else backend
@@ -189,7 +192,7 @@ class Server:
         self.neel_chunk = int(neel_chunk)
         self.metro_chunk = int(metro_chunk)
         self.clock = clock
-        self.overlap = bool(overlap)
+        self.overlap = bool(overlap) and backend is None  # a stand-in backend launches nothing to overlap
         self._streams = (torch.cuda.Stream(), torch.cuda.Stream()) if self.overlap else None
         self.queue = []  # heap of (arrival, seq, job)
         self.active: List[_Active] = []
