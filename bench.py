@@ -1,6 +1,6 @@
 """Benchmark of the Hotspice stepping path on one MI355X.
 
-Default workload (`--workload neel`, the plain Neel configuration): in-plane pinwheel ASI
+Headline workload (`--workload neel`, the plain Neel configuration): in-plane pinwheel ASI
 256x256 (32768 magnets), full 511x511 dipolar kernels + 1 mT Zeeman field, Neel-Arrhenius,
 FP32 + Kahan fields with an FP64 recompute every 1024 switches. One benchmark step = 1024 Neel
 switch attempts + the FP64 recompute, i.e. one recompute period; `value` = switch attempts/s.
@@ -43,10 +43,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PERIOD = 1024
-# HBM bytes per launch of k_neel_team on the 256x256 pinwheel workload (1024 attempts), from two
-# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy of known size; the
-# commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
+# HBM bytes per launch of k_neel_team on the 256x256 pinwheel workload (1024 attempts), a profile
+# figure: two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy of known
+# size; the commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
 NEEL_TEAM_TRAFFIC_PER_LAUNCH = 96.3e6
+NEEL_TEAM_TRAFFIC_SOURCE = "profile figure (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, profiles/neel_team_pinwheel256_pmc.md)"
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("all", "neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
@@ -173,9 +174,25 @@ def bench_neel(args):
                                 {"attempts_per_launch": attempts / len(launches), "bytes_per_attempt": bytes_per})
     if abs(attempts / len(launches) - PERIOD) < 1:
         line["roofline"]["traffic"] = NEEL_TEAM_TRAFFIC_PER_LAUNCH
+        line["roofline"]["traffic_source"] = NEEL_TEAM_TRAFFIC_SOURCE
+    line["latency_roofline"] = neel_latency_roofline(mm, line["roofline"]["avg_launch_us"] / (attempts / len(launches)))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_neel(args.cpu_seconds)
     return line
+
+
+def neel_latency_roofline(mm, us_per_attempt):
+    """The step's latency floor, measured live: one exchange round of the instance's team (G
+    workgroups publish a candidate and poll each other's through device-scope atomics, nothing
+    else), against the measured time per attempt of k_neel_team."""
+    import torch
+    from hotspice_amd import _lib, batch
+    G = batch.neel_team_size(mm)
+    torch.cuda.synchronize()
+    floor, failed = _lib.exchange_round_us(G, rounds=20000)
+    return {"bound": "latency", "floor_us": floor, "achieved_us": us_per_attempt, "frac": floor / us_per_attempt,
+            "unit": "us per attempt", "team_workgroups": G, "failed_rounds": failed,
+            "floor_what": f"one cross-workgroup exchange round of {G} workgroups (hs_exchange_round_us, 20000 rounds)"}
 
 
 # ---- plain Metropolis -------------------------------------------------------------------------

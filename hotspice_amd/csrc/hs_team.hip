@@ -744,3 +744,91 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
   if (e != hipSuccess) return hs_set_error(HS_ERR_HIP, "cooperative launch of k_neel_team failed: %s", hipGetErrorString(e));
   return HS_OK;
 }
+
+// ---- the exchange round alone ----------------------------------------------------------------
+namespace {
+
+// G workgroups of one wave exchange a tagged candidate per round through the same records,
+// stores and bounded polls as k_neel_team's step (device-scope relaxed atomics, one REC_BYTES
+// record per workgroup, three slots), and reduce them with the same 64-bit minimum; nothing else.
+// A round whose records do not all arrive within SPIN_LIMIT polls counts in *fails (the kernel
+// always ends).
+__global__ void __launch_bounds__(64) k_exchange_probe(TeamRec *rec, int G, int rounds, int *fails,
+                                                       unsigned long long *sink) {
+  const int rank = blockIdx.x, lane = threadIdx.x;
+  unsigned long long acc = 0ull;
+  int bad_rounds = 0;
+  for (int round = 0; round < rounds; ++round) {
+    const unsigned long long tg = (unsigned long long)(round & 63);
+    TeamRec *slot = rec + (size_t)(round % 3) * G;
+    const unsigned long long pc = ((unsigned long long)(rank * 2654435761u) << 32) | ((unsigned long long)rank << 2);
+    if (lane == 0) __hip_atomic_store(&slot[rank].c, (pc & ~TAG_MASK) | (tg << 26), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long c2 = ~0ull;
+    bool bad = false;
+    for (int r = lane; r < G; r += 64) {
+      uint64_t spins = 0;
+      unsigned long long cr;
+      for (;;) {
+        cr = __hip_atomic_load(&slot[r].c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (((cr & TAG_MASK) >> 26) == tg) break;
+        if (++spins > SPIN_LIMIT) {
+          bad = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      c2 = cr < c2 ? cr : c2;
+    }
+    acc ^= hs_wave_min_u64(c2);
+    if (__ballot(bad) != 0ull) ++bad_rounds;
+  }
+  if (lane == 0) {
+    sink[rank] = acc;
+    if (bad_rounds) atomicAdd(fails, bad_rounds);
+  }
+}
+
+}  // namespace
+
+extern "C" int hs_exchange_round_us(int32_t G, int32_t rounds, double *us_per_round, int32_t *failed_rounds) {
+  hs_clear_error();
+  if (!us_per_round || G < 1 || rounds < 1) return hs_set_error(HS_ERR_ARG, "need G >= 1, rounds >= 1 and an output");
+  int dev = 0, cus = 0;
+  HS_HIP_TRY(hipGetDevice(&dev));
+  HS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (G > cus) return hs_set_error(HS_ERR_ARG, "G = %d exceeds the %d co-resident workgroups of one per CU", G, cus);
+  const size_t rec_bytes = sizeof(TeamRec) * 3 * (size_t)G;
+  char *buf = nullptr;
+  HS_HIP_TRY(hipMalloc(&buf, rec_bytes + 256 + 8 * (size_t)G));
+  TeamRec *rec = (TeamRec *)buf;
+  int *fails = (int *)(buf + rec_bytes);
+  unsigned long long *sink = (unsigned long long *)(buf + rec_bytes + 256);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HS_OK;
+  float ms = 0.f;
+  int hf = 0;
+  auto fail = [&](hipError_t e, const char *what) {
+    if (rc == HS_OK && e != hipSuccess) rc = hs_set_error(HS_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+  };
+  fail(hipMemset(buf, 0xff, rec_bytes), "hipMemset");
+  fail(hipMemset(fails, 0, 4), "hipMemset");
+  fail(hipEventCreate(&e0), "hipEventCreate");
+  fail(hipEventCreate(&e1), "hipEventCreate");
+  if (rc == HS_OK) {
+    void *args[] = {(void *)&rec, (void *)&G, (void *)&rounds, (void *)&fails, (void *)&sink};
+    fail(hipEventRecord(e0, nullptr), "hipEventRecord");
+    fail(hipLaunchCooperativeKernel((const void *)&k_exchange_probe, dim3(G), dim3(64), args, 0, nullptr),
+         "cooperative launch of k_exchange_probe");
+    fail(hipEventRecord(e1, nullptr), "hipEventRecord");
+    fail(hipEventSynchronize(e1), "hipEventSynchronize");
+    fail(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    fail(hipMemcpy(&hf, fails, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(buf);
+  if (rc != HS_OK) return rc;
+  *us_per_round = (double)ms * 1000.0 / rounds;
+  if (failed_rounds) *failed_rounds = hf;
+  return HS_OK;
+}

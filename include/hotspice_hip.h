@@ -435,6 +435,16 @@ int64_t hs_neel_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *ws, int64_t ws_bytes, void *stream);
 int64_t hs_metro_teams_ws_bytes(const hs_team_t *teams, int32_t nteams);
 
+/*
+ * Latency floor of the persistent Neel step: G co-resident workgroups (one wave each) exchange a
+ * tagged candidate per round through the records, stores and bounded polls of k_neel_team and
+ * reduce them, `rounds` times in one cooperative launch on the null stream; *us_per_round is the
+ * launch's duration (HIP events) over rounds, *failed_rounds (optional) the rounds whose records
+ * did not all arrive within the kernel's bounded spin. Allocates and frees its own records and
+ * synchronises the device: a measurement, not a stepping call. No counterpart in the reference.
+ */
+int hs_exchange_round_us(int32_t G, int32_t rounds, double *us_per_round, int32_t *failed_rounds);
+
 /* Steps per captured hipGraph in the step loop of hs_metro_run_teams by default: plain launches
  * (the loop is bound by the GPU, not by the host's launch rate, and replaying graphs measured no
  * faster on MI355X). */

@@ -398,3 +398,13 @@ def test_metropolis_update_runs_the_periodic_recompute():
         np.testing.assert_array_equal(mm.update(r=4e-6), om.update(r=4e-6))
     assert mm.recomputes - r0 == 2
     np.testing.assert_allclose(mm.E, om.E, rtol=0, atol=1e-9 * np.abs(om.E).max())
+
+
+@pytest.mark.parametrize("G", [1, 8, 64])
+def test_exchange_round_probe(G):
+    """hs_exchange_round_us: every round's records arrive, and a round costs less than a Neel
+    step of a team of that size (it is that step's exchange alone)."""
+    from hotspice_amd import _lib
+    us, failed = _lib.exchange_round_us(G, rounds=2000)
+    assert failed == 0
+    assert 0.0 < us < 5.0
