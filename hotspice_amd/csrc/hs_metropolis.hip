@@ -73,7 +73,11 @@ struct MetroWS {
   float *acc_mmu;    // [nsc] (m mu) after the flip
   int32_t *acc_n;    // [1]  number accepted this step
   int32_t *trunc;    // [HS_MAX_UC] per unit-cell site: 1 = truncated window (convolution branch)
-  int32_t *info;     // [4] grid offset ox, oy of the selection that succeeded, 1 = single sample
+  int32_t *info;     // [5] grid offset ox, oy of the selection that succeeded, 1 = single sample;
+                     // [3] the step a presampling launch finished (FUSED_MARK), [4] 1 = the update
+                     // takes its sources from the sample array only (no ordered list was written)
+  int32_t *fz;       // [8 + HS_MAX_UC] presampling launch: ticket, in-grid / valid / accepted
+                     // samples, then accepted samples per unit-cell site (zero between steps)
   float *pax, *pay;  // [nsc] Poisson: accepted point of each background cell (NaN = none)
   float *pcx, *pcy;  // [nsc] Poisson: this round's candidate point
   uint32_t *pcp;     // [nsc] Poisson: its priority (0xffffffff = no candidate)
@@ -121,6 +125,8 @@ MetroWS carve(void *base, int nsc) {
   p += align256(HS_MAX_UC * 4);
   w.info = (int32_t *)p;
   p += 256;
+  w.fz = (int32_t *)p;
+  p += 512;
   w.pax = (float *)p;
   p += align256(n * 4);
   w.pay = (float *)p;
@@ -143,7 +149,7 @@ MetroWS carve(void *base, int nsc) {
 // Per-job scratch (the MetroWS arrays).
 size_t ws_bytes(int nsc) {
   const size_t n = (size_t)(nsc > 0 ? nsc : 1);
-  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 +
+  return align256(n * 4) + align256(n) + align256(n * 4) + align256(n * 4) + 256 + align256(HS_MAX_UC * 4) + 256 + 512 +
          7 * align256(n * 4) + align256(8 * ((n + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS));
 }
 
@@ -389,6 +395,10 @@ HS_INLINE bool metro_accepts(const hs_instance_t &I, uint64_t step, int c, float
   return ua < p;
 }
 
+// The value info[3] holds once a presampling launch has completed the step that leaves the state
+// at `step` (bit 31 set, so zeroed scratch never matches).
+HS_INLINE int fused_mark(int64_t step) { return (int)(0x80000000u | ((uint32_t)step & 0x7fffffffu)); }
+
 template <int QB, bool PRE = false>
 __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
                                              int job = 0) {
@@ -409,6 +419,11 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   const int tid = threadIdx.x;
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
+  if (PRE && ws.info[3] == fused_mark(S.step)) {  // the presampling launch completed this step
+    __syncthreads();
+    if (tid == 0) ws.info[3] = 0;
+    return *ws.acc_n;
+  }
   if (tid < HS_MAX_UC) s_site[tid] = 0;
   bool go = !S.done && S.steps_done < R.max_steps &&
             (hs_step_free(R.first_free, S.steps_done) ||
@@ -730,6 +745,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
     ws.info[0] = ox;
     ws.info[1] = oy;
     ws.info[2] = single ? 1 : 0;
+    ws.info[4] = 0;
     *ws.acc_n = total;
     if (R.log) log_count[S.steps_done] = total;
     S.attempts += nvalid;
@@ -793,11 +809,12 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   // everything that does not depend on the step's sources is loaded up front, together: the
   // step's counts, the per-site tables and the cell's own state
   const int a = *ws.acc_n;
-  int info_ox = 0, info_oy = 0, info_single = 0;
+  int info_ox = 0, info_oy = 0, info_single = 0, info_samples = 0;
   if (PRELOAD) {
     info_ox = ws.info[0];
     info_oy = ws.info[1];
     info_single = ws.info[2];
+    info_samples = ws.info[4];
   }
   if (PRELOAD && tid < I.ux * I.uy) {
     s_sub[tid] = I.sub_of[tid];
@@ -891,7 +908,10 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   // within the largest window of the tile can reach it.
   bool lookup = false;
   int rs0 = 0, rs1 = -1, cs0 = 0, cs1 = -1;
-  if (!PRELOAD) info_single = ws.info[2];
+  if (!PRELOAD) {
+    info_single = ws.info[2];
+    info_samples = ws.info[4];
+  }
   if (SEL.scheme == 0 && !I.pbc && info_single == 0) {
     const int ox = PRELOAD ? info_ox : ws.info[0], oy = PRELOAD ? info_oy : ws.info[1];
     const int Wx = max(has_dip ? I.kwx : 0, has_ex ? hx : 0), Wy = max(has_dip ? I.kwy : 0, has_ex ? hy : 0);
@@ -900,7 +920,8 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
     cs1 = min(SEL.ncols - 1, floor_div(x0 + MU_TX - 1 + Wx - ox, 2 * SEL.Rx));
     rs0 = max(0, -floor_div(-(y0 - Wy - oy - SEL.Ry + 1), 2 * SEL.Ry));
     rs1 = min(nrows - 1, floor_div(y0 + MU_TY - 1 + Wy - oy, 2 * SEL.Ry));
-    lookup = (int64_t)max(0, cs1 - cs0 + 1) * max(0, rs1 - rs0 + 1) <= (int64_t)a;
+    // the sample array only, when the step wrote no ordered list (same sources, same order)
+    lookup = info_samples != 0 || (int64_t)max(0, cs1 - cs0 + 1) * max(0, rs1 - rs0 + 1) <= (int64_t)a;
   }
   // stage the sources that reach the tile in LDS (in selection order), chunk by chunk, and let
   // every cell of the tile add their contributions
@@ -1024,8 +1045,14 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   const hs_run_t &R = J.R;
   const MetroWS &ws = J.ws;
   __shared__ long long s_l[PRESAMPLE_THREADS / 64];
+  __shared__ int s_site[HS_MAX_UC];
+  __shared__ bool s_last;
   const hs_state_t S = J.st[0];
   const double n = (double)I.n_occ;
+  // the step is completed here rather than by k_metro_select_pre: without PBC (that selection
+  // lists the tiles itself) and without a step log (which wants the ordered list)
+  const bool fuse = !I.pbc && !R.log;
+  if (fuse && threadIdx.x < HS_MAX_UC) s_site[threadIdx.x] = 0;
   const bool go = !S.done && S.steps_done < R.max_steps &&
                   (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
@@ -1037,7 +1064,8 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
   const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
   const int j = blk * PRESAMPLE_THREADS + threadIdx.x;
-  int inb = 0, val = 0;
+  int inb = 0, val = 0, nacc = 0;
+  if (fuse) __syncthreads();  // s_site cleared
   if (j < SEL.nsc) {
     const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u, (uint64_t)j), 0) * cnt);
     bool ok;
@@ -1058,9 +1086,54 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     ws.cand[j] = c;
     ws.accf[j] = acc ? 1 : 0;
     ws.cmmu[j] = mmu;
+    nacc = acc ? 1 : 0;
+    if (acc && fuse) atomicAdd(&s_site[hs_site(I, c / I.nx, c % I.nx)], 1);
   }
-  const long long both = block_sum_i64(((long long)inb << 32) + val, s_l);
-  if (threadIdx.x == 0) ws.part[blk] = both;
+  // in-grid, valid and accepted samples of the workgroup, 21 bits each (at most 256)
+  const long long sums = block_sum_i64(((long long)inb << 42) + ((long long)val << 21) + nacc, s_l);
+  const int b_inb = (int)(sums >> 42), b_val = (int)((sums >> 21) & 0x1fffff), b_acc = (int)(sums & 0x1fffff);
+  if (threadIdx.x == 0) ws.part[blk] = ((long long)b_inb << 32) + b_val;
+  if (!fuse) return;
+  // Fused listing: the workgroups add their counts to the job's counters; the last one to finish
+  // (ticket) completes the step as the selection workgroup would (truncation flags, offsets,
+  // counters and state), and k_metro_select_pre then finds the step done. The update takes its
+  // sources from the sample array (supercell order is selection order), so no ordered list is
+  // needed. A step whose presampled attempt has no valid sample is left to k_metro_select_pre.
+  const int tid = threadIdx.x, nsites = I.ux * I.uy;
+  if (tid < nsites && s_site[tid]) atomicAdd(&ws.fz[8 + tid], s_site[tid]);
+  if (tid == 0) {
+    atomicAdd(&ws.fz[1], b_inb);
+    atomicAdd(&ws.fz[2], b_val);
+    atomicAdd(&ws.fz[3], b_acc);
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) s_last = atomicAdd(&ws.fz[0], 1) == (SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int tot_inb = __hip_atomic_load(&ws.fz[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int tot_val = __hip_atomic_load(&ws.fz[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int tot_acc = __hip_atomic_load(&ws.fz[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int site_acc = tid < nsites ? __hip_atomic_load(&ws.fz[8 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  __syncthreads();
+  if (tid < 8 + HS_MAX_UC) ws.fz[tid] = 0;  // zero for the next step
+  if (tot_inb <= 0 || tot_val <= 0) return;
+  if (tid < nsites) ws.trunc[tid] = (tot_acc > 1 && SEL.reduced_k != 0 && site_acc > SEL.cutoff) ? 1 : 0;
+  if (tid == 0) {
+    ws.info[0] = ox;
+    ws.info[1] = oy;
+    ws.info[2] = 0;
+    ws.info[4] = 1;
+    *ws.acc_n = tot_acc;
+    hs_state_t o = S;
+    o.attempts += tot_val;
+    o.switches += tot_acc;
+    o.step += 1;
+    o.steps_done += 1;
+    J.st[0] = o;
+    ws.info[3] = fused_mark(o.step);
+  }
 }
 
 __global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J) { metro_presample(J, blockIdx.x); }
@@ -1206,9 +1279,28 @@ __global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_many(
   if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
 }
 
-// Clears both worklist counters (a count left by an earlier call is discarded).
-__global__ void k_metro_clear(WorkList WL) {
+// Clears both worklist counters (a count left by an earlier call is discarded) and the fused
+// presampling counters and marks of the call's job(s): the workspace may have held other data.
+__global__ void k_metro_clear(WorkList WL, MetroWS ws) {
   if (threadIdx.x < 2) WL.count[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < 8 + HS_MAX_UC; i += blockDim.x) ws.fz[i] = 0;
+  if (threadIdx.x == 0) {
+    ws.info[3] = 0;
+    ws.info[4] = 0;
+  }
+}
+
+__global__ void k_metro_clear_many(WorkList WL, const MetroJob *__restrict__ jobs, const int32_t *__restrict__ which,
+                                   int n) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) WL.count[threadIdx.x] = 0;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const MetroWS &ws = jobs[which[k]].ws;
+    for (int i = threadIdx.x; i < 8 + HS_MAX_UC; i += blockDim.x) ws.fz[i] = 0;
+    if (threadIdx.x == 0) {
+      ws.info[3] = 0;
+      ws.info[4] = 0;
+    }
+  }
 }
 
 // Persistent Metropolis stepping of small instances: one 1024-thread workgroup per job runs the
@@ -1737,7 +1829,7 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   const int nt = J.tiles_x * J.tiles_y;
   char *wlp = (char *)ws + ws_bytes(J.SEL.nsc);
   const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
-  k_metro_clear<<<1, 64, 0, s>>>(WL);
+  k_metro_clear<<<1, 64, 0, s>>>(WL, J.ws);
   HS_LAUNCH_CHECK("k_metro_clear");
   const unsigned wgs = update_grid(nt, run->nblocks);
   // grid selection without METROPOLIS_TIME and with more samples than one batch per selection
@@ -1950,12 +2042,12 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   if (steps > 0) {
     char *wlp = base + L.wl_off;
     const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)L.ntiles))};
-    k_metro_clear<<<1, 64, 0, s>>>(WL);
-    HS_LAUNCH_CHECK("k_metro_clear");
+    const int32_t *lj = (const int32_t *)(base + L.large_off);
+    k_metro_clear_many<<<(unsigned)std::min<size_t>(L.large.size(), 1024), 64, 0, s>>>(WL, dj, lj, (int)L.large.size());
+    HS_LAUNCH_CHECK("k_metro_clear_many");
     int64_t req = 0;  // the largest update grid a team asks for (0: automatic)
     for (int i : L.large) req = std::max<int64_t>(req, teams[i].run.nblocks);
     const unsigned wgs = update_grid(L.ntiles, req);
-    const int32_t *lj = (const int32_t *)(base + L.large_off);
     // the launches of steps j0 .. j0+count-1 on stream `st`
     auto record = [&](hipStream_t st, int64_t j0, int64_t count) -> int {
       for (int64_t j = j0; j < j0 + count; ++j) {
