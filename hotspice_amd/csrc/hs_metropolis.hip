@@ -395,11 +395,14 @@ HS_INLINE bool metro_accepts(const hs_instance_t &I, uint64_t step, int c, float
   return ua < p;
 }
 
-// The value info[3] holds once a presampling launch has completed the step that leaves the state
-// at `step` (bit 31 set, so zeroed scratch never matches).
-HS_INLINE int fused_mark(int64_t step) { return (int)(0x80000000u | ((uint32_t)step & 0x7fffffffu)); }
+// info[3] == FUSED_DONE: the presampling launch of this step completed it (that launch clears the
+// mark first thing, so a mark never outlives its step).
+constexpr int FUSED_DONE = 1;
 
-template <int QB, bool PRE = false>
+// LIST_TILES (with PRE): the selection lists the reached tiles itself, for the fallback run by
+// k_metro_tiles when the presampled attempt did not complete the step. NT: the workgroup size
+// (the selection kernels use MS_THREADS; the tile kernel's fallback runs with its own 256).
+template <int QB, bool PRE = false, bool LIST_TILES = false, int NT = MS_THREADS>
 __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = WorkList{nullptr, nullptr}, int par = 0,
                                              int job = 0) {
   const hs_instance_t &I = J.I;
@@ -408,9 +411,9 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   hs_state_t *st = J.st;
   const MetroWS &ws = J.ws;
   int32_t *log_cells = J.log_cells, *log_count = J.log_count;
-  __shared__ int s_i[MS_THREADS / 64];
-  __shared__ long long s_l[MS_THREADS / 64];
-  __shared__ double s_d[MS_THREADS / 64];
+  __shared__ int s_i[NT / 64];
+  __shared__ long long s_l[NT / 64];
+  __shared__ double s_d[NT / 64];
   __shared__ int s_site[HS_MAX_UC];
   __shared__ int s_single;
   __shared__ float s_single_mmu;
@@ -419,11 +422,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   const int tid = threadIdx.x;
   hs_state_t S = st[0];
   const double n = (double)I.n_occ;
-  if (PRE && ws.info[3] == fused_mark(S.step)) {  // the presampling launch completed this step
-    __syncthreads();
-    if (tid == 0) ws.info[3] = 0;
-    return *ws.acc_n;
-  }
+  if (PRE && ws.info[3] == FUSED_DONE) return *ws.acc_n;  // the presampling launch completed this step
   if (tid < HS_MAX_UC) s_site[tid] = 0;
   bool go = !S.done && S.steps_done < R.max_steps &&
             (hs_step_free(R.first_free, S.steps_done) ||
@@ -441,7 +440,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   }
   const uint64_t step = (uint64_t)S.step;
   const int nsc = SEL.nsc;
-  const int q = (nsc + MS_THREADS - 1) / MS_THREADS;
+  const int q = (nsc + NT - 1) / NT;
   const int j0 = min(nsc, tid * q), j1 = min(nsc, j0 + q);
   const int nx = I.nx, ny = I.ny;
   // all of this thread's grid samples fit one batch: they stay in registers (the Poisson scheme
@@ -464,7 +463,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
   bool pre = false;
   if (PRE) {
     long long mine = 0;
-    for (int b = tid; b < (nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; b += MS_THREADS) mine += ws.part[b];
+    for (int b = tid; b < (nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; b += NT) mine += ws.part[b];
     if (regs) {  // this thread's presampled candidates, loaded together with the counts
 #pragma unroll
       for (int u = 0; u < QB; ++u) {
@@ -684,7 +683,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
     ws.trunc[tid] = (total > 1 && SEL.reduced_k != 0 && s_site[tid] > SEL.cutoff) ? 1 : 0;
   }
   // the presampled grid scheme without PBC leaves the tiles to k_metro_tiles (many workgroups)
-  const bool defer_tiles = PRE && !single && !I.pbc;
+  const bool defer_tiles = PRE && !LIST_TILES && !single && !I.pbc;
   if (WL.wl && total > 0 && !defer_tiles) {
     // the tiles this step changes: the windows metro_update adds (dipolar: stored or truncated
     // window, wrapped when truncated with PBC; exchange: the NN stencil, wrapped with PBC) and the
@@ -694,7 +693,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
     const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
     const int nword = (ntile + 31) / 32;
     if (tid == 0) s_all = nword > MARK_WORDS ? 1 : 0;
-    for (int i = tid; i < min(nword, MARK_WORDS); i += MS_THREADS) s_bits[i] = 0u;
+    for (int i = tid; i < min(nword, MARK_WORDS); i += NT) s_bits[i] = 0u;
     __syncthreads();
     if (s_all == 0) {
       for (int jb = a0; jb < a1; jb += QB) {
@@ -726,13 +725,13 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
       return valid >= 32 ? v : v & ((1u << valid) - 1u);
     };
     int mine = 0;
-    for (int w = tid; w < nword; w += MS_THREADS) mine += __popc(word_bits(w));
+    for (int w = tid; w < nword; w += NT) mine += __popc(word_bits(w));
     int nlist = 0;
     int off = block_exclusive_scan(mine, s_i, nlist);
     if (tid == 0) s_base = atomicAdd(&WL.count[par], nlist);
     __syncthreads();
     off += s_base;
-    for (int w = tid; w < nword; w += MS_THREADS) {
+    for (int w = tid; w < nword; w += NT) {
       uint32_t bits = word_bits(w);
       while (bits) {
         const int b = __ffs((int)bits) - 1;
@@ -1053,6 +1052,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   // lists the tiles itself) and without a step log (which wants the ordered list)
   const bool fuse = !I.pbc && !R.log;
   if (fuse && threadIdx.x < HS_MAX_UC) s_site[threadIdx.x] = 0;
+  // the previous step's mark goes first (before this workgroup's ticket, so before the last
+  // workgroup sets it again)
+  if (fuse && blk == 0 && threadIdx.x == 0) ws.info[3] = 0;
   const bool go = !S.done && S.steps_done < R.max_steps &&
                   (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
@@ -1132,7 +1134,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     o.step += 1;
     o.steps_done += 1;
     J.st[0] = o;
-    ws.info[3] = fused_mark(o.step);
+    ws.info[3] = FUSED_DONE;
   }
 }
 
@@ -1163,13 +1165,22 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_pre_many(const Metr
 // with the update's reach rules (dipolar window, truncated per site; exchange stencil; the tile of
 // the switch itself). Reached tiles are appended to the worklist, one atomic per wave (the order
 // of the entries does not matter: every tile's update is independent of the others).
-__device__ __forceinline__ void metro_tiles(const MetroJob &J, WorkList WL, int par, int job, int t) {
+// Returns false (doing nothing) when `fallback` is set and the presampling launch did not
+// complete the step.
+__device__ __forceinline__ bool metro_tiles(const MetroJob &J, WorkList WL, int par, int job, int t,
+                                            bool fallback = false) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const MetroWS &ws = J.ws;
-  const int a = *ws.acc_n;
-  if (a == 0 || ws.info[2] != 0 || I.pbc) return;  // nothing switched, the single-sample step or PBC (listed already)
+  // the per-site truncation flags go to LDS, their loads issued with the step's counts, offsets
+  // and completion mark
+  __shared__ int s_tr[HS_MAX_UC];
+  if ((int)threadIdx.x < I.ux * I.uy) s_tr[threadIdx.x] = ws.trunc[threadIdx.x];
+  const int a = *ws.acc_n, single = ws.info[2], mark = ws.info[3];
   const int ox = ws.info[0], oy = ws.info[1];
+  __syncthreads();
+  if (fallback && mark != FUSED_DONE) return false;
+  if (a == 0 || single != 0 || I.pbc) return true;  // nothing switched, the single-sample step or PBC (listed already)
   const int nx = I.nx, ny = I.ny, tiles_x = J.tiles_x, ntile = J.tiles_x * J.tiles_y;
   bool touched = false;
   if (t < ntile) {
@@ -1219,7 +1230,7 @@ __device__ __forceinline__ void metro_tiles(const MetroJob &J, WorkList WL, int 
       const int sy = sc / nx, sx = sc % nx, ss = hs_site(I, sy, sx);
       bool rel = sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1;  // the switch's own tile (its flip)
       if (!rel && has_dip && I.sub_of[ss] >= 0) {
-        const bool tr = ws.trunc[ss] != 0;
+        const bool tr = s_tr[ss] != 0;
         rel = reaches(sx, tr ? tkx : I.kwx, nx, false, x0, x0 + MU_TX - 1) &&
               reaches(sy, tr ? tky : I.kwy, ny, false, y0, y0 + MU_TY - 1);
       }
@@ -1234,10 +1245,15 @@ __device__ __forceinline__ void metro_tiles(const MetroJob &J, WorkList WL, int 
   if (lane == 0 && ball) base = atomicAdd(&WL.count[par], __popcll(ball));
   base = __shfl(base, 0, 64);
   if (touched) WL.wl[base + __popcll(ball & ((1ull << lane) - 1ull))] = make_int2(job, t);
+  return true;
 }
 
-__global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, int par) {
-  metro_tiles(J, WL, par, 0, blockIdx.x * blockDim.x + threadIdx.x);
+// One thread per tile. With `fallback` (no k_metro_select_pre launch in the step): when the
+// presampling launch did not complete the step (its attempt had no valid sample, or the stop
+// predicate ended the run), workgroup 0 runs the selection instead, listing the tiles itself.
+__global__ void __launch_bounds__(256) k_metro_tiles(MetroJob J, WorkList WL, int par, int fallback) {
+  if (!metro_tiles(J, WL, par, 0, blockIdx.x * blockDim.x + threadIdx.x, fallback != 0) && blockIdx.x == 0)
+    metro_select<PRESELECT_BATCH, true, true, 256>(J, WL, par, 0);
 }
 
 // A batch: block b covers tiles 256*map[b].y .. +255 of job map[b].x.
@@ -1841,10 +1857,15 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
     if (presample) {
       k_metro_presample<<<pblocks, PRESAMPLE_THREADS, 0, s>>>(J);
       HS_LAUNCH_CHECK("k_metro_presample");
-      k_metro_select_pre<<<1, MS_THREADS, 0, s>>>(J, WL, par);
-      HS_LAUNCH_CHECK("k_metro_select_pre");
+      // the presampling launch completes the step itself without PBC or log; the tile launch then
+      // runs the selection only when it did not (the rare attempt without a valid sample)
+      const bool fused = !inst->pbc && !run->log;
+      if (!fused) {
+        k_metro_select_pre<<<1, MS_THREADS, 0, s>>>(J, WL, par);
+        HS_LAUNCH_CHECK("k_metro_select_pre");
+      }
       if (!inst->pbc) {
-        k_metro_tiles<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(J, WL, par);
+        k_metro_tiles<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(J, WL, par, fused ? 1 : 0);
         HS_LAUNCH_CHECK("k_metro_tiles");
       }
     } else {
