@@ -46,8 +46,12 @@ PERIOD = 1024
 # HBM bytes per launch of k_neel_team on the 256x256 pinwheel workload (1024 attempts), a profile
 # figure: two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB copy of known
 # size; the commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
-NEEL_TEAM_TRAFFIC_PER_LAUNCH = 96.3e6
+NEEL_TEAM_TRAFFIC_PER_LAUNCH = 94.8e6
 NEEL_TEAM_TRAFFIC_SOURCE = "profile figure (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, profiles/neel_team_pinwheel256_pmc.md)"
+# HBM bytes per device step of the 2048x2048 Metropolis workload (presample + tiles + update), a
+# profile figure from the same two PMC passes: profiles/metro_pmc_oop2048_ipsq64.md
+METRO2048_TRAFFIC_PER_STEP = 30.6e6
+METRO2048_TRAFFIC_SOURCE = "profile figure per device step (rocprofv3 PMC, profiles/metro_pmc_oop2048_ipsq64.md)"
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("all", "neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
@@ -273,7 +277,8 @@ def bench_metro(args, large):
     else:
         mm, kw, sweeps, warm = oop_metropolis(), dict(r=8 * a), 200, 1
         desc = ("plain Metropolis: OOP_Square 64x64 (4096 magnets), a=200 nm, moment 1.1e-15, T=300 K, "
-                "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a, FP32+Kahan")
+                "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a; the small configuration's shape at "
+                "FP32+Kahan fields with FP64 acceptance, not its float64 stepping (error vs FP64 stated)")
         kwin = 10 ** 6
     steps = sweeps if args.steps is None else args.steps
     mm.progress(t_max=math.inf, MCsteps_max=warm, **kw)
@@ -304,9 +309,12 @@ def bench_metro(args, large):
     line = base_line(args, attempts / wall, steps, 0, wall, "f32+kahan (fields), f64 (acceptance)",
                      {"workload": desc, "attempts": attempts, "accepted": accepted, "device_steps": nsteps,
                       "rel_E_err_vs_fp64": err, "window_cells_per_accept": nt_per_acc, "parallelism": "single GPU"})
-    kname = ("k_metro_presample + k_metro_select_pre + k_metro_tiles + k_metro_update_one (per progress call)" if large
+    kname = ("k_metro_presample + k_metro_tiles + k_metro_update_one (per progress call)" if large
              else "k_metro_persist_one (per progress call)")
     line["roofline"] = roofline(kname, launches, bytes_total)
+    if large:
+        line["roofline"]["traffic"] = METRO2048_TRAFFIC_PER_STEP * nsteps / len(launches)
+        line["roofline"]["traffic_source"] = METRO2048_TRAFFIC_SOURCE
     if not args.no_cpu:
         if large:
             line["cpu_baseline"] = cpu_baseline_metro2048(args.cpu_seconds)
