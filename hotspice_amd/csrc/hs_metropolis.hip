@@ -1054,7 +1054,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   if (fuse && threadIdx.x < HS_MAX_UC) s_site[threadIdx.x] = 0;
   // the previous step's mark goes first (before this workgroup's ticket, so before the last
   // workgroup sets it again)
-  if (fuse && blk == 0 && threadIdx.x == 0) ws.info[3] = 0;
+  if (fuse && blk == 0 && threadIdx.x == 0) __hip_atomic_store(&ws.info[3], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const bool go = !S.done && S.steps_done < R.max_steps &&
                   (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
@@ -1108,12 +1108,17 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     atomicAdd(&ws.fz[2], b_val);
     atomicAdd(&ws.fz[3], b_acc);
   }
-  __threadfence();
+  // The counts are device-scope atomics and the mark's clear a device-scope store: each thread
+  // waits until they are performed (s_waitcnt, behind a workgroup fence for the compiler) before
+  // the ticket, and the last workgroup reads them with device-scope loads. No device-scope fence:
+  // on this GPU that writes back and invalidates the whole L2 of the XCD, per workgroup, and the
+  // following launches would find their data evicted.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0) s_last = atomicAdd(&ws.fz[0], 1) == (SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS - 1;
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
   const int tot_inb = __hip_atomic_load(&ws.fz[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int tot_val = __hip_atomic_load(&ws.fz[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int tot_acc = __hip_atomic_load(&ws.fz[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1134,7 +1139,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     o.step += 1;
     o.steps_done += 1;
     J.st[0] = o;
-    ws.info[3] = FUSED_DONE;
+    __hip_atomic_store(&ws.info[3], FUSED_DONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
