@@ -6,7 +6,8 @@ FP32 + Kahan fields with an FP64 recompute every 1024 switches. One benchmark st
 switch attempts + the FP64 recompute, i.e. one recompute period; `value` = switch attempts/s.
 
 Other workloads (one JSON line each, same keys):
-  metro64     plain Metropolis: OOP square 64x64, full 127x127 kernel, grid r = 8a; step = 1 MC sweep
+  metro64     plain Metropolis: OOP square 64x64, full 127x127 kernel, grid r = 8a, float64 fields;
+              step = 1 MC sweep
   metro2048   plain Metropolis: OOP square 2048x2048, 41x41 window + exchange, grid r = 20a;
               step = 1 MC sweep
   batched     64 IP_Square instances (n 32..480, own T and field) stepped together by Metropolis;
@@ -19,8 +20,9 @@ Other workloads (one JSON line each, same keys):
 
 The default run (`--workload all`) measures the plain Neel line above as `value` and adds, timed
 in the same run, a `secondary` object with the other configurations of the metric: metro2048
-(plain, large), batched (with `vs_plain`) and served, at the offered 200 jobs/s and at a rate
-below what the device sustains -- each with its own `roofline` and `cpu_baseline`.
+(plain, large), metro64 (plain, small, float64), batched (with `vs_plain`) and served, at the
+offered 200 jobs/s and at a rate below what the device sustains -- each with its own `roofline`
+and `cpu_baseline`.
 
 Prints ONE JSON line. `roofline` is for the dominant kernel: algorithmic bytes per launch /
 average launch duration, measured with HIP events recorded on the launch stream around every
@@ -275,10 +277,10 @@ def bench_metro(args, large):
                 "grid r=20a, FP32+Kahan, FP64 recompute every ~16 sweeps")
         kwin = 20
     else:
-        mm, kw, sweeps, warm = oop_metropolis(), dict(r=8 * a), 200, 1
+        mm, kw, sweeps, warm = oop_metropolis(fp64=True), dict(r=8 * a), 200, 1
         desc = ("plain Metropolis: OOP_Square 64x64 (4096 magnets), a=200 nm, moment 1.1e-15, T=300 K, "
-                "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a; the small configuration's shape at "
-                "FP32+Kahan fields with FP64 acceptance, not its float64 stepping (error vs FP64 stated)")
+                "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a, float64 fields, increments and "
+                "acceptance (Magnets(fp64_fields=True))")
         kwin = 10 ** 6
     steps = sweeps if args.steps is None else args.steps
     mm.progress(t_max=math.inf, MCsteps_max=warm, **kw)
@@ -306,11 +308,12 @@ def bench_metro(args, large):
         nt_per_acc = mm.n * frac * len(sample) / max(1, acc_s)
         bytes_total = attempts * 5 + nsteps * frac * mm.n * 17
     err = rel_field_error(mm)
-    line = base_line(args, attempts / wall, steps, 0, wall, "f32+kahan (fields), f64 (acceptance)",
+    dtype = "f32+kahan (fields), f64 (acceptance)" if large else "f64 (fields, increments, acceptance)"
+    line = base_line(args, attempts / wall, steps, 0, wall, dtype,
                      {"workload": desc, "attempts": attempts, "accepted": accepted, "device_steps": nsteps,
                       "rel_E_err_vs_fp64": err, "window_cells_per_accept": nt_per_acc, "parallelism": "single GPU"})
     kname = ("k_metro_presample + k_metro_tiles + k_metro_update_one (per progress call)" if large
-             else "k_metro_persist_one (per progress call)")
+             else "k_metro_persist_one_f64 (per progress call)")
     line["roofline"] = roofline(kname, launches, bytes_total)
     if large:
         line["roofline"]["traffic"] = METRO2048_TRAFFIC_PER_STEP * nsteps / len(launches)
@@ -639,6 +642,10 @@ def bench_all(args):
     sec = {}
     sec["metro2048"] = _secondary(bench_metro(sub, True))
     progress("plain Metropolis 2048x2048")
+    small = argparse.Namespace(**vars(sub))
+    small.cpu_seconds = args.cpu_seconds / 2
+    sec["metro64"] = _secondary(bench_metro(small, False))
+    progress("plain Metropolis 64x64 (float64)")
     sec["batched"] = _secondary(bench_batched(sub))
     progress("batched")
     served = bench_served(sub, rate=args.rate, count=args.jobs)

@@ -2,7 +2,7 @@
 
 PyTorch is used only for device memory and the current HIP stream; all arithmetic on the
 stepping path happens in the library's gfx950 kernels. Layout in HBM (row-major (ny, nx)):
-  m int8 | flags uint8 | E, Ec, Ep, Epc f32 (combined fields + Kahan compensation) |
+  m int8 | flags uint8 | E, Ec, Ep, Epc f32 (combined fields + Kahan compensation), or E64 f64 |
   EB, beta, mu f32 | occ_list int32 | kernels K32/Kpo32 f32 and K64/Kps64/Kpo64 f64 of shape
   [n_sub][2kwy+1][2kwx+1] | run state hs_state_t[2] | scratch (Neel partials, Metropolis lists).
 """
@@ -37,6 +37,8 @@ class DeviceState:
         self.Ec = torch.zeros(n, dtype=torch.float32, device=dev)
         self.Ep = torch.zeros(n, dtype=torch.float32, device=dev)
         self.Epc = torch.zeros(n, dtype=torch.float32, device=dev)
+        # the combined field in FP64 (Magnets(fp64_fields=True)); E/Ec are then unused
+        self.E64 = torch.zeros(n, dtype=torch.float64, device=dev) if getattr(mm, "fp64_fields", False) else None
         self.EB = torch.zeros(n, dtype=torch.float32, device=dev)
         self.beta = torch.zeros(n, dtype=torch.float32, device=dev)
         self.beta64c = None  # FP64 per-cell beta, kept only while T is not uniform (Neel waiting times)

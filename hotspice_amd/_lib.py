@@ -36,6 +36,7 @@ class Instance(C.Structure):
         ("Ep", C.c_void_p), ("Epc", C.c_void_p), ("EB", C.c_void_p), ("beta", C.c_void_p), ("mu", C.c_void_p),
         ("mu_src", C.c_void_p), ("mu64", C.c_void_p), ("pw64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
         ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p), ("beta64c", C.c_void_p),
+        ("E64", C.c_void_p),
         ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
         ("J", C.c_double), ("m_perp_factor", C.c_double), ("beta64", C.c_double),
         ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),

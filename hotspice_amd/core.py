@@ -99,13 +99,17 @@ class Magnets(ABC):
       device          HIP device index (default 0).
       recompute_every Neel switches (or Metropolis steps) between FP64 full recomputes of the
                       FP32 fields, which reset incremental drift (default 1024; None disables).
+      fp64_fields     hold the combined field in FP64 and step it with FP64 increments, as the
+                      reference's float64 arrays do (default False: FP32 + Kahan compensation).
+                      Metropolis stepping on grids of at most 80x80 cells, with DipolarEnergy,
+                      ZeemanEnergy and ExchangeEnergy and no E_perp; other stepping raises.
     """
 
     def __init__(self, nx: int, ny: int, dx, dy, *, T=300, E_B=0., moment=None, Msat=800e3, V=2e-22,
                  pattern: str = None, energies=None, PBC: bool = False, m_perp_factor: float = None,
                  E_B_std: float = None, major_axis: float = 0, minor_axis: float = None, angle: float = 0.,
                  params: SimParams = None, in_plane: bool = False, seed: int = None, device: int = None,
-                 recompute_every: int | None = 1024):
+                 recompute_every: int | None = 1024, fp64_fields: bool = False):
         self.seed = int(np.random.SeedSequence().entropy) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed) & 0xFFFFFFFFFFFFFFFF
         self.neel_persistent = True  # one persistent launch per progress() chunk (False: one launch per step)
         self.neel_blocks = 0         # workgroups in the instance's team (0: default, 'auto': autotuned)
@@ -114,6 +118,7 @@ class Magnets(ABC):
         self.rng = np.random.default_rng(self.seed)  # host draws at construction only ('random' pattern, E_B_std)
         self.params = SimParams() if params is None else params
         self.recompute_every = recompute_every
+        self.fp64_fields = bool(fp64_fields)
         self._metro_rate = None  # measured Metropolis samples per step (sizes launches only)
         self._energies = list()
         self.in_plane = in_plane
@@ -709,6 +714,7 @@ class Magnets(ABC):
         I.Ep, I.Epc, I.EB, I.beta, I.mu, I.mu64 = P(d.Ep), P(d.Epc), P(d.EB), P(d.beta), P(d.mu), P(d.mu64)
         I.occ_list = P(d.occ_list)
         I.beta64c = P(d.beta64c)
+        I.E64 = P(d.E64)
         I.seed = self.seed
         I.nx, I.ny, I.ux, I.uy = self.nx, self.ny, self.unitcell.x, self.unitcell.y
         I.pbc, I.in_plane, I.use_perp = int(self.PBC), int(self.in_plane), int(self.USE_PERP_ENERGY)

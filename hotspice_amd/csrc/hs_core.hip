@@ -212,6 +212,13 @@ int hs_check_instance(const hs_instance_t *I) {
       if (I->sub_of[s] >= I->n_sub) return hs_set_error(HS_ERR_ARG, "sub_of[%d] out of range", s);
   }
   if (I->n_occ <= 0 || I->n_occ > I->nx * I->ny) return hs_set_error(HS_ERR_ARG, "bad n_occ %d", I->n_occ);
+  if (I->E64 && (I->use_perp || I->pair_src_unit))
+    return hs_set_error(HS_ERR_ARG, "FP64 fields (E64) hold neither E_perp nor DiMonopolarEnergy pair terms");
+  return HS_OK;
+}
+
+int hs_check_f32_fields(const hs_instance_t *I, const char *who) {
+  if (I && I->E64) return hs_set_error(HS_ERR_ARG, "%s steps FP32 fields only; this instance holds FP64 fields (E64)", who);
   return HS_OK;
 }
 
@@ -707,7 +714,9 @@ __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const 
     e += xe;
     ep += xp;
   }
-  if (do_e) {
+  if (do_e && I.E64) {
+    I.E64[c] = e;
+  } else if (do_e) {
     const float h = (float)e;
     I.E[c] = h;
     I.Ec[c] = (float)((double)h - e);
@@ -1061,7 +1070,7 @@ __global__ void k_field_components(hs_instance_t I, double *out) {
   out[1 * (size_t)ncell + c] = I.use_perp ? zp : 0.0;
   out[2 * (size_t)ncell + c] = xe;
   out[3 * (size_t)ncell + c] = I.use_perp ? xp : 0.0;
-  out[4 * (size_t)ncell + c] = (double)I.E[c] - (double)I.Ec[c];
+  out[4 * (size_t)ncell + c] = I.E64 ? I.E64[c] : (double)I.E[c] - (double)I.Ec[c];
   out[5 * (size_t)ncell + c] = I.use_perp ? (double)I.Ep[c] - (double)I.Epc[c] : 0.0;
 }
 
@@ -1083,7 +1092,7 @@ __global__ void __launch_bounds__(1024) k_energy_total(hs_instance_t I, double *
   double acc = 0.0;
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
     const double mi = (double)I.m[c];
-    double v = (double)I.E[c] - (double)I.Ec[c];
+    double v = I.E64 ? I.E64[c] : (double)I.E[c] - (double)I.Ec[c];
     if (I.has_zeeman && mi != 0.0) {
       double Bx, By, f, fp;
       field_at(I, c, Bx, By);
@@ -1131,10 +1140,14 @@ __global__ void k_zeeman_swap(hs_instance_t I, double Bx0, double By0, const flo
   field_at(I, c, Bx, By);
   zeeman_factors(I, c, site, Bx, By, f1, fp1);
   zeeman_factors(I, c, site, bx0 ? (double)bx0[c] : Bx0, by0 ? (double)by0[c] : By0, f0, fp0);
-  float s = I.E[c], cc = I.Ec[c];
-  hs_kahan_add(s, cc, (float)(mi * (f1 - f0)));
-  I.E[c] = s;
-  I.Ec[c] = cc;
+  if (I.E64) {
+    I.E64[c] += mi * (f1 - f0);
+  } else {
+    float s = I.E[c], cc = I.Ec[c];
+    hs_kahan_add(s, cc, (float)(mi * (f1 - f0)));
+    I.E[c] = s;
+    I.Ec[c] = cc;
+  }
   if (I.use_perp) {
     float sp = I.Ep[c], cp = I.Epc[c];
     hs_kahan_add(sp, cp, (float)(mi * (fp1 - fp0)));

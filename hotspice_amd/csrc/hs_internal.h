@@ -40,6 +40,10 @@ int hs_graph_steps();
 // Validates the fields every stepping kernel relies on (sizes, pointers, table bounds).
 int hs_check_instance(const hs_instance_t *I);
 
+// HS_ERR_ARG (naming `who`) for an instance with FP64 fields (E64), which only the recompute,
+// readouts, Zeeman steps and small-grid Metropolis steps support.
+int hs_check_f32_fields(const hs_instance_t *I, const char *who);
+
 // Run-state helpers (hs_neel.hip): reset per-call fields of state[0]; copy state[from] to state[0].
 __global__ void k_state_begin(hs_state_t *st, int begin);
 __global__ void k_state_copy(hs_state_t *st, int from);

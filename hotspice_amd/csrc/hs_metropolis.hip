@@ -27,6 +27,7 @@
 #include <cmath>
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "hs_device.h"
@@ -212,7 +213,7 @@ HS_INLINE int block_exclusive_scan(int v, int *s_tmp, int &total) {
 
 // min-barrier of a sample for METROPOLIS_TIME (E_barrier(idx, min_only=True), core.py:625-656), FP64.
 HS_INLINE double barrier_min_f64(const hs_instance_t &I, int c) {
-  const double e = (double)I.E[c] - (double)I.Ec[c];
+  const double e = I.E64 ? I.E64[c] : (double)I.E[c] - (double)I.Ec[c];
   const double dE = -2.0 * e, El = e, eb = (double)I.EB[c];
   if (I.use_perp) {
     const double ep = I.m_perp_factor * ((double)I.Ep[c] - (double)I.Epc[c]);
@@ -387,9 +388,19 @@ HS_INLINE int grid_cell(const hs_instance_t &I, const hs_select_t &SEL, int ox, 
 // Metropolis acceptance of a sample at cell c (core.py:662-671): exp(-switch_energy*beta) with
 // switch_energy = -2E from the compensated field, relative error ~3e-7 (FP64 exponent split, FP32
 // mantissa), clipped to [1e-10, 1e10], against the Philox uniform (cell, step, tag 4).
+HS_INLINE bool metro_accepts_e(const hs_instance_t &I, uint64_t step, int c, double e, float beta);
 HS_INLINE bool metro_accepts(const hs_instance_t &I, uint64_t step, int c, float E, float Ec, float beta) {
-  const double e = (double)E - (double)Ec;
-  double p = hs_scaled_exp(1.f, 2.0 * e * (double)beta, 1.f, 1.0);
+  return metro_accepts_e(I, step, c, (double)E - (double)Ec, beta);
+}
+// The same from the field's value e (FP64 fields: E64).
+// With FP64 fields the probability is the FP64 exp of the FP64 product (beta64 / beta64c when
+// present), as the reference evaluates it.
+HS_INLINE bool metro_accepts_e(const hs_instance_t &I, uint64_t step, int c, double e, float beta) {
+  double p;
+  if (I.E64)
+    p = exp(2.0 * e * (I.beta64c ? I.beta64c[c] : (I.uniform_beta ? I.beta64 : (double)beta)));
+  else
+    p = hs_scaled_exp(1.f, 2.0 * e * (double)beta, 1.f, 1.0);
   p = fmin(fmax(p, 1e-10), 1e10);
   const double ua = hs_u53(hs_bits53(I.seed, step, 4u, (uint64_t)c), 0);
   return ua < p;
@@ -604,8 +615,9 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
       mv[u] = 0;
       ev[u] = ecv[u] = btv[u] = muv[u] = 0.f;
       if (cs[u] >= 0) {
-        ev[u] = I.E[cs[u]];
-        ecv[u] = I.Ec[cs[u]];
+        // FP64 fields: the value in E64 (ev - ecv below then stands for it exactly)
+        ev[u] = I.E64 ? 0.f : I.E[cs[u]];
+        ecv[u] = I.E64 ? 0.f : I.Ec[cs[u]];
         btv[u] = I.beta[cs[u]];
         mv[u] = (int)I.m[cs[u]];
         muv[u] = I.mu_src[cs[u]];  // source weight (1 for DiMonopolarEnergy)
@@ -626,7 +638,7 @@ __device__ __forceinline__ int metro_select(const MetroJob &J, WorkList WL = Wor
           const double b64 = I.uniform_beta ? I.beta64 : (double)btv[u];
           tsum += -log(ut) / R.attempt_freq / n * exp(barrier_min_f64(I, c) * b64);
         }
-        acc = metro_accepts(I, step, c, ev[u], ecv[u], btv[u]);
+        acc = I.E64 ? metro_accepts_e(I, step, c, I.E64[c], btv[u]) : metro_accepts(I, step, c, ev[u], ecv[u], btv[u]);
       }
       if (single) {
         if (tid == 0) {
@@ -787,9 +799,12 @@ HS_INLINE int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) /
 // PRELOAD: load the cell's own state (m, mu, fields) at the start, with the step's counts, for
 // tiles known to be reached (the worklist); otherwise only once a source reaches the tile (the
 // persistent kernel visits every tile and keeps its registers for the selection).
-template <int TX, int TY, bool PRELOAD>
+// F64 (an instance with FP64 fields, hs_instance_t.E64): the increments are FP64 sums of
+// K64 * (m mu64) and exchange terms, added to E64 without compensation.
+template <int TX, int TY, bool PRELOAD, bool F64 = false>
 __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int tile_y) {
   constexpr int MU_TX = TX, MU_TY = TY, MU_THREADS = TX * TY;
+  using FT = typename std::conditional<F64, double, float>::type;
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const MetroWS &ws = J.ws;
@@ -799,6 +814,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   __shared__ int s_cnt[MU_THREADS / 64 + 1];
   __shared__ int s_sub[HS_MAX_UC], s_trunc[HS_MAX_UC];
   __shared__ float s_oxu[HS_MAX_UC], s_oyu[HS_MAX_UC], s_nn[HS_MAX_NN];
+  __shared__ double s_mmu64[F64 ? MU_THREADS : 1];  // F64: the sources' (m mu64) after their flips
   const int tid = threadIdx.x;
   const int x0 = tile_x * MU_TX, y0 = tile_y * MU_TY;
   const int x = x0 + (tid % MU_TX), y = y0 + (tid / MU_TX);
@@ -826,8 +842,8 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   // descriptor (fewer live registers in the persistent kernel)
   auto sub_of = [&](int ss) { return PRELOAD ? s_sub[ss] : I.sub_of[ss]; };
   auto trunc_of = [&](int ss) { return PRELOAD ? s_trunc[ss] : ws.trunc[ss]; };
-  auto ox_of = [&](int ss) { return PRELOAD ? s_oxu[ss] : (float)I.ox_uc[ss]; };
-  auto oy_of = [&](int ss) { return PRELOAD ? s_oyu[ss] : (float)I.oy_uc[ss]; };
+  auto ox_of = [&](int ss) { return F64 ? (FT)I.ox_uc[ss] : (FT)(PRELOAD ? s_oxu[ss] : (float)I.ox_uc[ss]); };
+  auto oy_of = [&](int ss) { return F64 ? (FT)I.oy_uc[ss] : (FT)(PRELOAD ? s_oyu[ss] : (float)I.oy_uc[ss]); };
   int m0 = 0;
   float mu0 = 0.f, E0 = 0.f, Ec0 = 0.f, Ep0 = 0.f, Epc0 = 0.f;
   if (PRELOAD && inside) {
@@ -849,12 +865,14 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
   const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
-  float oxi = 0.f, oyi = 0.f;
-  float acc = 0.f, accp = 0.f, ex = 0.f, exp_ = 0.f;
+  FT oxi = 0, oyi = 0;
+  FT acc = 0, ex = 0;
+  float accp = 0.f, exp_ = 0.f;
   bool touched = false, flipped = false;
 
-  // the contribution of one switched source (cell scell, (m mu) after its flip, unit-cell site ss)
-  auto add_source = [&](int scell, float mm, int ss) {
+  // the contribution of one switched source (cell scell, (m mu) after its flip, unit-cell site ss,
+  // staged at LDS slot r)
+  auto add_source = [&](int scell, float mm, int ss, int r) {
     const int sy = scell / nx, sx = scell % nx;
     if (has_dip) {
       const int n = sub_of(ss);
@@ -870,8 +888,12 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         for (int a1 = 0; a1 < ndy; ++a1)
           for (int a2 = 0; a2 < ndx; ++a2) {
             const size_t o = (size_t)(I.kwy + dys[a1]) * KW + (I.kwx + dxs[a2]);
-            acc += mm * Kn[o];
-            if (Kp) accp += mm * Kp[o];
+            if (F64) {
+              acc += s_mmu64[r] * I.K64[(size_t)n * KS + o];
+            } else {
+              acc += mm * Kn[o];
+              if (Kp) accp += mm * Kp[o];
+            }
             touched = true;
           }
       }
@@ -888,13 +910,13 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         const int q = (dy + hy) * I.nn_w + (dx + hx);
         const float wq = PRELOAD ? s_nn[q] : I.nn_mask[q];
         if (wq != 0.f) {
-          const float msn = mm > 0.f ? 1.f : -1.f;  // m of the source after its flip
+          const FT msn = mm > 0.f ? 1 : -1;  // m of the source after its flip
           if (I.in_plane) {
-            const float oxs = ox_of(ss), oys = oy_of(ss);
-            ex += msn * wq * (oxi * oxs + oyi * oys);
-            exp_ += msn * wq * (oys * oxi - oxs * oyi);
+            const FT oxs = ox_of(ss), oys = oy_of(ss);
+            ex += msn * (FT)wq * (oxi * oxs + oyi * oys);
+            if (!F64) exp_ += (float)(msn * (FT)wq * (oys * oxi - oxs * oyi));
           } else {
-            ex += msn * wq;
+            ex += msn * (FT)wq;
           }
           touched = true;
         }
@@ -977,6 +999,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       s_cell[p] = sc;
       s_mmu[p] = smmu;
       s_site_of[p] = ssite;
+      if (F64) s_mmu64[p] = (smmu > 0.f ? 1.0 : -1.0) * I.mu64[sc];
     }
     __syncthreads();
     if (nrel == 0) continue;
@@ -991,13 +1014,19 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
     if (!inside || mi == 0) continue;
     for (int r = 0; r < nrel; ++r) {
       flipped = flipped || s_cell[r] == c;
-      add_source(s_cell[r], s_mmu[r], s_site_of[r]);
+      add_source(s_cell[r], s_mmu[r], s_site_of[r], r);
     }
   }
   if (!inside || mi == 0 || mi == 2 || !(touched || flipped)) return;
   if (flipped) {  // the flip first (E[w] *= -1 before the sum, energies.py:332-334)
     mi = -mi;
     I.m[c] = (int8_t)mi;
+  }
+  if (F64) {  // E[w] *= -1 for the flip, then the FP64 increment (no compensation needed)
+    double e = (flipped ? -1.0 : 1.0) * I.E64[c];
+    if (touched) e += 2.0 * I.dip_pf * ((double)mi * I.mu64[c]) * (double)acc + (-2.0 * I.J) * ((double)mi * (double)ex);
+    I.E64[c] = e;
+    return;
   }
   const float sgn = flipped ? -1.f : 1.f;
   if (!PRELOAD) {
@@ -1009,7 +1038,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   // E += 2*prefactor*(m mu)_i*conv (energies.py:362-363); exchange: -J m_i * 2 m_s * NN
   float e = sgn * E0, ec = sgn * Ec0;
   if (touched) {
-    const float de = (float)(2.0 * I.dip_pf) * (mmui * acc) + (float)(-2.0 * I.J) * ((float)mi * ex);
+    const float de = (float)(2.0 * I.dip_pf) * (mmui * (float)acc) + (float)(-2.0 * I.J) * ((float)mi * (float)ex);
     hs_kahan_add(e, ec, de);
   }
   I.E[c] = e;
@@ -1328,6 +1357,7 @@ __global__ void k_metro_clear_many(WorkList WL, const MetroJob *__restrict__ job
 // selection, then the update of every 32x32 tile of its grid, step after step, until the job's
 // stop predicate or step budget ends it. The arithmetic is that of the two-launch step (the same
 // selection code; every cell adds the same sources in the same order whatever the tile shape).
+template <bool F64 = false>
 __device__ __forceinline__ void metro_persist(const MetroJob &J) {
   const int tx = (J.I.nx + 31) / 32, ty = (J.I.ny + 31) / 32;
   for (int64_t s = 0; s < J.R.max_steps; ++s) {
@@ -1335,12 +1365,15 @@ __device__ __forceinline__ void metro_persist(const MetroJob &J) {
     __syncthreads();
     if (accepted < 0) break;
     if (accepted == 0) continue;  // nothing switched: no field changes this step
-    for (int t = 0; t < tx * ty; ++t) metro_update<32, 32, false>(J, t % tx, t / tx);
+    for (int t = 0; t < tx * ty; ++t) metro_update<32, 32, false, F64>(J, t % tx, t / tx);
     __syncthreads();
   }
 }
 
 __global__ void __launch_bounds__(MS_THREADS) k_metro_persist_one(MetroJob J) { metro_persist(J); }
+
+// An instance with FP64 fields (E64).
+__global__ void __launch_bounds__(MS_THREADS) k_metro_persist_one_f64(MetroJob J) { metro_persist<true>(J); }
 
 __global__ void __launch_bounds__(MS_THREADS) k_metro_persist_many(const MetroJob *__restrict__ jobs,
                                                                    const int32_t *__restrict__ which) {
@@ -1684,6 +1717,8 @@ extern "C" int hs_relax_sweep(const hs_instance_t *inst, const int32_t *order, i
   hs_clear_error();
   int rc = hs_check_instance(inst);
   if (rc) return rc;
+  rc = hs_check_f32_fields(inst, "hs_relax_sweep");
+  if (rc) return rc;
   if (!state || !ws || (n_order > 0 && !order) || n_order < 0) return hs_set_error(HS_ERR_ARG, "missing order/state/ws");
   if (step_x < 1 || step_y < 1) return hs_set_error(HS_ERR_ARG, "bad subset steps %d,%d", step_x, step_y);
   if (cutoff < 0 || reduced_k < 0) return hs_set_error(HS_ERR_ARG, "negative cutoff / reduced kernel size");
@@ -1726,6 +1761,8 @@ extern "C" int hs_minimize_scan(const hs_instance_t *inst, int32_t with_barrier,
   hs_clear_error();
   int rc = hs_check_instance(inst);
   if (rc) return rc;
+  rc = hs_check_f32_fields(inst, "hs_minimize_scan");
+  if (rc) return rc;
   if (!cells || !result) return hs_set_error(HS_ERR_ARG, "missing cells/result");
   k_min_scan<<<1, MS_THREADS, 0, (hipStream_t)stream>>>(*inst, with_barrier ? 1 : 0, cells, result);
   HS_LAUNCH_CHECK("k_min_scan");
@@ -1741,6 +1778,8 @@ extern "C" int hs_flip_cells(const hs_instance_t *inst, const int32_t *cells, in
                              int32_t reduced_k, hs_state_t *state, void *ws, int64_t ws_bytes_, void *stream) {
   hs_clear_error();
   int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  rc = hs_check_f32_fields(inst, "hs_flip_cells");
   if (rc) return rc;
   const int ncell = inst->nx * inst->ny;
   if (!state || !ws || k < 0 || k > ncell || (k > 0 && !cells)) return hs_set_error(HS_ERR_ARG, "bad flip arguments");
@@ -1781,6 +1820,8 @@ extern "C" int hs_wolff_run(const hs_instance_t *inst, const hs_run_t *run, int3
                             void *stream) {
   hs_clear_error();
   int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  rc = hs_check_f32_fields(inst, "hs_wolff_run");
   if (rc) return rc;
   if (!run || !state || !ws) return hs_set_error(HS_ERR_ARG, "missing run/state/ws");
   if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
@@ -1839,9 +1880,17 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   rc = check_select(sel);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (inst->E64 && inst->nx * inst->ny > PERSIST_CELLS)
+    return hs_set_error(HS_ERR_ARG, "FP64 fields (E64) step on grids of at most %d cells (this one has %d)", PERSIST_CELLS,
+                        inst->nx * inst->ny);
   const MetroJob J = make_job(inst, sel, run, state, ws, log_cells, log_count);
   k_state_begin<<<1, 1, 0, s>>>(state, run->begin ? 1 : 0);
   HS_LAUNCH_CHECK("k_state_begin");
+  if (inst->E64) {
+    k_metro_persist_one_f64<<<1, MS_THREADS, 0, s>>>(J);
+    HS_LAUNCH_CHECK("k_metro_persist_one_f64");
+    return HS_OK;
+  }
   if (inst->nx * inst->ny <= PERSIST_CELLS) {
     k_metro_persist_one<<<1, MS_THREADS, 0, s>>>(J);
     HS_LAUNCH_CHECK("k_metro_persist_one");
@@ -2025,6 +2074,8 @@ extern "C" int hs_metro_run_teams(const hs_team_t *teams, int32_t nteams, void *
   for (int i = 0; i < nteams; ++i) {
     const hs_team_t &t = teams[i];
     int rc = hs_check_instance(t.inst);
+    if (rc) return rc;
+    rc = hs_check_f32_fields(t.inst, "hs_metro_run_teams");
     if (rc) return rc;
     if (!t.sel || !t.state) return hs_set_error(HS_ERR_ARG, "team %d has no selection / state", i);
     rc = check_select(t.sel);

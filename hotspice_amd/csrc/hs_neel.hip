@@ -412,6 +412,8 @@ extern "C" int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_st
   hs_clear_error();
   int rc = hs_check_instance(inst);
   if (rc) return rc;
+  rc = hs_check_f32_fields(inst, "hs_neel_run");
+  if (rc) return rc;
   if (!run || !state || !partials) return hs_set_error(HS_ERR_ARG, "missing run/state/partials");
   if (run->max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
   if (run->log && (!log_idx || !log_dt)) return hs_set_error(HS_ERR_ARG, "log requested without arrays");

@@ -679,6 +679,8 @@ extern "C" int hs_neel_run_teams(const hs_team_t *teams, int32_t nteams, void *w
     const hs_team_t &t = teams[i];
     int rc = hs_check_instance(t.inst);
     if (rc) return rc;
+    rc = hs_check_f32_fields(t.inst, "hs_neel_run_teams");
+    if (rc) return rc;
     if (!t.state) return hs_set_error(HS_ERR_ARG, "team %d has no state", i);
     if (t.run.max_steps < 0) return hs_set_error(HS_ERR_ARG, "negative step budget");
     if (t.run.log && (!t.log_idx || !t.log_dt)) return hs_set_error(HS_ERR_ARG, "team %d: log without arrays", i);

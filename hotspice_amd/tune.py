@@ -89,7 +89,7 @@ def metro_candidates(mm):
 
 
 def _snapshot(dev):
-    names = ("m", "E", "Ec", "Ep", "Epc", "state")
+    names = ("m", "E", "Ec", "Ep", "Epc", "E64", "state")
     return {n: getattr(dev, n).clone() for n in names if getattr(dev, n, None) is not None}
 
 

@@ -68,6 +68,12 @@ typedef struct hs_instance {
   const int32_t *occ_list; /* [n_occ] flat indices of occupied cells, row-major (core.py:97-98) */
   const double *beta64c;  /* (ny,nx) 1/(kB T) in FP64: Neel stepping with uniform_beta = 0 needs it
                              (the per-cell waiting times); NULL otherwise */
+  double *E64;          /* (ny,nx) combined local field in FP64, or NULL. When set, the instance holds
+                           its field here instead of in E/Ec (which are then unused): the recompute
+                           writes it, readouts and Zeeman steps use it, and Metropolis steps on grids
+                           of at most 80x80 cells update it with FP64 increments from K64 and mu64
+                           (DipolarEnergy / ExchangeEnergy / ZeemanEnergy, no E_perp). Every other
+                           stepping entry point refuses such an instance with HS_ERR_ARG. */
   uint64_t seed;        /* Philox key */
   double zee_Bx, zee_By; /* uniform field components when zee_bx/zee_by are NULL */
   double dip_pf;        /* DipolarEnergy.prefactor (0 = no dipolar energy) */
