@@ -119,8 +119,9 @@ def launch_neel(items, attempt_freq=1e10, begin=False, full=None, updates=False)
 
 def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
     """The launches of launch_neel on the current stream, without reading anything back.
-    free[i]: hs_run_t.first_free of item i (0: a progress() call, 2: update() calls). Returns the
-    instances whose recompute was queued behind the launch (see launch_neel's `full`)."""
+    free[i]: hs_run_t.first_free of item i (0: a progress() call, 2: update() calls). begin: one
+    flag for every item or one per item. Returns the instances whose recompute was queued behind
+    the launch (see launch_neel's `full`)."""
     mags = [it[0] for it in items]
     dev = mags[0]._dev
     for m in mags:
@@ -128,13 +129,14 @@ def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
             raise NotImplementedError("the batched Neel launch (persistent team kernel) needs a uniform temperature; "
                                       "progress_many steps instances with a varying one through Magnets.progress")
     free = [0] * len(items) if free is None else list(free)
+    begin = [int(begin)] * len(items) if np.ndim(begin) == 0 else [int(b) for b in begin]
     # one launch per E_perp setting: the team kernel is compiled for either, and instances are
     # independent, so two launches in stream order step each exactly as one would
     perp = [bool(m.USE_PERP_ENERGY) for m in mags]
     for flag in sorted(set(perp)):
         g = [i for i, p in enumerate(perp) if p == flag]
         teams = _team_array([mags[i] for i in g], [items[i][1] for i in g], [items[i][2] for i in g],
-                            [items[i][3] for i in g], attempt_freq, begin, nblocks=[items[i][4] for i in g],
+                            [items[i][3] for i in g], attempt_freq, [begin[i] for i in g], nblocks=[items[i][4] for i in g],
                             first_free=[free[i] for i in g])
         dev.run_teams(teams, "neel")
     queued = []
@@ -162,7 +164,7 @@ def _team_array(mags, steps, t_max, mc, attempt_freq, begin, nblocks=None, sels=
     run['mc_max'] = np.asarray(mc, dtype=float)
     run['attempt_freq'] = float(attempt_freq)
     run['max_steps'] = np.asarray(steps, dtype=np.int64)
-    run['begin'] = int(begin)
+    run['begin'] = np.asarray(begin, dtype=np.int32)  # one flag for all, or one per instance
     run['first_free'] = np.asarray(first_free, dtype=np.int32)
     if nblocks is not None:
         arr['nblocks'] = np.asarray(nblocks, dtype=np.int32)
@@ -186,7 +188,8 @@ def launch_metropolis(items, attempt_freq=1e10, begin=False):
 
 
 def enqueue_metropolis(items, attempt_freq=1e10, begin=False):
-    """The launches of launch_metropolis on the current stream, without reading anything back."""
+    """The launches of launch_metropolis on the current stream, without reading anything back
+    (begin: one flag for every item or one per item)."""
     mags = [it[0] for it in items]
     teams = _team_array(mags, [it[2] for it in items], [it[3] for it in items], [it[4] for it in items],
                         attempt_freq, begin, sels=[it[1] for it in items], grid=[m._metro_grid() for m in mags])

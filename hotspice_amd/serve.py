@@ -119,6 +119,10 @@ class _Active:
         if self.zeeman is None:
             raise JobError("a served job's instance needs a ZeemanEnergy")
         self.index = 0
+        # the field step's first launch sets its progress origin on the device (hs_run_t.begin:
+        # t0, attempts0 := t, attempts); until it has run, the step cannot be done
+        self.pending_begin = False
+        self.st = None  # the job's state as last read back (by the admission or a collect)
         job.admitted = now
         job.status = "running"
 
@@ -128,44 +132,51 @@ class _Active:
 
     def begin(self):
         """Applies field steps from the current one on: sets the field, and stops at the first step
-        that advances the instance (its origin is recorded for the stop test). Returns True when
-        the program is complete."""
+        that advances the instance, whose first launch will record its origin (t, attempts) on the
+        device. Returns True when the program is complete."""
         while self.index < len(self.steps):
             st = self.current
             self.zeeman.set_field(magnitude=st.magnitude, angle=st.angle)
             if st.advance != 'none':
                 self.mm._pre_step()
-                s = self.mm._state()
-                self.start = (s.t, s.attempts)
-                self.mm._set_progress_origin(*self.start)
+                self.pending_begin = True
                 return False
             self.index += 1
         return True
 
+    def origin(self):
+        """(t0, attempts0) of the current field step: the state's own origin once the step's first
+        launch has set it, else the state as it is (the origin that launch will set)."""
+        st = self.st
+        return (st.t, st.attempts) if self.pending_begin else (st.t0, st.attempts0)
+
     def step_done(self, st) -> bool:
         """'update': `count` update() calls have run (each steps unconditionally, also once t has
         overflowed to +inf). 'progress': the call has returned, i.e. its predicate (core.py:756)
-        fails on the MC steps or on the elapsed time (also an infinite or NaN one)."""
+        fails on the MC steps or on the elapsed time (also an infinite or NaN one). A step whose
+        first launch has not run yet is not done."""
+        if self.pending_begin:
+            return False
         cur = self.current
         if cur.advance == 'update':
-            return st.attempts - self.start[1] >= cur.count
-        return not (lower_than(st.t - self.start[0], cur.t_max)
-                    and lower_than(st.attempts / self.mm.n - self.start[1] / self.mm.n, cur.MCsteps_max))
+            return st.attempts - st.attempts0 >= cur.count
+        return not (lower_than(st.t - st.t0, cur.t_max)
+                    and lower_than(st.attempts / self.mm.n - st.attempts0 / self.mm.n, cur.MCsteps_max))
 
-    def neel_item(self, st, chunk, nblocks):
+    def neel_item(self, chunk, nblocks):
         """(launch item, first_free) of this Neel job's next launch."""
-        cur = self.current
+        cur, st, start = self.current, self.st, self.origin()
         if cur.advance == 'update':
-            steps = min(cur.count - (st.attempts - self.start[1]), chunk)
+            steps = min(cur.count - (st.attempts - start[1]), chunk)
             if self.mm.recompute_every:
                 steps = min(steps, max(1, self.mm.recompute_every - self.mm._since_recompute))
             return (self.mm, max(1, steps), cur.t_max, math.inf, nblocks), 2
-        steps = self.batch.neel_chunk(self.mm, self.start, cur.t_max, cur.MCsteps_max, chunk, st)
+        steps = self.batch.neel_chunk(self.mm, start, cur.t_max, cur.MCsteps_max, chunk, st)
         return (self.mm, steps, cur.t_max, cur.MCsteps_max, nblocks), 0
 
-    def metro_item(self, st, chunk):
+    def metro_item(self, chunk):
         cur = self.current
-        steps = self.batch.metropolis_chunk(self.mm, self.sel, self.start, cur.MCsteps_max, chunk, st)
+        steps = self.batch.metropolis_chunk(self.mm, self.sel, self.origin(), cur.MCsteps_max, chunk, self.st)
         return (self.mm, self.sel, steps, cur.t_max, cur.MCsteps_max)
 
 
@@ -233,6 +244,7 @@ class Server:
                 mm = job.make()
                 a = _Active(job, mm, now, self.batch)
                 complete = a.begin()
+                a.st = self.batch.read_states([mm])[0]
             except Exception as e:  # noqa: BLE001 -- one job's error must not stop the others
                 self._fail(job, e, now, mm, counted=False)
                 continue
@@ -277,11 +289,12 @@ class Server:
         return good
 
     def _iteration(self):
+        """One launch per scheme over the active jobs, sized from the states the last collect read
+        back (no read before the launches), then one collect of the launched jobs' states."""
         neel = [a for a in self.active if a.scheme == Scheme.NEEL]
         metro = [a for a in self.active if a.scheme == Scheme.METROPOLIS]
         self._pending_fail = []
         bt = self.batch
-        sts = dict(zip((id(a) for a in self.active), bt.read_states([a.mm for a in self.active])))
         groups = []
         if neel:
             # round-robin packing of teams up to the co-resident workgroup limit
@@ -292,18 +305,22 @@ class Server:
                 g = bt.neel_team_size(a.mm)
                 if items and used + g > self.max_wg:
                     continue
-                it, free = a.neel_item(sts[id(a)], self.neel_chunk, g)
+                it, free = a.neel_item(self.neel_chunk, g)
                 items.append(it)
                 frees.append(free)
                 acts.append(a)
                 used += g
             free_of = {id(it[0]): f for it, f in zip(items, frees)}
+            begin_of = {id(a.mm): int(a.pending_begin) for a in acts}
             groups.append(("neel", (items, acts),
-                           lambda its: bt.enqueue_neel(its, 1e10, free=[free_of[id(i[0])] for i in its])))
+                           lambda its: bt.enqueue_neel(its, 1e10, free=[free_of[id(i[0])] for i in its],
+                                                       begin=[begin_of[id(i[0])] for i in its])))
         if metro:
-            items = [a.metro_item(sts[id(a)], self.metro_chunk) for a in metro]
+            items = [a.metro_item(self.metro_chunk) for a in metro]
             f0 = metro[0].job.kwargs.get('attempt_freq', 1e10)
-            groups.append(("metropolis", (items, metro), lambda its: bt.enqueue_metropolis(its, f0)))
+            mbegin_of = {id(a.mm): int(a.pending_begin) for a in metro}
+            groups.append(("metropolis", (items, metro),
+                           lambda its: bt.enqueue_metropolis(its, f0, begin=[mbegin_of[id(i[0])] for i in its])))
         stepped = []
         if self.overlap and len(groups) == 2:
             torch = self.torch
@@ -318,10 +335,13 @@ class Server:
             for what, group, enqueue in groups:
                 stepped.append(self._launch(group, enqueue))
         acts = [a for g in stepped for a in g]
-        before = [sts[id(a)].attempts for a in acts]
+        for a in acts:
+            a.pending_begin = False  # its launch has set the field step's origin
+        before = [a.st.attempts for a in acts]
         after, errs = bt.collect([a.mm for a in acts], before)
         now = self.clock() - self._t0
         for i, (a, st) in enumerate(zip(acts, after)):
+            a.st = st
             if i in errs:
                 self._pending_fail.append((a, f"device step error (err={errs[i]})"))
             else:
@@ -343,7 +363,6 @@ class Server:
                         if a.begin():
                             finished = True
                             break
-                        st = a.mm._state()
                 except Exception as e:  # noqa: BLE001
                     self._fail(a.job, e, now, a.mm)
                     continue

@@ -105,17 +105,20 @@ class FakeBatch:
     def metropolis_chunk(self, m, sel, start, mc, chunk=None, st=None):
         return 2
 
-    def _enqueue(self, items):
+    def _enqueue(self, items, begin):
         if any(it[0].reject for it in items):
             raise _lib.HotspiceError("bad n_occ 0")
-        for it in items:
+        begin = [begin] * len(items) if np.ndim(begin) == 0 else begin
+        for it, b in zip(items, begin):
+            if b:  # hs_run_t.begin: the launch records the progress origin first
+                it[0]._set_progress_origin(it[0].st.t, it[0].st.attempts)
             it[0].step(it[1] if it[0].params.UPDATE_SCHEME == Scheme.NEEL else it[2])
 
     def enqueue_neel(self, items, attempt_freq=1e10, begin=False, free=None, full=None):
-        self._enqueue(items)
+        self._enqueue(items, begin)
 
     def enqueue_metropolis(self, items, attempt_freq=1e10, begin=False):
-        self._enqueue(items)
+        self._enqueue(items, begin)
 
     def collect(self, mags, before=None, recomputed=()):
         sts = self.read_states(mags)
