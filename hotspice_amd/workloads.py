@@ -59,11 +59,14 @@ def oop_metropolis_inputs(n=64, seed=0):
 
 
 def oop_metropolis(n=64, seed=0, fp64=False):
-    """The 64x64 plain Metropolis configuration; fp64 holds and steps its field in float64."""
+    """The 64x64 plain Metropolis configuration; fp64 holds and steps its field in float64, with
+    no periodic recompute (as the reference's float64 arrays have none: there is no FP32 drift
+    to reset)."""
     import hotspice_amd as hs
     p = oop_metropolis_inputs(n, seed)
     return hs.ASI.OOP_Square(p["a"], n, T=p["T"], E_B=p["E_B"], moment=p["moment"], pattern=p["pattern"],
                              energies=[hs.DipolarEnergy()], seed=seed, fp64_fields=fp64,
+                             recompute_every=None if fp64 else 1024,
                              params=hs.SimParams(UPDATE_SCHEME='Metropolis', MULTISAMPLING_SCHEME='grid',
                                                  REDUCED_KERNEL_SIZE=0))
 
