@@ -85,6 +85,57 @@ def test_fp64_lockstep_with_oracle(kind):
         assert abs(mm.t - om.t) <= 1e-12 * abs(om.t)
 
 
+def _c1_with(seed, **params):
+    """The small configuration's geometry with other selection parameters."""
+    import hotspice_amd as hs
+    from oracle import model
+    n, a, T, mom = 64, 200e-9, 300, 1.1e-15
+    p = _inputs(n, seed, T)
+    params = dict(UPDATE_SCHEME="Metropolis", REDUCED_KERNEL_SIZE=0, **params)
+    mm = hs.ASI.OOP_Square(a, n, T=T, moment=mom, seed=seed, fp64_fields=True, energies=[hs.DipolarEnergy()],
+                           params=hs.SimParams(**params), **p)
+    om = model.make_magnets("OOP_Square", a, n=n, T=T, moment=mom, seed=seed, energies=[model.Dipolar()],
+                            params=params, **p)
+    return mm, om
+
+
+def test_fp64_single_scheme_and_explicit_samples_match_oracle():
+    """The 'single' scheme (core.py:489-494) and the caller's sample lists (update(idx=...),
+    core.py:660-661) reach the FP64 acceptance through the selection's unloaded branch."""
+    mm, om = _c1_with(8, MULTISAMPLING_SCHEME='single')
+    scale = _scale(om)
+    for s in range(300):
+        a = np.asarray(mm.update()).reshape(2, -1)
+        b = np.asarray(om.update()).reshape(2, -1)
+        assert np.array_equal(a, b), f"single step {s}"
+    assert mm.switches > 0 and np.array_equal(mm.m, om.m)
+    assert float(np.max(np.abs(mm.E - om.E))) < 1e-12 * scale
+    mm, om = _c1_with(9)
+    rng = np.random.default_rng(1)
+    for s, k in enumerate([1, 7, 300, 0, 64, 1000, 5]):
+        flat = rng.choice(64 * 64, size=k, replace=False)
+        idx = np.asarray([flat // 64, flat % 64])
+        a = np.asarray(mm.update(idx=idx)).reshape(2, -1)
+        b = np.asarray(om.update(idx=idx)).reshape(2, -1)
+        assert np.array_equal(a, b), f"explicit samples, call {s}"
+    assert np.array_equal(mm.m, om.m)
+    assert float(np.max(np.abs(mm.E - om.E))) < 1e-12 * scale
+
+
+def test_fp64_poisson_steps_keep_exact_fields():
+    """Poisson-disc steps (no exact replay against the reference's sampler, a documented
+    divergence) still add FP64 increments: the fields equal a fresh FP64 recompute."""
+    mm, _ = _c1_with(10, MULTISAMPLING_SCHEME='Poisson')
+    mm.recompute_every = None
+    s0 = mm.switches
+    for _ in range(200):
+        mm.update(r=6 * 200e-9)
+    assert mm.switches > s0
+    E_inc = mm.E
+    mm._device_recompute()
+    assert float(np.max(np.abs(E_inc - mm.E)) / np.max(np.abs(mm.E))) < 1e-12
+
+
 def test_fp64_progress_matches_oracle():
     """progress() with an MCsteps bound: same steps, spins and fields as the oracle's progress()."""
     mm, om, kw = _pair("c1_shape", seed=5)
