@@ -61,6 +61,15 @@ class DeviceState:
         self.rec_scratch = None
         self.gen = 0  # bumped whenever the instance's parameters are refilled
         self._rec_need = None  # (gen, recompute scratch bytes)
+        # FP64 shadow of the dipolar correlations (C64) and the spins it holds (m_ref): a full
+        # recompute of every component stores it, a periodic recompute then refreshes the fields
+        # from it by adding only the changed sources (hs_refresh_fields)
+        self.C64 = None
+        self.m_ref = None
+        self._shadow_ver = 0  # bumped when kernels, weights or occupation change
+        self._shadow = None  # (version, E_perp held) of the full recompute that stored the shadow
+        self.shadow_refreshes = 0  # refreshes since that full recompute
+        self.refreshes = 0  # refreshes run so far
         self.team_ws = None
         self.sel_cache = {}
         self.log_cap = 0
@@ -123,6 +132,7 @@ class DeviceState:
                                                          self.ptr(Kpo64), self.ptr(K32), self.ptr(Kpo32),
                                                          self.stream()))
         torch.cuda.current_stream(dev).synchronize()
+        self.invalidate_shadow()
         self.kern = dict(K64=K64, K32=K32, Kps64=Kps64, Kpo64=Kpo64, Kpo32=Kpo32, kwx=kwx, kwy=kwy, n_sub=ns,
                          _rr=(t_rrx, t_rry))
 
@@ -142,15 +152,68 @@ class DeviceState:
         out[:, ny - 1 - kwy:ny + kwy, nx - 1 - kwx:nx + kwx] = t.cpu().numpy()
         return out
 
+    # ---- FP64 shadow -------------------------------------------------------------------------
+    # a refresh adds at most this many refreshes' rounding before the next full recompute, and is
+    # taken only while the changed cells are at most this fraction of the grid (above it the full
+    # correlation costs less than the list-driven additions)
+    SHADOW_MAX_REFRESHES = 64
+    SHADOW_MAX_CHANGED = 1 / 16
+
+    def shadow_ptrs(self, in_plane, enabled=True):
+        """Device pointers (C64, m_ref) for the instance descriptor, allocating them on first use
+        (C64 holds E and E_perp correlations for in-plane lattices); NULLs when disabled."""
+        if not enabled:
+            self.C64 = self.m_ref = None
+            self.invalidate_shadow()
+            return self.ptr(None), self.ptr(None)
+        n = self.nx * self.ny
+        need = (2 if in_plane else 1) * n
+        if self.C64 is None or self.C64.numel() < need:
+            self.C64 = self.torch.zeros(need, dtype=self.torch.float64, device=self.device)
+            self.m_ref = self.torch.zeros(n, dtype=self.torch.int8, device=self.device)
+            self.invalidate_shadow()
+        return self.ptr(self.C64), self.ptr(self.m_ref)
+
+    def invalidate_shadow(self):
+        """The next periodic recompute is a full one: the kernels, pair weights or occupation the
+        shadow was computed with changed (the dipolar prefactor, Zeeman, exchange, temperatures
+        and barriers do not enter it; the finishing pass applies them)."""
+        self._shadow_ver += 1
+        self._shadow = None
+
+    def _stores_shadow(self, which):
+        I = self.inst
+        return bool(I.C64) and I.dip_pf != 0.0 and bool(which & 1) and bool(which & 2 or not I.use_perp)
+
+    def shadow_usable(self, which, changed):
+        """True when a refresh from the shadow reproduces the full recompute `which` for at most
+        `changed` changed cells (None: unknown)."""
+        return (changed is not None and self._shadow == (self._shadow_ver, bool(self.inst.use_perp))
+                and self.shadow_refreshes < self.SHADOW_MAX_REFRESHES and self._stores_shadow(which)
+                and changed <= self.SHADOW_MAX_CHANGED * self.nx * self.ny)
+
+    def _shadow_stored(self, which):
+        if self._stores_shadow(which):
+            self._shadow = (self._shadow_ver, bool(self.inst.use_perp))
+            self.shadow_refreshes = 0
+
+    def refresh(self, which):
+        """hs_refresh_fields: the fields of the current spins from the shadow."""
+        scr = self.recompute_scratch()
+        _lib.check(self.lib.hs_refresh_fields(C.byref(self.inst), which, self.ptr(scr), scr.numel(), self.stream()))
+        self.shadow_refreshes += 1
+        self.refreshes += 1
+
     # ---- calls -----------------------------------------------------------------------------
-    def recompute(self, which=3):
-        need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
-        if need < 0:
-            raise _lib.HotspiceError("bad instance for recompute")
-        if self.rec_scratch is None or self.rec_scratch.numel() < need:
-            self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
-        _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.ptr(self.rec_scratch),
-                                                self.rec_scratch.numel(), self.stream()))
+    def recompute(self, which=3, changed=None):
+        """FP64 recompute of the fields `which`: a refresh from the shadow when `changed` (an upper
+        bound on the cells whose spin changed since the last recompute) allows it, else a full one."""
+        if self.shadow_usable(which, changed):
+            self.refresh(which)
+            return
+        scr = self.recompute_scratch()
+        _lib.check(self.lib.hs_recompute_fields(C.byref(self.inst), which, self.ptr(scr), scr.numel(), self.stream()))
+        self._shadow_stored(which)
 
     def recompute_scratch(self):
         """The recompute scratch tensor of this instance, sized for it (the size is asked of the
@@ -159,14 +222,27 @@ class DeviceState:
             need = int(self.lib.hs_recompute_scratch_bytes(C.byref(self.inst)))
             if need < 0:
                 raise _lib.HotspiceError("bad instance for recompute")
+            if self.inst.C64:
+                need = max(need, int(self.lib.hs_refresh_scratch_bytes(C.byref(self.inst))))
             self._rec_need = (self.gen, need)
         need = self._rec_need[1]
         if self.rec_scratch is None or self.rec_scratch.numel() < need:
             self.rec_scratch = self.torch.empty(max(need, 8), dtype=self.torch.uint8, device=self.device)
         return self.rec_scratch
 
-    def recompute_many(self, devs, which):
-        """hs_recompute_fields for several instances (DeviceStates on this device) in one call."""
+    def recompute_many(self, devs, which, changed=None):
+        """hs_recompute_fields for several instances (DeviceStates on this device) in one call;
+        those whose shadow allows it (changed[i]: as in recompute) are refreshed from it instead."""
+        if changed is not None:
+            keep = []
+            for d, w, c in zip(devs, which, changed):
+                if d.shadow_usable(w, c):
+                    d.refresh(w)
+                else:
+                    keep.append((d, w))
+            if not keep:
+                return
+            devs, which = [d for d, _ in keep], [w for _, w in keep]
         n = len(devs)
         insts = (C.c_void_p * n)(*[C.addressof(d.inst) for d in devs])
         wh = (C.c_int32 * n)(*which)
@@ -180,6 +256,8 @@ class DeviceState:
             self.many_ws = self.torch.empty(max(need, 256), dtype=self.torch.uint8, device=self.device)
         _lib.check(self.lib.hs_recompute_fields_many(insts, wh, ptrs, sizes, n, self.ptr(self.many_ws),
                                                      self.many_ws.numel(), self.stream()))
+        for d, w in zip(devs, which):
+            d._shadow_stored(w)
 
     def bins(self, x_start, y_start, weights=None):
         """A readout's bins and weights, uploaded once (see Bins)."""

@@ -24,7 +24,8 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
            "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
            "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes",
-           "hs_metropolis_run_static", "hs_set_graph_steps", "hs_exchange_round_us")
+           "hs_metropolis_run_static", "hs_set_graph_steps", "hs_exchange_round_us", "hs_refresh_fields",
+           "hs_refresh_scratch_bytes")
 
 HS_METRO_STATIC_SCRATCH_BYTES = 4 << 20
 
@@ -36,7 +37,7 @@ class Instance(C.Structure):
         ("Ep", C.c_void_p), ("Epc", C.c_void_p), ("EB", C.c_void_p), ("beta", C.c_void_p), ("mu", C.c_void_p),
         ("mu_src", C.c_void_p), ("mu64", C.c_void_p), ("pw64", C.c_void_p), ("K32", C.c_void_p), ("Kpo32", C.c_void_p), ("K64", C.c_void_p), ("Kps64", C.c_void_p),
         ("zee_bx", C.c_void_p), ("zee_by", C.c_void_p), ("occ_list", C.c_void_p), ("beta64c", C.c_void_p),
-        ("E64", C.c_void_p),
+        ("E64", C.c_void_p), ("C64", C.c_void_p), ("m_ref", C.c_void_p),
         ("seed", C.c_uint64), ("zee_Bx", C.c_double), ("zee_By", C.c_double), ("dip_pf", C.c_double),
         ("J", C.c_double), ("m_perp_factor", C.c_double), ("beta64", C.c_double),
         ("nx", C.c_int32), ("ny", C.c_int32), ("ux", C.c_int32), ("uy", C.c_int32), ("n_sub", C.c_int32),
@@ -128,6 +129,8 @@ def load():
         "hs_build_dimonopolar_kernels": ([C.POINTER(KernelGeom), vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "hs_recompute_fields": ([C.POINTER(Instance), i32, vp, i64, vp], C.c_int),
         "hs_recompute_scratch_bytes": ([C.POINTER(Instance)], C.c_int64),
+        "hs_refresh_fields": ([C.POINTER(Instance), i32, vp, i64, vp], C.c_int),
+        "hs_refresh_scratch_bytes": ([C.POINTER(Instance)], C.c_int64),
         "hs_field_components": ([C.POINTER(Instance), vp, vp], C.c_int),
         "hs_energy_total": ([C.POINTER(Instance), C.POINTER(dbl), vp], C.c_int),
         "hs_zeeman_swap": ([C.POINTER(Instance), dbl, dbl, vp, vp, vp], C.c_int),

@@ -141,10 +141,12 @@ def enqueue_neel(items, attempt_freq=1e10, begin=False, free=None, full=None):
         dev.run_teams(teams, "neel")
     queued = []
     if full is not None:
-        queued = [m for (m, steps, *_), f in zip(items, full)
-                  if f and m.recompute_every and m._since_recompute + steps >= m.recompute_every]
+        due = [(m, steps) for (m, steps, *_), f in zip(items, full)
+               if f and m.recompute_every and m._since_recompute + steps >= m.recompute_every]
+        queued = [m for m, _ in due]
         if queued:
-            dev.recompute_many([m._dev for m in queued], [3 if m.USE_PERP_ENERGY else 1 for m in queued])
+            dev.recompute_many([m._dev for m in queued], [3 if m.USE_PERP_ENERGY else 1 for m in queued],
+                               [m._changed_bound(ahead=steps) for m, steps in due])
     return queued
 
 
@@ -216,16 +218,16 @@ def collect(mags, before=None, recomputed=()):
             m._metro_rate = max(1.0, (st.attempts - before[i]) / st.steps_done)
         m._since_recompute += int(st.steps_done)
         if id(m) in recomputed:
-            m._since_recompute = 0
-            m.recomputes += 1
+            m._recompute_done(st)
             continue
         if m.recompute_every and m._since_recompute >= m.recompute_every and m._incremental_exact():
             due.append(m)
-    if due:  # the periodic FP64 recomputes of all instances due, in one call
-        due[0]._dev.recompute_many([m._dev for m in due], [3 if m.USE_PERP_ENERGY else 1 for m in due])
+    if due:  # the periodic FP64 recomputes of all instances due, in one call (or refreshes)
+        stm = {id(m): st for m, st in zip(mags, sts)}
+        due[0]._dev.recompute_many([m._dev for m in due], [3 if m.USE_PERP_ENERGY else 1 for m in due],
+                                   [m._changed_bound(stm[id(m)]) for m in due])
         for m in due:
-            m._since_recompute = 0
-            m.recomputes += 1
+            m._recompute_done(stm[id(m)])
     return sts, errs
 
 

@@ -97,8 +97,11 @@ class Magnets(ABC):
       seed            Philox key of this instance (default: random); trajectories are a pure
                       function of (seed, state), so runs are reproducible.
       device          HIP device index (default 0).
-      recompute_every Neel switches (or Metropolis steps) between FP64 full recomputes of the
+      recompute_every Neel switches (or Metropolis steps) between FP64 recomputes of the
                       FP32 fields, which reset incremental drift (default 1024; None disables).
+                      Such a periodic recompute refreshes the fields from an FP64 shadow of the
+                      dipolar correlations, adding only the sources that changed since the last
+                      one, while few did (shadow_refresh = False: always a full recompute).
       fp64_fields     hold the combined field in FP64 and step it with FP64 increments, as the
                       reference's float64 arrays do (default False: FP32 + Kahan compensation).
                       Metropolis stepping on grids of at most 80x80 cells, with DipolarEnergy,
@@ -146,7 +149,9 @@ class Magnets(ABC):
         self._m_perp_factor = 0.0
         self._calc_r_cache = {}
         self._since_recompute = 0
-        self.recomputes = 0  # FP64 full recomputes run so far
+        self.recomputes = 0  # FP64 recomputes run so far (full, or refreshed from the shadow)
+        self.shadow_refresh = True  # periodic recomputes may refresh from the FP64 shadow
+        self._sw_at_recompute = None  # switch count at the last Metropolis recompute (None: unknown)
         self._sel_key = None
         self.initialize_m(self._get_groundstate() if pattern is None else pattern, update_energy=False)
         self.T = T
@@ -336,6 +341,7 @@ class Magnets(ABC):
         neel = np.isfinite(self._occupation_inf)
         flags = occ.astype(np.uint8) * _lib.HS_FLAG_OCC + neel.astype(np.uint8) * _lib.HS_FLAG_NEEL
         self._dev.upload(self._dev.flags, flags)
+        self._dev.invalidate_shadow()  # targets that were empty have no shadow entry
         occ_list = np.flatnonzero(np.isfinite(self._occupation_inf)).astype(np.int32)
         self._dev.upload(self._dev.occ_list[:occ_list.size], occ_list)
         self._occ_ref = np.array(self.occupation, copy=True)
@@ -597,6 +603,7 @@ class Magnets(ABC):
         (moment/d)^2 for DiMonopolarEnergy (energies.py:413-415, fixed when that energy is
         initialised)."""
         d = self._dev
+        d.invalidate_shadow()  # the shadow holds correlations of the old weights
         pair = self._pair_energy()
         if isinstance(pair, DiMonopolarEnergy) and hasattr(pair, "charge_sq"):
             q2 = np.ascontiguousarray(pair.charge_sq, dtype=np.float64)
@@ -748,6 +755,7 @@ class Magnets(ABC):
             I.dip_pf = 0.0
             I.K32 = I.K64 = I.Kpo32 = I.Kps64 = None
             I.kwx = I.kwy = I.n_sub = 0
+        I.C64, I.m_ref = d.shadow_ptrs(self.in_plane, enabled=bool(I.dip_pf))
         ex = self.get_energy('exchange', verbose=False)
         if ex is not None and ex.J != 0:
             nn = np.asarray(ex.local_interaction, dtype=float)
@@ -775,13 +783,35 @@ class Magnets(ABC):
                 d.zee_by = t.from_numpy(np.ascontiguousarray(by, dtype=np.float32).reshape(-1)).to(d.device)
                 I.zee_bx, I.zee_by = P(d.zee_bx), P(d.zee_by)
 
-    def _device_recompute(self):
-        """FP64 full recompute of the combined fields (Energy.update for every energy)."""
+    def _device_recompute(self, periodic=False, st=None):
+        """FP64 recompute of the combined fields (Energy.update for every energy): a full one, or,
+        for a periodic recompute (`st`: the current run state, if read already), a refresh from
+        the FP64 shadow when few spins changed since the last recompute."""
         if not hasattr(self, "_dev"):
             return
-        self._dev.recompute(3 if self.USE_PERP_ENERGY else 1)
+        self._dev.recompute(3 if self.USE_PERP_ENERGY else 1, self._changed_bound(st) if periodic else None)
+        self._recompute_done(st)
+
+    def _changed_bound(self, st=None, ahead=0):
+        """Upper bound on the cells whose spin changed since the last FP64 recompute: the switches
+        since then, plus `ahead` more in launches queued before the recompute; None when unknown.
+        It only chooses between a refresh from the shadow and a full recompute (both give the
+        fields of the current spins), so a host write to the spins or counters cannot make it wrong."""
+        if not self.shadow_refresh or self._dev.C64 is None:
+            return None
+        if self.params.UPDATE_SCHEME == Scheme.NEEL:
+            return self._since_recompute + ahead  # one switch per Neel step
+        if self._sw_at_recompute is None:
+            return None
+        sw = int((self._state() if st is None else st).switches)
+        return sw - self._sw_at_recompute + ahead if sw >= self._sw_at_recompute else None
+
+    def _recompute_done(self, st=None):
+        """Counters after a recompute (`st`: the run state it ran at, if read already)."""
         self._since_recompute = 0
         self.recomputes += 1
+        if self.shadow_refresh and self._dev.C64 is not None and self.params.UPDATE_SCHEME != Scheme.NEEL:
+            self._sw_at_recompute = int((self._state() if st is None else st).switches)
 
     def _device_zeeman_changed(self, z: ZeemanEnergy, old):
         """ZeemanEnergy.set_field: swap the Zeeman term of the combined device field."""
@@ -932,7 +962,7 @@ class Magnets(ABC):
         c = cells[:a].cpu().numpy().astype(np.int64)
         # the periodic FP64 recompute counts update() steps as progress() counts its steps
         self._since_recompute += 1
-        self._maybe_recompute()
+        self._maybe_recompute(st)
         return np.asarray([c // self.nx, c % self.nx])
 
     def _wolff_check(self):
@@ -962,9 +992,9 @@ class Magnets(ABC):
         c = cells[:k].cpu().numpy().astype(np.int64)
         return np.asarray([c // self.nx, c % self.nx])
 
-    def _maybe_recompute(self):
+    def _maybe_recompute(self, st=None):
         if self.recompute_every and self._since_recompute >= self.recompute_every and self._incremental_exact():
-            self._device_recompute()
+            self._device_recompute(periodic=True, st=st)
 
     def _incremental_exact(self):
         """True when incremental updates reproduce a full recompute (no kernel truncation)."""
@@ -1086,7 +1116,7 @@ class Magnets(ABC):
                 if st.err:
                     raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
                 self._since_recompute += int(st.steps_done)
-                self._maybe_recompute()
+                self._maybe_recompute(st)
         else:
             r = kwargs.get('r', 0)
             Q = kwargs.get('Q', 0.05)
@@ -1124,7 +1154,7 @@ class Magnets(ABC):
                 if st.steps_done > 0:  # samples per step, to size the next launch
                     self._metro_rate = max(1.0, (st.attempts - a_before) / st.steps_done)
                 self._since_recompute += int(st.steps_done)
-                self._maybe_recompute()
+                self._maybe_recompute(st)
         self._after_device_step()
         st = self._state()
         return st.t - t0, st.attempts / self.n - a0 / self.n
@@ -1155,7 +1185,7 @@ class Magnets(ABC):
             if st.err:
                 raise _lib.HotspiceError(f"device Wolff step failed (err={st.err})")
             self._since_recompute += int(st.steps_done)
-            self._maybe_recompute()
+            self._maybe_recompute(st)
         self._after_device_step()
         st = self._state()
         return st.t - t0, st.attempts / self.n - a0 / self.n

@@ -701,6 +701,11 @@ __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const 
       }
       e += (I.dip_pf * mu) * (mi * acc);
       ep += (I.dip_pf * mu) * (mi * accp);
+      // the shadow of the raw correlations, when this pass has both components
+      if (I.C64 && F > 0 && do_e && (do_p || !I.use_perp)) {
+        I.C64[c] = acc;
+        if (do_p) I.C64[(size_t)ncell + c] = accp;
+      }
     }
     if (I.has_zeeman) {
       double Bx, By, f, fp;
@@ -714,6 +719,7 @@ __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const 
     e += xe;
     ep += xp;
   }
+  if (I.m_ref && F > 0 && do_e && (do_p || !I.use_perp)) I.m_ref[c] = I.m[c];
   if (do_e && I.E64) {
     I.E64[c] = e;
   } else if (do_e) {
@@ -730,6 +736,163 @@ __device__ __forceinline__ void rec_finish(const hs_instance_t &I, int F, const 
 
 __global__ void k_recompute_finish(hs_instance_t I, int F, const double *partE, const double *partP, int which) {
   rec_finish(I, F, partE, partP, which, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- refresh from the FP64 shadow (hs_refresh_fields) -------------------------------------------
+// The raw correlations C_i = sum_j K_n(i)[j - i] (m w)_j are linear in the sources, so the shadow of
+// the last full recompute plus the change of the sources since then, (m_j - m_ref_j) w_j over their
+// windows, is the recompute of the current spins (up to FP64 rounding). The changed cells are
+// listed in cell order (row counts, then an ordered compaction), so the additions to each
+// target run in a fixed order and the result does not depend on scheduling.
+constexpr int DIFF_THREADS = 256;
+
+// One workgroup per grid row: the number of cells of the row whose spin differs from m_ref.
+__global__ void __launch_bounds__(DIFF_THREADS) k_diff_count(hs_instance_t I, int32_t *row_count) {
+  __shared__ int s[DIFF_THREADS / 64];
+  const int y = blockIdx.x;
+  int v = 0;
+  for (int x = threadIdx.x; x < I.nx; x += DIFF_THREADS) v += I.m[y * I.nx + x] != I.m_ref[y * I.nx + x];
+  v = hs_wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < DIFF_THREADS / 64; ++w) t += s[w];
+    row_count[y] = t;
+  }
+}
+
+// Row y writes its changed cells after those of the rows before it, in cell order, with their
+// source change (m - m_ref) w, takes m_ref := m for them and records where it starts (row_off[y]);
+// the last row also writes the end (row_off[ny]).
+__global__ void __launch_bounds__(DIFF_THREADS) k_diff_list(hs_instance_t I, const int32_t *row_count,
+                                                            int32_t *row_off, int32_t *list_cell, double *list_w) {
+  __shared__ int s[DIFF_THREADS / 64];
+  __shared__ int s_off;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, y = blockIdx.x;
+  int before = 0;
+  for (int r = tid; r < y; r += DIFF_THREADS) before += row_count[r];
+  before = hs_wave_sum(before);
+  if (lane == 0) s[wv] = before;
+  __syncthreads();
+  if (tid == 0) {
+    int t = 0;
+    for (int w = 0; w < DIFF_THREADS / 64; ++w) t += s[w];
+    s_off = t;
+    row_off[y] = t;
+    if (y == I.ny - 1) row_off[I.ny] = t + row_count[y];
+  }
+  for (int x0 = 0; x0 < I.nx; x0 += DIFF_THREADS) {
+    __syncthreads();  // s and s_off of the previous chunk are read by every thread
+    const int x = x0 + tid, c = y * I.nx + x;
+    const bool ch = x < I.nx && I.m[c] != I.m_ref[c];
+    const unsigned long long ball = __ballot(ch);
+    if (lane == 0) s[wv] = __popcll(ball);
+    __syncthreads();
+    int off = s_off, total = 0;
+    for (int w = 0; w < DIFF_THREADS / 64; ++w) {
+      off += w < wv ? s[w] : 0;
+      total += s[w];
+    }
+    if (ch) {
+      const int pos = off + __popcll(ball & ((1ull << lane) - 1ull));
+      const int m = I.m[c];
+      const double w = I.pair_src_unit ? 1.0 : I.pw64[c];
+      list_cell[pos] = c;
+      list_w[pos] = (double)(m - I.m_ref[c]) * w;
+      I.m_ref[c] = (int8_t)m;
+    }
+    __syncthreads();
+    if (tid == 0) s_off += total;
+  }
+}
+
+// One thread per target cell (32 x 8 tiles). The listed sources in the rows the tile's window
+// reaches (a contiguous part of the list, from row_off) are staged in LDS chunk by chunk and add
+// K_n(target)[source - target] * change to the shadow (and K_perpself for E_perp), in list order.
+__global__ void __launch_bounds__(256) k_shadow_add(hs_instance_t I, const int32_t *row_off, const int32_t *list_cell,
+                                                    const double *list_w, int do_p) {
+  __shared__ int s_c[256];
+  __shared__ double s_w[256];
+  const int ntx = (I.nx + 31) / 32;
+  const int x0 = (blockIdx.x % ntx) * 32, y0 = (blockIdx.x / ntx) * 8;
+  const int x = x0 + (threadIdx.x & 31), y = y0 + (threadIdx.x >> 5);
+  const bool inside = x < I.nx && y < I.ny;
+  const int c = inside ? y * I.nx + x : 0;
+  const int n = inside ? I.sub_of[hs_site(I, y, x)] : -1;
+  const int lo = row_off[max(0, y0 - I.kwy)], hi = row_off[min(I.ny, y0 + 8 + I.kwy)];
+  const int KW = 2 * I.kwx + 1;
+  const size_t KS = (size_t)KW * (2 * I.kwy + 1);
+  const double *Kn = I.K64 + (size_t)(n >= 0 ? n : 0) * KS;
+  const double *Kp = I.Kps64 ? I.Kps64 + (size_t)(n >= 0 ? n : 0) * KS : nullptr;
+  double acc = 0.0, accp = 0.0;
+  for (int base = lo; base < hi; base += 256) {
+    __syncthreads();
+    if (base + (int)threadIdx.x < hi) {
+      s_c[threadIdx.x] = list_cell[base + threadIdx.x];
+      s_w[threadIdx.x] = list_w[base + threadIdx.x];
+    }
+    __syncthreads();
+    const int cnt = min(256, hi - base);
+    if (n < 0) continue;
+    for (int r = 0; r < cnt; ++r) {
+      const int sc = s_c[r];
+      const int dy = sc / I.nx - y, dx = sc % I.nx - x;
+      if (dy < -I.kwy || dy > I.kwy || dx < -I.kwx || dx > I.kwx) continue;
+      const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
+      acc = fma(Kn[o], s_w[r], acc);
+      if (do_p) accp = fma(Kp[o], s_w[r], accp);
+    }
+  }
+  if (n < 0 || hi == lo) return;
+  I.C64[c] += acc;
+  if (do_p) I.C64[(size_t)I.nx * I.ny + c] += accp;
+}
+
+extern "C" int64_t hs_refresh_scratch_bytes(const hs_instance_t *inst) {
+  if (!inst || inst->nx <= 0 || inst->ny <= 0) return -1;
+  const int64_t ncell = (int64_t)inst->nx * inst->ny;
+  return 8 * ncell + 4 * ncell + 4 * (int64_t)inst->ny + 4 * ((int64_t)inst->ny + 1);
+}
+
+extern "C" int hs_refresh_fields(const hs_instance_t *inst, int32_t which, void *scratch, int64_t scratch_bytes,
+                                 void *stream) {
+  hs_clear_error();
+  int rc = hs_check_instance(inst);
+  if (rc) return rc;
+  if (!inst->C64 || !inst->m_ref) return hs_set_error(HS_ERR_ARG, "no FP64 shadow (C64, m_ref) to refresh from");
+  const bool do_p = (which & 2) && inst->use_perp;
+  if (!(which & 1) || (inst->use_perp && !do_p))
+    return hs_set_error(HS_ERR_ARG, "a refresh covers every field component the shadow holds (which = %s)",
+                        inst->use_perp ? "3" : "1 or 3");
+  hipStream_t s = (hipStream_t)stream;
+  const int ncell = inst->nx * inst->ny;
+  if (inst->dip_pf != 0.0) {
+    const int64_t need = hs_refresh_scratch_bytes(inst);
+    if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 7))
+      return hs_set_error(HS_ERR_ARG, "refresh scratch of %lld bytes (8-byte aligned) is below the %lld needed",
+                          (long long)scratch_bytes, (long long)need);
+    if (do_p && !inst->Kps64) return hs_set_error(HS_ERR_ARG, "E_perp refresh without K_perpself");
+    char *p = (char *)scratch;
+    double *list_w = (double *)p;
+    p += 8 * (size_t)ncell;
+    int32_t *list_cell = (int32_t *)p;
+    p += 4 * (size_t)ncell;
+    int32_t *row_count = (int32_t *)p;
+    p += 4 * (size_t)inst->ny;
+    int32_t *row_off = (int32_t *)p;
+    k_diff_count<<<inst->ny, DIFF_THREADS, 0, s>>>(*inst, row_count);
+    HS_LAUNCH_CHECK("k_diff_count");
+    k_diff_list<<<inst->ny, DIFF_THREADS, 0, s>>>(*inst, row_count, row_off, list_cell, list_w);
+    HS_LAUNCH_CHECK("k_diff_list");
+    const unsigned tiles = (unsigned)(((inst->nx + 31) / 32) * ((inst->ny + 7) / 8));
+    k_shadow_add<<<tiles, 256, 0, s>>>(*inst, row_off, list_cell, list_w, do_p ? 1 : 0);
+    HS_LAUNCH_CHECK("k_shadow_add");
+  }
+  const double *pe = inst->C64, *pp = do_p ? inst->C64 + (size_t)ncell : nullptr;
+  k_recompute_finish<<<(ncell + 255) / 256, 256, 0, s>>>(*inst, inst->dip_pf != 0.0 ? 1 : 0, pe, pp, which);
+  HS_LAUNCH_CHECK("k_recompute_finish");
+  return HS_OK;
 }
 
 // Small grids (at most SMALL_REC_CELLS cells): one thread per target cell sums

@@ -74,6 +74,11 @@ typedef struct hs_instance {
                            of at most 80x80 cells update it with FP64 increments from K64 and mu64
                            (DipolarEnergy / ExchangeEnergy / ZeemanEnergy, no E_perp). Every other
                            stepping entry point refuses such an instance with HS_ERR_ARG. */
+  double *C64;          /* (2, ny, nx) FP64 shadow of the raw dipolar correlations of the last full
+                           recompute, sum_j K_n(i)[j - i] (m w)_j for E and (K_perpself) for E_perp,
+                           or NULL; a recompute of both components (which = 3, or 1 without E_perp)
+                           stores it, hs_refresh_fields updates it */
+  int8_t *m_ref;        /* (ny,nx) the spins C64 was computed for, or NULL (set with C64) */
   uint64_t seed;        /* Philox key */
   double zee_Bx, zee_By; /* uniform field components when zee_bx/zee_by are NULL */
   double dip_pf;        /* DipolarEnergy.prefactor (0 = no dipolar energy) */
@@ -263,6 +268,22 @@ int hs_recompute_fields(const hs_instance_t *inst, int32_t which, void *scratch,
 
 /* Scratch bytes hs_recompute_fields needs for this instance. */
 int64_t hs_recompute_scratch_bytes(const hs_instance_t *inst);
+
+/*
+ * The FP64 recompute from the shadow (inst->C64, inst->m_ref, stored by the last full
+ * hs_recompute_fields of both components): the cells whose spin differs from m_ref are listed in
+ * cell order (per-row counts and an ordered compaction), their change (m - m_ref) w is added to the shadow
+ * over their windows in that order, m_ref takes the spins, and the fields are finished from the
+ * shadow as hs_recompute_fields finishes them (Zeeman and exchange closed forms included). Equal
+ * to a full recompute up to FP64 rounding, at a cost of (changed cells x window) instead of
+ * (cells x window): DipolarEnergy.update energies.py:290-308 by linearity in the sources. The
+ * caller keeps the shadow valid: after any change of kernels, weights or occupation, a full
+ * recompute comes first. scratch: hs_refresh_scratch_bytes bytes. Non-blocking.
+ */
+int hs_refresh_fields(const hs_instance_t *inst, int32_t which, void *scratch, int64_t scratch_bytes, void *stream);
+
+/* Scratch bytes hs_refresh_fields needs for this instance (the ordered list of changed cells). */
+int64_t hs_refresh_scratch_bytes(const hs_instance_t *inst);
 
 /*
  * hs_recompute_fields for n instances at once (one launch for all dipolar correlations, one for
