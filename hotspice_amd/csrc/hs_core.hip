@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(DIFF_THREADS) k_diff_list(hs_instance_t I, con
       const int pos = off + __popcll(ball & ((1ull << lane) - 1ull));
       const int m = I.m[c];
       const double w = I.pair_src_unit ? 1.0 : I.pw64[c];
-      list_cell[pos] = c;
+      list_cell[pos] = (y << 16) | x;  // packed (y, x): no division in the additions
       list_w[pos] = (double)(m - I.m_ref[c]) * w;
       I.m_ref[c] = (int8_t)m;
     }
@@ -810,6 +810,8 @@ __global__ void __launch_bounds__(DIFF_THREADS) k_diff_list(hs_instance_t I, con
 // One thread per target cell (32 x 8 tiles). The listed sources in the rows the tile's window
 // reaches (a contiguous part of the list, from row_off) are staged in LDS chunk by chunk and add
 // K_n(target)[source - target] * change to the shadow (and K_perpself for E_perp), in list order.
+// A source outside a target's window adds an exact zero (its load is masked), so the loop has no
+// branch and its kernel loads can be issued several sources ahead.
 __global__ void __launch_bounds__(256) k_shadow_add(hs_instance_t I, const int32_t *row_off, const int32_t *list_cell,
                                                     const double *list_w, int do_p) {
   __shared__ int s_c[256];
@@ -835,13 +837,15 @@ __global__ void __launch_bounds__(256) k_shadow_add(hs_instance_t I, const int32
     __syncthreads();
     const int cnt = min(256, hi - base);
     if (n < 0) continue;
+#pragma unroll 4
     for (int r = 0; r < cnt; ++r) {
-      const int sc = s_c[r];
-      const int dy = sc / I.nx - y, dx = sc % I.nx - x;
-      if (dy < -I.kwy || dy > I.kwy || dx < -I.kwx || dx > I.kwx) continue;
-      const size_t o = (size_t)(I.kwy + dy) * KW + (I.kwx + dx);
-      acc = fma(Kn[o], s_w[r], acc);
-      if (do_p) accp = fma(Kp[o], s_w[r], accp);
+      const int v = s_c[r];
+      const int dy = (v >> 16) - y, dx = (v & 0xffff) - x;
+      const bool in = dy >= -I.kwy && dy <= I.kwy && dx >= -I.kwx && dx <= I.kwx;
+      const int o = in ? (I.kwy + dy) * KW + (I.kwx + dx) : 0;
+      const double w = in ? s_w[r] : 0.0;
+      acc = fma(Kn[o], w, acc);
+      if (do_p) accp = fma(Kp[o], w, accp);
     }
   }
   if (n < 0 || hi == lo) return;
@@ -861,6 +865,7 @@ extern "C" int hs_refresh_fields(const hs_instance_t *inst, int32_t which, void 
   int rc = hs_check_instance(inst);
   if (rc) return rc;
   if (!inst->C64 || !inst->m_ref) return hs_set_error(HS_ERR_ARG, "no FP64 shadow (C64, m_ref) to refresh from");
+  if (inst->nx > 32767 || inst->ny > 32767) return hs_set_error(HS_ERR_ARG, "refresh of a grid side above 32767");
   const bool do_p = (which & 2) && inst->use_perp;
   if (!(which & 1) || (inst->use_perp && !do_p))
     return hs_set_error(HS_ERR_ARG, "a refresh covers every field component the shadow holds (which = %s)",
