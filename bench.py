@@ -412,6 +412,9 @@ def bench_batched(args):
         graph_rate = (sum(m.attempted_switches for m in mags) - g0) / (time.perf_counter() - tg)
     finally:
         _lib.set_graph_steps(old)
+        # live executable graphs slow every later plain launch of the process (ROCm 7.2, MI355X:
+        # the served mix measured after them ran at 60 % of its rate), so none outlives this line
+        _lib.clear_graph_cache()
     err = max(rel_field_error(m) for m in mags[::8])
     # plain path: the same instances (fresh copies) stepped one at a time, one sweep each
     plain = [ipsq_batch_instance(s) for s in specs]

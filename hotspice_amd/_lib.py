@@ -24,7 +24,7 @@ EXPORTS = ("hs_version", "hs_last_error", "hs_abi_layout", "hs_device_info", "hs
            "hs_metro_run_teams", "hs_metro_teams_ws_bytes", "hs_recompute_fields_many", "hs_recompute_many_ws_bytes",
            "hs_binned_sums", "hs_binned_sums_scratch_bytes", "hs_relax_sweep", "hs_relax_ws_bytes",
            "hs_minimize_scan", "hs_flip_cells", "hs_flip_ws_bytes", "hs_wolff_run", "hs_wolff_ws_bytes",
-           "hs_metropolis_run_static", "hs_set_graph_steps", "hs_exchange_round_us", "hs_refresh_fields",
+           "hs_metropolis_run_static", "hs_set_graph_steps", "hs_graph_cache_clear", "hs_exchange_round_us", "hs_refresh_fields",
            "hs_refresh_scratch_bytes")
 
 HS_METRO_STATIC_SCRATCH_BYTES = 4 << 20
@@ -142,6 +142,7 @@ def load():
         "hs_neel_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
         "hs_metro_run_teams": ([C.POINTER(Team), i32, vp, i64, vp], C.c_int),
         "hs_set_graph_steps": ([i32], C.c_int),
+        "hs_graph_cache_clear": ([], C.c_int),
         "hs_exchange_round_us": ([i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
         "hs_metro_teams_ws_bytes": ([C.POINTER(Team), i32], C.c_int64),
         "hs_recompute_fields_many": ([C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), C.POINTER(i64), i32, vp, i64, vp],
@@ -198,6 +199,11 @@ def set_graph_steps(steps):
     """Steps of the batched Metropolis loop captured per hipGraph (0: plain launches); returns the
     previous setting. Results are bit-identical either way."""
     return int(load().hs_set_graph_steps(int(steps)))
+
+
+def clear_graph_cache():
+    """Destroys the cached step graphs (hs_graph_cache_clear); later graph-mode calls capture again."""
+    check(load().hs_graph_cache_clear())
 
 
 def exchange_round_us(G, rounds=20000):

@@ -151,6 +151,26 @@ int hs_graph_launch(const std::vector<int64_t> &key, hipStream_t s, const std::f
   return HS_OK;
 }
 
+extern "C" int hs_graph_cache_clear(void) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  int cur = 0;
+  HS_HIP_TRY(hipGetDevice(&cur));
+  for (GraphEntry &e : g_graphs) {  // once its device has drained, nothing of the graph runs any more
+    HS_HIP_TRY(hipSetDevice(e.device));
+    HS_HIP_TRY(hipDeviceSynchronize());
+    HS_HIP_TRY(hipGraphExecDestroy(e.exec));
+  }
+  g_graphs.clear();
+  for (int d = 0; d < GRAPH_DEVICES; ++d)
+    if (g_capture_stream[d]) {
+      HS_HIP_TRY(hipSetDevice(d));
+      HS_HIP_TRY(hipStreamDestroy(g_capture_stream[d]));
+      g_capture_stream[d] = nullptr;
+    }
+  HS_HIP_TRY(hipSetDevice(cur));
+  return HS_OK;
+}
+
 extern "C" int hs_version(void) { return 100; }
 
 extern "C" const char *hs_last_error(void) { return g_err.c_str(); }

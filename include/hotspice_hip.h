@@ -488,6 +488,15 @@ int hs_exchange_round_us(int32_t G, int32_t rounds, double *us_per_round, int32_
  */
 int hs_set_graph_steps(int32_t steps);
 
+/*
+ * Destroys every cached step graph and the private capture streams (after draining the devices
+ * they ran on); later graph-mode calls capture again. Measured on MI355X (ROCm 7.2): while
+ * executable graphs are alive, plain kernel launches of the process run slower, so a caller that
+ * replays graphs in one phase and launches plainly in the next clears the cache in between.
+ * Blocking. No counterpart in the reference (launch mechanics of the device loop).
+ */
+int hs_graph_cache_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -244,7 +244,8 @@ def test_metropolis_graph_replay_equals_plain_launches(names):
     """The batched Metropolis loop replays a captured hipGraph per group of 4 steps (two replays
     and two plain steps per call, the graph captured on the first call and reused after it, with
     the host reading the states and running recomputes between calls); after every call each
-    instance is bit-identical to the same batch stepped with plain launches only."""
+    instance is bit-identical to the same batch stepped with plain launches only. Halfway the
+    graph cache is cleared (hs_graph_cache_clear), so the next call captures again."""
     from cases import spec
     from hotspice_amd import _lib, batch
     kws = [spec(n)["update_kw"] for n in names]
@@ -260,6 +261,8 @@ def test_metropolis_graph_replay_equals_plain_launches(names):
     old = _lib.set_graph_steps(4)
     try:
         for call in range(4):
+            if call == 2:
+                _lib.clear_graph_cache()
             _lib.set_graph_steps(0)
             batch.launch_metropolis([(m, s, 10, math.inf, math.inf) for m, s in zip(plain, sp)], begin=call == 0)
             _lib.set_graph_steps(4)
@@ -271,6 +274,7 @@ def test_metropolis_graph_replay_equals_plain_launches(names):
             assert all(m.attempted_switches > 0 for m in graphed)
     finally:
         _lib.set_graph_steps(old)
+        _lib.clear_graph_cache()  # live graphs slow the plain launches of later tests
 
 
 def test_varying_temperature_instance_in_a_neel_batch():
