@@ -66,6 +66,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-shadow-refresh", dest="shadow_refresh", action="store_false",
+                   help="every periodic FP64 recompute is a full one (for comparison)")
     p.add_argument("--jobs", type=int, default=32, help="served: number of jobs")
     p.add_argument("--window", type=int, default=None,
                    help="served: dipolar cutoff in cell pitches with the matching REDUCED_KERNEL_SIZE (default: the "
@@ -399,22 +401,28 @@ def bench_batched(args):
     attempts = sum(m.attempted_switches for m in mags) - a0
     accepted = sum(m.switches for m in mags) - s0
     launches, dev.launch_events = dev.launch_events, None
-    # the same sweeps again with the step loop replayed from captured hipGraphs of 16 steps
+    # the same sweeps of a second copy of the batch (same seeds, so the same trajectories: the
+    # acceptance rate, and with it the update work per attempt, falls as the spins relax) with the
+    # step loop replayed from captured hipGraphs of 16 steps
     from hotspice_amd import _lib
-    g0 = sum(m.attempted_switches for m in mags)
+    twin = [ipsq_batch_instance(s) for s in specs]
+    batch.progress_many(twin, t_max=math.inf, MCsteps_max=0.5)
+    g0 = sum(m.attempted_switches for m in twin)
     old = _lib.set_graph_steps(16)
     try:
         torch.cuda.synchronize()
         tg = time.perf_counter()
         for _ in range(sweeps):
-            batch.progress_many(mags, t_max=math.inf, MCsteps_max=1.0)
+            batch.progress_many(twin, t_max=math.inf, MCsteps_max=1.0)
         torch.cuda.synchronize()
-        graph_rate = (sum(m.attempted_switches for m in mags) - g0) / (time.perf_counter() - tg)
+        graph_rate = (sum(m.attempted_switches for m in twin) - g0) / (time.perf_counter() - tg)
     finally:
         _lib.set_graph_steps(old)
         # live executable graphs slow every later plain launch of the process (ROCm 7.2, MI355X:
         # the served mix measured after them ran at 60 % of its rate), so none outlives this line
         _lib.clear_graph_cache()
+    twin_same = all(np.array_equal(a.m, b.m) for a, b in zip(mags[::8], twin[::8]))
+    del twin
     err = max(rel_field_error(m) for m in mags[::8])
     # plain path: the same instances (fresh copies) stepped one at a time, one sweep each
     plain = [ipsq_batch_instance(s) for s in specs]
@@ -438,6 +446,7 @@ def bench_batched(args):
                       "instances": len(mags), "magnets": int(sum(m.n for m in mags)), "attempts": attempts,
                       "accepted": accepted, "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
                       "vs_plain": attempts / wall / plain_rate, "graph_replay_attempts_per_s": graph_rate,
+                      "graph_replay_same_spins": twin_same,
                       "parallelism": "single GPU"})
     line["roofline"] = roofline("k_metro_select_many + k_metro_update_many (per batched call)", launches, bytes_total)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline_batched(specs, args.cpu_seconds)
@@ -464,6 +473,9 @@ def bench_neel_batched(args):
     wall = time.perf_counter() - t0
     attempts = sum(m.attempted_switches for m in mags) - a0
     launches, dev.launch_events = dev.launch_events, None
+    # error of the incremental fields at the end of a recompute period (PERIOD-1 switches after
+    # the recompute each timed call ended with)
+    batch.progress_many(mags[::32], t_max=math.inf, MCsteps_max=(PERIOD - 1) / n)
     err = max(rel_field_error(m) for m in mags[::32])
     one = mags[0]
     torch.cuda.synchronize()
@@ -479,8 +491,7 @@ def bench_neel_batched(args):
                                   "the 256x256 pinwheel workload, one persistent launch, FP64 recompute every 1024 "
                                   "switches per instance", "instances": count, "attempts": attempts,
                       "rel_E_err_vs_fp64": err, "plain_attempts_per_s": plain_rate,
-                      "vs_plain": attempts / wall / plain_rate, "graph_replay_attempts_per_s": graph_rate,
-                      "parallelism": "single GPU"})
+                      "vs_plain": attempts / wall / plain_rate, "parallelism": "single GPU"})
     line["roofline"] = roofline("k_neel_team (256 teams of 1 workgroup)", launches, bytes_total)
     return line
 
@@ -680,7 +691,13 @@ def main():
     fn = {"all": bench_all, "neel": bench_neel, "metro64": lambda a: bench_metro(a, False),
           "metro2048": lambda a: bench_metro(a, True), "batched": bench_batched, "neel_batched": bench_neel_batched,
           "served": bench_served}[args.workload]
-    print(json.dumps(fn(args)), flush=True)
+    if not args.shadow_refresh:
+        from hotspice_amd.core import Magnets
+        Magnets.shadow_refresh = False
+    line = fn(args)
+    if not args.shadow_refresh:
+        line["config"]["shadow_refresh"] = False
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -108,6 +108,8 @@ class Magnets(ABC):
                       ZeemanEnergy and ExchangeEnergy and no E_perp; other stepping raises.
     """
 
+    shadow_refresh = True  # periodic recomputes may refresh from the FP64 shadow (per instance or class)
+
     def __init__(self, nx: int, ny: int, dx, dy, *, T=300, E_B=0., moment=None, Msat=800e3, V=2e-22,
                  pattern: str = None, energies=None, PBC: bool = False, m_perp_factor: float = None,
                  E_B_std: float = None, major_axis: float = 0, minor_axis: float = None, angle: float = 0.,
@@ -150,7 +152,6 @@ class Magnets(ABC):
         self._calc_r_cache = {}
         self._since_recompute = 0
         self.recomputes = 0  # FP64 recomputes run so far (full, or refreshed from the shadow)
-        self.shadow_refresh = True  # periodic recomputes may refresh from the FP64 shadow
         self._sw_at_recompute = None  # switch count at the last Metropolis recompute (None: unknown)
         self._sel_key = None
         self.initialize_m(self._get_groundstate() if pattern is None else pattern, update_energy=False)
