@@ -175,3 +175,38 @@ def test_periodic_recomputes_refresh_in_progress():
     E_f, P_f = _fields(mm)
     assert float(np.max(np.abs(E_i - E_f))) < 1e-5 * float(np.max(np.abs(E_f)))
     assert float(np.max(np.abs(P_i - P_f))) < 1e-5 * float(np.max(np.abs(P_f)))
+
+
+def _pin64(seed, every):
+    import hotspice_amd as hs
+    n = 64
+    return hs.ASI.IP_Pinwheel(1e-6, n, T=300, moment=2.37e-16, seed=seed, recompute_every=every,
+                              energies=[hs.DipolarEnergy(), hs.ZeemanEnergy(1e-3, math.pi / 8)],
+                              params=hs.SimParams(UPDATE_SCHEME='Néel'), **_inputs((n, n), seed, EB_kT=40))
+
+
+def _dump(mm):
+    d = mm._dev
+    return [t.cpu().numpy().copy() for t in (d.m, d.E, d.Ec, d.Ep, d.Epc)] + [mm.t, mm.switches]
+
+
+def test_batched_refreshes_equal_alone():
+    """Neel instances stepped in one batched launch, with their periodic recomputes queued behind
+    it (batch.progress_many), take the same refresh-or-full decisions as each stepped alone
+    through progress(), so both end bit-identical; refreshes do happen."""
+    from hotspice_amd import batch
+    seeds = [21, 22, 23]
+    alone = [_pin64(s, 96) for s in seeds]
+    together = [_pin64(s, 96) for s in seeds]
+    for _ in range(4):
+        for m in alone:
+            m.progress(t_max=math.inf, MCsteps_max=300 / m.n)
+        batch.progress_many(together, t_max=math.inf, MCsteps_max=300 / together[0].n)
+    for a, b in zip(alone, together):
+        assert a._dev.refreshes > 0 and a._dev.refreshes == b._dev.refreshes
+        assert a.recomputes == b.recomputes
+        for x, y in zip(_dump(a), _dump(b)):
+            if isinstance(x, np.ndarray):
+                np.testing.assert_array_equal(x, y)
+            else:
+                assert x == y
