@@ -41,10 +41,11 @@ constexpr int MAX_ATTEMPTS = 4096;
 // instances up to this many grid cells step in the persistent one-workgroup kernel
 constexpr int PERSIST_CELLS = 80 * 80;
 // workgroups of an update launch over the worklist when the caller leaves it automatic
-// (hs_run_t.nblocks = 0; 8 per CU of a 256-CU device), and the waves per SIMD the update kernels
-// are compiled for, so that all of them are resident at once
-constexpr int UPDATE_WGS = 2048;
-constexpr int UPDATE_WAVES = 8;
+// (hs_run_t.nblocks = 0; 6 per CU of a 256-CU device), and the waves per SIMD the update kernels
+// are compiled for, so that all of them are resident at once (6: 80 VGPRs, no spills; measured
+// faster than 8 waves with spills, 5 or 4 waves)
+constexpr int UPDATE_WGS = 1536;
+constexpr int UPDATE_WAVES = 6;
 
 // Workgroups of an update launch over `ntiles` tiles: `requested` (hs_run_t.nblocks) when
 // positive, else UPDATE_WGS; never more than one per tile.
@@ -815,6 +816,11 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   __shared__ int s_sub[HS_MAX_UC], s_trunc[HS_MAX_UC];
   __shared__ float s_oxu[HS_MAX_UC], s_oyu[HS_MAX_UC], s_nn[HS_MAX_NN];
   __shared__ double s_mmu64[F64 ? MU_THREADS : 1];  // F64: the sources' (m mu64) after their flips
+  // decoded per staged source once, for the whole tile (grids without PBC): (sy << 16) | sx, the
+  // kernel offset n*KS + (kwy - sy)*KW + (kwx - sx) (target (y, x) reads entry offset + y*KW + x),
+  // and the window half-widths wy | wx << 16 (-1: no dipolar term)
+  __shared__ int s_yx[MU_THREADS], s_win[MU_THREADS];
+  __shared__ long long s_koff[MU_THREADS];
   const int tid = threadIdx.x;
   const int x0 = tile_x * MU_TX, y0 = tile_y * MU_TY;
   const int x = x0 + (tid % MU_TX), y = y0 + (tid / MU_TX);
@@ -865,6 +871,11 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   const int KW = 2 * I.kwx + 1;
   const size_t KS = (size_t)KW * (2 * I.kwy + 1);
   const int tkx = min(min(SEL.reduced_k, nx - 1), I.kwx), tky = min(min(SEL.reduced_k, ny - 1), I.kwy);
+  // the sources' coordinates, kernel offsets and windows are decoded once per staged source (no
+  // wrapped images without PBC; coordinates packed in 16 bits)
+  const bool decoded = !I.pbc && nx <= 32767 && ny <= 32767;
+  const long long cellk = (long long)y * KW + x;
+  const int yx_self = (y << 16) | x;
   FT oxi = 0, oyi = 0;
   FT acc = 0, ex = 0;
   float accp = 0.f, exp_ = 0.f;
@@ -951,7 +962,7 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
   for (int base = 0; base < nsrc; base += MU_THREADS) {
     const int sidx = base + tid;
     bool rel = false;
-    int sc = 0, ssite = 0;
+    int sc = 0, ssite = 0, sy = 0, sx = 0;
     float smmu = 0.f;
     if (sidx < nsrc) {
       bool have;
@@ -970,7 +981,8 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
         smmu = ws.acc_mmu[sidx];
       }
       if (have) {
-        const int sy = sc / nx, sx = sc % nx;
+        sy = sc / nx;
+        sx = sc % nx;
         ssite = hs_site(I, sy, sx);
         if (has_dip && sub_of(ssite) >= 0) {
           const bool tr = trunc_of(ssite) != 0;
@@ -1000,6 +1012,19 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       s_mmu[p] = smmu;
       s_site_of[p] = ssite;
       if (F64) s_mmu64[p] = (smmu > 0.f ? 1.0 : -1.0) * I.mu64[sc];
+      if (decoded) {
+        int win = -1;
+        long long ko = 0;
+        const int n = has_dip ? sub_of(ssite) : -1;
+        if (n >= 0) {
+          const bool tr = trunc_of(ssite) != 0;
+          win = (tr ? tky : I.kwy) | ((tr ? tkx : I.kwx) << 16);
+          ko = (long long)n * (long long)KS + (long long)(I.kwy - sy) * KW + (I.kwx - sx);
+        }
+        s_yx[p] = (sy << 16) | sx;
+        s_win[p] = win;
+        s_koff[p] = ko;
+      }
     }
     __syncthreads();
     if (nrel == 0) continue;
@@ -1012,9 +1037,57 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       }
     }
     if (!inside || mi == 0) continue;
+    if (!decoded) {
+      for (int r = 0; r < nrel; ++r) {
+        flipped = flipped || s_cell[r] == c;
+        add_source(s_cell[r], s_mmu[r], s_site_of[r], r);
+      }
+      continue;
+    }
+    // the same terms as add_source, in the same order and arithmetic, from the decoded sources:
+    // a source outside the target's window loads entry 0 and adds nothing (no branch per source)
     for (int r = 0; r < nrel; ++r) {
-      flipped = flipped || s_cell[r] == c;
-      add_source(s_cell[r], s_mmu[r], s_site_of[r], r);
+      const int syx = s_yx[r], win = s_win[r];
+      const float mm = s_mmu[r];
+      flipped = flipped || syx == yx_self;
+      const int sy = syx >> 16, sx = syx & 0xffff;
+      if (has_dip) {
+        const int wy = win & 0xffff, wx = win >> 16;
+        const bool in = win >= 0 && (unsigned)(y - sy + wy) <= (unsigned)(2 * wy) &&
+                        (unsigned)(x - sx + wx) <= (unsigned)(2 * wx);
+        const size_t o = in ? (size_t)(s_koff[r] + cellk) : 0;
+        if (F64) {
+          const double kv = I.K64[o];
+          if (in) acc += s_mmu64[r] * kv;
+        } else {
+          const float kv = I.K32[o];
+          if (in) acc += mm * kv;
+          if (I.use_perp) {
+            const float kp = I.Kpo32[o];
+            if (in) accp += mm * kp;
+          }
+        }
+        touched = touched || in;
+      }
+      if (has_ex) {
+        const int dy = y - sy, dx = x - sx;
+        if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
+          const int q = (dy + hy) * I.nn_w + (dx + hx);
+          const float wq = PRELOAD ? s_nn[q] : I.nn_mask[q];
+          if (wq != 0.f) {
+            const FT msn = mm > 0.f ? 1 : -1;  // m of the source after its flip
+            if (I.in_plane) {
+              const int ss = s_site_of[r];
+              const FT oxs = ox_of(ss), oys = oy_of(ss);
+              ex += msn * (FT)wq * (oxi * oxs + oyi * oys);
+              if (!F64) exp_ += (float)(msn * (FT)wq * (oys * oxi - oxs * oyi));
+            } else {
+              ex += msn * (FT)wq;
+            }
+            touched = true;
+          }
+        }
+      }
     }
   }
   if (!inside || mi == 0 || mi == 2 || !(touched || flipped)) return;
