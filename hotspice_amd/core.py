@@ -1088,14 +1088,16 @@ class Magnets(ABC):
         if scheme == Scheme.WOLFF:
             return self._progress_wolff(t_max, MCsteps_max, **kwargs)
         attempt_freq = kwargs.get('attempt_freq', 1e10)
+        # the run state is read once here and once after every launch: nothing else changes it in
+        # between (the periodic recompute leaves t and the counters alone)
         st0 = self._state()
         t0, a0 = st0.t, st0.attempts
         begin = True
+        st = st0
         if scheme == Scheme.NEEL:
             if kwargs.keys() - {'attempt_freq'}:
                 raise TypeError(f"unexpected Neel keywords {sorted(kwargs.keys() - {'attempt_freq'})}")
             while True:
-                st = self._state()
                 done_mc = (st.attempts - a0)
                 if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
                     break
@@ -1109,9 +1111,10 @@ class Magnets(ABC):
                     chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
                 run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
                                max_steps=int(chunk), log=0, nblocks=0, begin=int(begin))
-                if begin:
-                    self._set_progress_origin(t0, a0)
-                self._dev.neel_run(run, persistent=self._neel_team_ok(), nblocks=self._neel_nblocks())
+                team = self._neel_team_ok()
+                if begin and not team:  # the team kernel takes the origin itself (hs_run_t.begin)
+                    self._set_progress_origin(t0, a0, st0)
+                self._dev.neel_run(run, persistent=team, nblocks=self._neel_nblocks())
                 begin = False
                 st = self._state()
                 if st.err:
@@ -1132,7 +1135,6 @@ class Magnets(ABC):
                 sel = self._select_geometry(r)
             while True:
                 per_step = self._metro_rate or max(1.0, 0.25 * sel.nsc)
-                st = self._state()
                 if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
                     break
                 if math.isfinite(MCsteps_max):
@@ -1144,9 +1146,7 @@ class Magnets(ABC):
                     chunk = min(chunk, max(1, self.recompute_every - self._since_recompute))
                 run = _lib.Run(t_max=float(t_max), mc_max=float(MCsteps_max), attempt_freq=float(attempt_freq),
                                max_steps=int(chunk), log=0, nblocks=nblocks, begin=int(begin))
-                if begin:
-                    self._set_progress_origin(t0, a0)
-                a_before = st.attempts
+                a_before = st.attempts  # (the launch takes the origin itself: hs_run_t.begin)
                 self._dev.metropolis_run(sel, run)
                 begin = False
                 st = self._state()
@@ -1157,7 +1157,6 @@ class Magnets(ABC):
                 self._since_recompute += int(st.steps_done)
                 self._maybe_recompute(st)
         self._after_device_step()
-        st = self._state()
         return st.t - t0, st.attempts / self.n - a0 / self.n
 
     def _progress_wolff(self, t_max, MCsteps_max, **kwargs):
@@ -1191,8 +1190,10 @@ class Magnets(ABC):
         st = self._state()
         return st.t - t0, st.attempts / self.n - a0 / self.n
 
-    def _set_progress_origin(self, t0, a0):
-        st = self._state()
+    def _set_progress_origin(self, t0, a0, st=None):
+        """Stores the origin (t0, attempts0) of a progress call in the run state (`st`: the current
+        state, if read already; it is updated in place)."""
+        st = self._state() if st is None else st
         st.t0, st.attempts0 = t0, a0
         self._dev.write_state(st)
 
