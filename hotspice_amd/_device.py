@@ -188,9 +188,10 @@ class DeviceState:
     def shadow_usable(self, which, changed):
         """True when a refresh from the shadow reproduces the full recompute `which` for at most
         `changed` changed cells (None: unknown)."""
-        return (changed is not None and self._shadow == (self._shadow_ver, bool(self.inst.use_perp))
-                and self.shadow_refreshes < self.SHADOW_MAX_REFRESHES and self._stores_shadow(which)
-                and changed <= self.SHADOW_MAX_CHANGED * self.nx * self.ny)
+        return (changed is not None and changed <= self.SHADOW_MAX_CHANGED * self.nx * self.ny
+                and self.nx <= 32767 and self.ny <= 32767  # hs_refresh_fields' limit
+                and self._shadow == (self._shadow_ver, bool(self.inst.use_perp))
+                and self.shadow_refreshes < self.SHADOW_MAX_REFRESHES and self._stores_shadow(which))
 
     def _shadow_stored(self, which):
         if self._stores_shadow(which):

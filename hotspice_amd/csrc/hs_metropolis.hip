@@ -53,6 +53,8 @@ inline unsigned update_grid(int64_t ntiles, int64_t requested) {
   const int64_t g = requested > 0 ? requested : UPDATE_WGS;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, std::max<int64_t>(1, ntiles)));
 }
+// sources whose kernel entries a cell of the persistent kernel's update loads together
+constexpr int PERSIST_SRC_BATCH = 8;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
 // the selection marks tiles in an LDS bitmap of this many 32-bit words (jobs with more tiles list
@@ -1041,6 +1043,82 @@ __device__ __forceinline__ void metro_update(const MetroJob &J, int tile_x, int 
       for (int r = 0; r < nrel; ++r) {
         flipped = flipped || s_cell[r] == c;
         add_source(s_cell[r], s_mmu[r], s_site_of[r], r);
+      }
+      continue;
+    }
+    if constexpr (!PRELOAD) {
+      // the persistent kernel (one workgroup, every tile, mostly windows that cover the tile):
+      // the kernel entries of PERSIST_SRC_BATCH sources are loaded together, then added in list
+      // order (a source outside the window, or past the list's end, loads entry 0 and adds nothing)
+      constexpr int SB = PERSIST_SRC_BATCH;
+      for (int r0 = 0; r0 < nrel; r0 += SB) {
+        int syxv[SB];
+        bool inv[SB];
+        float mmv[SB], kpv[SB];
+        FT kvv[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int r = r0 + u;
+          const bool live = r < nrel;
+          const int rr = live ? r : 0;
+          const int syx = s_yx[rr], win = s_win[rr];
+          const int sy = syx >> 16, sx = syx & 0xffff;
+          const int wy = win & 0xffff, wx = win >> 16;
+          const bool in = live && has_dip && win >= 0 && (unsigned)(y - sy + wy) <= (unsigned)(2 * wy) &&
+                          (unsigned)(x - sx + wx) <= (unsigned)(2 * wx);
+          const size_t o = in ? (size_t)(s_koff[rr] + cellk) : 0;
+          syxv[u] = syx;
+          inv[u] = in;
+          mmv[u] = s_mmu[rr];
+          kvv[u] = 0;
+          kpv[u] = 0.f;
+          if (has_dip) {  // the kernel arrays exist only with a pair energy
+            if (F64) {
+              kvv[u] = (FT)I.K64[o];
+            } else {
+              kvv[u] = (FT)I.K32[o];
+              if (I.use_perp) kpv[u] = I.Kpo32[o];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int r = r0 + u;
+          if (r >= nrel) break;
+          const int syx = syxv[u];
+          const float mm = mmv[u];
+          flipped = flipped || syx == yx_self;
+          if (has_dip) {
+            if (inv[u]) {
+              if (F64) {
+                acc += s_mmu64[r] * kvv[u];
+              } else {
+                acc += mm * kvv[u];
+                if (I.use_perp) accp += mm * kpv[u];
+              }
+            }
+            touched = touched || inv[u];
+          }
+          if (has_ex) {
+            const int dy = y - (syx >> 16), dx = x - (syx & 0xffff);
+            if (dy >= -hy && dy <= hy && dx >= -hx && dx <= hx) {
+              const int q = (dy + hy) * I.nn_w + (dx + hx);
+              const float wq = I.nn_mask[q];
+              if (wq != 0.f) {
+                const FT msn = mm > 0.f ? 1 : -1;  // m of the source after its flip
+                if (I.in_plane) {
+                  const int ss = s_site_of[r];
+                  const FT oxs = ox_of(ss), oys = oy_of(ss);
+                  ex += msn * (FT)wq * (oxi * oxs + oyi * oys);
+                  if (!F64) exp_ += (float)(msn * (FT)wq * (oys * oxi - oxs * oyi));
+                } else {
+                  ex += msn * (FT)wq;
+                }
+                touched = true;
+              }
+            }
+          }
+        }
       }
       continue;
     }

@@ -162,6 +162,29 @@ def test_fp64_fields_stay_exact_without_recompute():
     assert float(np.max(np.abs(E_inc - E_ref)) / np.max(np.abs(E_ref))) < 1e-12
 
 
+def test_fp64_exchange_only_matches_oracle():
+    """An instance without a pair energy (exchange and Zeeman only) has no kernel arrays: its FP64
+    steps add the exchange terms alone, in lock-step with the oracle."""
+    import hotspice_amd as hs
+    from oracle import model
+    n, T = 32, 600
+    kT = kB * T
+    p = _inputs(n, 12, T)
+    params = dict(UPDATE_SCHEME="Metropolis")
+    mm = hs.ASI.OOP_Square(1e-6, n, T=T, moment=1.1e-15, seed=12, fp64_fields=True,
+                           energies=[hs.ZeemanEnergy(8e-4), hs.ExchangeEnergy(J=0.2 * kT)],
+                           params=hs.SimParams(**params), **p)
+    om = model.make_magnets("OOP_Square", 1e-6, n=n, T=T, moment=1.1e-15, seed=12,
+                            energies=[model.Zeeman(8e-4), model.Exchange(J=0.2 * kT)], params=params, **p)
+    scale = _scale(om)
+    for s in range(100):
+        a = np.asarray(mm.update(r=4e-6)).reshape(2, -1)
+        b = np.asarray(om.update(r=4e-6)).reshape(2, -1)
+        assert np.array_equal(a, b), f"step {s}"
+    assert mm.switches > 0 and np.array_equal(mm.m, om.m)
+    assert float(np.max(np.abs(mm.E - om.E))) < 1e-12 * scale
+
+
 def test_fp64_refused_where_not_supported():
     import hotspice_amd as hs
     from hotspice_amd import _lib
