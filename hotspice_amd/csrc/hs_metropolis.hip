@@ -55,6 +55,9 @@ inline unsigned update_grid(int64_t ntiles, int64_t requested) {
 }
 // sources whose kernel entries a cell of the persistent kernel's update loads together
 constexpr int PERSIST_SRC_BATCH = 8;
+// tile bitmaps of at most this many words (32768 tiles of 32x8 cells) are read by the update
+// workgroups from LDS; larger grids list their tiles with k_metro_tiles
+constexpr int BITMAP_MAX_WORDS = 1024;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
 // the selection marks tiles in an LDS bitmap of this many 32-bit words (jobs with more tiles list
@@ -96,6 +99,11 @@ struct MetroWS {
 struct WorkList {
   int2 *wl;
   int32_t *count;
+  // Tile bitmaps of a presampled single-instance step (two, by step parity; nwords words each), or
+  // NULL: the presampling launch marks the tiles its accepted switches reach and the update reads
+  // them, so the step needs no tile-listing launch.
+  uint32_t *bits = nullptr;
+  int32_t nwords = 0;
 };
 
 // Everything one job's step needs (passed by value for one instance, from an array for a batch).
@@ -157,8 +165,11 @@ size_t ws_bytes(int nsc) {
          7 * align256(n * 4) + align256(8 * ((n + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS));
 }
 
-// A worklist for `ntiles` tiles: the entries and the two step counters.
-size_t wl_bytes(int64_t ntiles) { return align256(8 * (size_t)(ntiles > 0 ? ntiles : 1)) + 256; }
+// A worklist for `ntiles` tiles: the entries, the two step counters and the two tile bitmaps.
+inline int64_t tile_words(int64_t ntiles) { return (ntiles + 31) / 32; }
+size_t wl_bytes(int64_t ntiles) {
+  return align256(8 * (size_t)(ntiles > 0 ? ntiles : 1)) + 256 + 2 * align256(4 * (size_t)std::max<int64_t>(1, tile_words(ntiles)));
+}
 
 int tiles_of(const hs_instance_t &in) { return ((in.nx + MU_TX - 1) / MU_TX) * ((in.ny + MU_TY - 1) / MU_TY); }
 
@@ -1218,7 +1229,8 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_one(MetroJob J, Wor
 // thread through chains of dependent loads). Writes cand / accf / cmmu of every sample and the
 // workgroup's counts; k_metro_select_pre lists them (or redoes the step when attempt 0 has no
 // valid sample). Same draws and arithmetic as metro_select.
-__device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
+__device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, WorkList WL = WorkList{nullptr, nullptr},
+                                                int par = 0) {
   const hs_instance_t &I = J.I;
   const hs_select_t &SEL = J.SEL;
   const hs_run_t &R = J.R;
@@ -1231,6 +1243,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   // the step is completed here rather than by k_metro_select_pre: without PBC (that selection
   // lists the tiles itself) and without a step log (which wants the ordered list)
   const bool fuse = !I.pbc && !R.log;
+  // tile bitmap mode (single instance, fused): the accepted switches mark the tiles they reach,
+  // and this launch also runs the selection when the step is not completed here
+  const bool mark = fuse && WL.bits != nullptr;
   if (fuse && threadIdx.x < HS_MAX_UC) s_site[threadIdx.x] = 0;
   // the previous step's mark goes first (before this workgroup's ticket, so before the last
   // workgroup sets it again)
@@ -1239,7 +1254,17 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
                   (hs_step_free(R.first_free, S.steps_done) ||
                    (hs_lower_than(S.t - S.t0, R.t_max) &&
                     hs_lower_than((double)S.attempts / n - (double)S.attempts0 / n, R.mc_max)));
-  if (!go) return;
+  if (!go) {  // the run is over: as the selection records it (no switch, done)
+    if (mark && blk == 0 && threadIdx.x == 0) {
+      *ws.acc_n = 0;
+      if (!S.done) {
+        hs_state_t o = S;
+        o.done = 1;
+        J.st[0] = o;
+      }
+    }
+    return;
+  }
   const uint64_t step = (uint64_t)S.step;
   int ox, oy;
   grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
@@ -1270,6 +1295,25 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
     ws.cmmu[j] = mmu;
     nacc = acc ? 1 : 0;
     if (acc && fuse) atomicAdd(&s_site[hs_site(I, c / I.nx, c % I.nx)], 1);
+    if (acc && mark) {
+      // the tiles within the largest window of the switch (dipolar or exchange; the switch's own
+      // tile included): a superset of those the update's reach rules select, whose extra tiles
+      // find no source in reach and write nothing. Read by the update after this launch.
+      const bool has_dip = I.dip_pf != 0.0, has_ex = I.J != 0.0 && I.nn_w > 0;
+      const int Wx = max(has_dip ? I.kwx : 0, has_ex ? I.nn_w / 2 : 0), Wy = max(has_dip ? I.kwy : 0, has_ex ? I.nn_h / 2 : 0);
+      const int sy = c / I.nx, sx = c % I.nx;
+      const int tx0 = max(0, sx - Wx) / MU_TX, tx1 = min(I.nx - 1, sx + Wx) / MU_TX;
+      const int ty0 = max(0, sy - Wy) / MU_TY, ty1 = min(I.ny - 1, sy + Wy) / MU_TY;
+      uint32_t *bits = WL.bits + (size_t)par * WL.nwords;
+      for (int ty = ty0; ty <= ty1; ++ty) {
+        const int t0 = ty * J.tiles_x + tx0, t1 = ty * J.tiles_x + tx1;
+        for (int w = t0 >> 5; w <= (t1 >> 5); ++w) {
+          const int b0 = max(t0, w << 5) & 31, b1 = min(t1, (w << 5) + 31) & 31;
+          const uint32_t hi = b1 == 31 ? 0xffffffffu : ((1u << (b1 + 1)) - 1u);
+          atomicOr(&bits[w], hi & ~((1u << b0) - 1u));
+        }
+      }
+    }
   }
   // in-grid, valid and accepted samples of the workgroup, 21 bits each (at most 256)
   const long long sums = block_sum_i64(((long long)inb << 42) + ((long long)val << 21) + nacc, s_l);
@@ -1305,7 +1349,16 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   const int site_acc = tid < nsites ? __hip_atomic_load(&ws.fz[8 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
   __syncthreads();
   if (tid < 8 + HS_MAX_UC) ws.fz[tid] = 0;  // zero for the next step
-  if (tot_inb <= 0 || tot_val <= 0) return;
+  if (tot_inb <= 0 || tot_val <= 0) {
+    // attempt 0 has no valid sample: with the tile bitmap there is no tile-listing launch to run
+    // the selection, so this workgroup runs it (redrawing attempt 0, then the next ones, and
+    // listing the tiles of its switches in the worklist), with the draws of the selection kernels
+    if (mark) {
+      __syncthreads();
+      metro_select<PRESELECT_BATCH, false, false, PRESAMPLE_THREADS>(J, WL, par, 0);
+    }
+    return;
+  }
   if (tid < nsites) ws.trunc[tid] = (tot_acc > 1 && SEL.reduced_k != 0 && site_acc > SEL.cutoff) ? 1 : 0;
   if (tid == 0) {
     ws.info[0] = ox;
@@ -1323,7 +1376,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk) {
   }
 }
 
-__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J) { metro_presample(J, blockIdx.x); }
+__global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample(MetroJob J, WorkList WL, int par) {
+  metro_presample(J, blockIdx.x, WL, par);
+}
 
 // A batch: block b presamples block map[b].y of job map[b].x.
 __global__ void __launch_bounds__(PRESAMPLE_THREADS) k_metro_presample_many(const MetroJob *__restrict__ jobs,
@@ -1456,16 +1511,60 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob
   metro_select<SELECT_BATCH>(jobs[j], WL, par, j);
 }
 
-// The update of the step's listed tiles (one tile at a time per workgroup); then the counter of
-// the next step is cleared.
+// The update of the step's tiles (one tile at a time per workgroup): the listed ones, or, when
+// the presampling launch completed the step in tile bitmap mode, the marked ones (the k-th marked
+// tile, in tile order, goes to workgroup k mod gridDim); then the counter and the bitmap of the
+// next step are cleared.
 __global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_one(MetroJob J, WorkList WL, int par) {
-  const int n = WL.count[par];
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const int t = WL.wl[i].y;
-    metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
+  if (WL.bits && J.ws.info[3] == FUSED_DONE) {
+    __shared__ uint32_t s_w[BITMAP_MAX_WORDS];
+    __shared__ int s_pref[BITMAP_MAX_WORDS];
+    __shared__ int s_tmp[MU_THREADS / 64];
+    const int tid = threadIdx.x, nw = WL.nwords;
+    const uint32_t *bw = WL.bits + (size_t)par * nw;
+    // each thread takes a contiguous run of words; their set bits are counted and scanned
+    const int per = (nw + MU_THREADS - 1) / MU_THREADS;
+    const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
+    int mine = 0;
+    for (int w = w0; w < w1; ++w) {
+      const uint32_t v = bw[w];
+      s_w[w] = v;
+      mine += __popc(v);
+    }
+    int total = 0;
+    int off = block_exclusive_scan(mine, s_tmp, total);
+    for (int w = w0; w < w1; ++w) {
+      s_pref[w] = off;
+      off += __popc(s_w[w]);
+    }
     __syncthreads();
+    for (int k = blockIdx.x; k < total; k += gridDim.x) {
+      // the word holding the k-th marked tile: the last one whose prefix is at most k
+      int lo = 0, hi = nw - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pref[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      uint32_t v = s_w[lo];
+      for (int r = k - s_pref[lo]; r > 0; --r) v &= v - 1u;  // drop the lower marked tiles
+      const int t = 32 * lo + __ffs((int)v) - 1;
+      metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
+      __syncthreads();
+    }
+  } else {
+    const int n = WL.count[par];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+      const int t = WL.wl[i].y;
+      metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
+      __syncthreads();
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) WL.count[par ^ 1] = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) WL.count[par ^ 1] = 0;
+    if (WL.bits)
+      for (int w = threadIdx.x; w < WL.nwords; w += MU_THREADS) WL.bits[(size_t)(par ^ 1) * WL.nwords + w] = 0u;
+  }
 }
 
 __global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_many(const MetroJob *__restrict__ jobs, WorkList WL,
@@ -1484,6 +1583,8 @@ __global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_many(
 // presampling counters and marks of the call's job(s): the workspace may have held other data.
 __global__ void k_metro_clear(WorkList WL, MetroWS ws) {
   if (threadIdx.x < 2) WL.count[threadIdx.x] = 0;
+  if (WL.bits)
+    for (int w = threadIdx.x; w < 2 * WL.nwords; w += blockDim.x) WL.bits[w] = 0u;
   for (int i = threadIdx.x; i < 8 + HS_MAX_UC; i += blockDim.x) ws.fz[i] = 0;
   if (threadIdx.x == 0) {
     ws.info[3] = 0;
@@ -2049,27 +2150,35 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   }
   const int nt = J.tiles_x * J.tiles_y;
   char *wlp = (char *)ws + ws_bytes(J.SEL.nsc);
-  const WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
-  k_metro_clear<<<1, 64, 0, s>>>(WL, J.ws);
-  HS_LAUNCH_CHECK("k_metro_clear");
-  const unsigned wgs = update_grid(nt, run->nblocks);
   // grid selection without METROPOLIS_TIME and with more samples than one batch per selection
   // thread: the samples are drawn and accepted by a wide launch
   const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time && J.SEL.nsc > PRESAMPLE_MIN;
+  const bool fused = !inst->pbc && !run->log;
+  // the presampling launch completes the step itself without PBC or log; it then also marks the
+  // step's tiles in a bitmap (grids of at most BITMAP_MAX_WORDS * 32 tiles), which the update reads
+  const int nwords = (int)tile_words(nt);
+  const bool bitmap = presample && fused && nwords <= BITMAP_MAX_WORDS;
+  WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
+  if (bitmap) {
+    WL.bits = (uint32_t *)(wlp + align256(8 * (size_t)nt) + 256);
+    WL.nwords = nwords;
+  }
+  k_metro_clear<<<1, 64, 0, s>>>(WL, J.ws);
+  HS_LAUNCH_CHECK("k_metro_clear");
+  const unsigned wgs = update_grid(nt, run->nblocks);
   const unsigned pblocks = (unsigned)((J.SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS);
   for (int64_t j = 0; j < run->max_steps; ++j) {
     const int par = (int)(j & 1);
     if (presample) {
-      k_metro_presample<<<pblocks, PRESAMPLE_THREADS, 0, s>>>(J);
+      k_metro_presample<<<pblocks, PRESAMPLE_THREADS, 0, s>>>(J, WL, par);
       HS_LAUNCH_CHECK("k_metro_presample");
-      // the presampling launch completes the step itself without PBC or log; the tile launch then
-      // runs the selection only when it did not (the rare attempt without a valid sample)
-      const bool fused = !inst->pbc && !run->log;
+      // without the bitmap, the tile launch runs the selection only when the presampling launch
+      // did not complete the step (the rare attempt without a valid sample)
       if (!fused) {
         k_metro_select_pre<<<1, MS_THREADS, 0, s>>>(J, WL, par);
         HS_LAUNCH_CHECK("k_metro_select_pre");
       }
-      if (!inst->pbc) {
+      if (!inst->pbc && !bitmap) {
         k_metro_tiles<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(J, WL, par, fused ? 1 : 0);
         HS_LAUNCH_CHECK("k_metro_tiles");
       }
