@@ -55,8 +55,8 @@ inline unsigned update_grid(int64_t ntiles, int64_t requested) {
 }
 // sources whose kernel entries a cell of the persistent kernel's update loads together
 constexpr int PERSIST_SRC_BATCH = 8;
-// tile bitmaps of at most this many words (32768 tiles of 32x8 cells) are read by the update
-// workgroups from LDS; larger grids list their tiles with k_metro_tiles
+// tile bitmaps of at most this many words (32768 tiles of 32x8 cells) are listed by the last
+// presampling workgroup (a few words per thread); larger grids list their tiles with k_metro_tiles
 constexpr int BITMAP_MAX_WORDS = 1024;
 // a switch whose window spans more tiles than this lists every tile of its job instead
 constexpr int MARK_MAX_TILES = 256;
@@ -100,8 +100,8 @@ struct WorkList {
   int2 *wl;
   int32_t *count;
   // Tile bitmaps of a presampled single-instance step (two, by step parity; nwords words each), or
-  // NULL: the presampling launch marks the tiles its accepted switches reach and the update reads
-  // them, so the step needs no tile-listing launch.
+  // NULL: the presampling launch marks the tiles its accepted switches reach and its last
+  // workgroup lists them, so the step needs no tile-listing launch.
   uint32_t *bits = nullptr;
   int32_t nwords = 0;
 };
@@ -1360,6 +1360,33 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
     return;
   }
   if (tid < nsites) ws.trunc[tid] = (tot_acc > 1 && SEL.reduced_k != 0 && site_acc > SEL.cutoff) ? 1 : 0;
+  if (mark) {
+    // the marked tiles (every workgroup's marks were performed before its ticket) become the
+    // step's worklist, in tile order: each thread takes a contiguous run of words
+    const uint32_t *bits = WL.bits + (size_t)par * WL.nwords;
+    const int nw = WL.nwords, per = (nw + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS;
+    const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
+    uint32_t wv[(BITMAP_MAX_WORDS + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < (BITMAP_MAX_WORDS + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; ++k) {
+      wv[k] = w0 + k < w1 ? __hip_atomic_load(&bits[w0 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+      mine += __popc(wv[k]);
+    }
+    __shared__ int s_scan[PRESAMPLE_THREADS / 64];
+    int ntl = 0;
+    int off = block_exclusive_scan(mine, s_scan, ntl);
+#pragma unroll
+    for (int k = 0; k < (BITMAP_MAX_WORDS + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS; ++k) {
+      uint32_t v = wv[k];
+      while (v) {
+        const int b = __ffs((int)v) - 1;
+        v &= v - 1u;
+        WL.wl[off++] = make_int2(0, 32 * (w0 + k) + b);
+      }
+    }
+    if (tid == 0) WL.count[par] = ntl;
+  }
   if (tid == 0) {
     ws.info[0] = ox;
     ws.info[1] = oy;
@@ -1511,54 +1538,14 @@ __global__ void __launch_bounds__(MS_THREADS) k_metro_select_many(const MetroJob
   metro_select<SELECT_BATCH>(jobs[j], WL, par, j);
 }
 
-// The update of the step's tiles (one tile at a time per workgroup): the listed ones, or, when
-// the presampling launch completed the step in tile bitmap mode, the marked ones (the k-th marked
-// tile, in tile order, goes to workgroup k mod gridDim); then the counter and the bitmap of the
-// next step are cleared.
+// The update of the step's listed tiles (one tile at a time per workgroup); then the counter and
+// the tile bitmap of the next step are cleared.
 __global__ void __launch_bounds__(MU_THREADS, UPDATE_WAVES) k_metro_update_one(MetroJob J, WorkList WL, int par) {
-  if (WL.bits && J.ws.info[3] == FUSED_DONE) {
-    __shared__ uint32_t s_w[BITMAP_MAX_WORDS];
-    __shared__ int s_pref[BITMAP_MAX_WORDS];
-    __shared__ int s_tmp[MU_THREADS / 64];
-    const int tid = threadIdx.x, nw = WL.nwords;
-    const uint32_t *bw = WL.bits + (size_t)par * nw;
-    // each thread takes a contiguous run of words; their set bits are counted and scanned
-    const int per = (nw + MU_THREADS - 1) / MU_THREADS;
-    const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
-    int mine = 0;
-    for (int w = w0; w < w1; ++w) {
-      const uint32_t v = bw[w];
-      s_w[w] = v;
-      mine += __popc(v);
-    }
-    int total = 0;
-    int off = block_exclusive_scan(mine, s_tmp, total);
-    for (int w = w0; w < w1; ++w) {
-      s_pref[w] = off;
-      off += __popc(s_w[w]);
-    }
+  const int n = WL.count[par];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int t = WL.wl[i].y;
+    metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
     __syncthreads();
-    for (int k = blockIdx.x; k < total; k += gridDim.x) {
-      // the word holding the k-th marked tile: the last one whose prefix is at most k
-      int lo = 0, hi = nw - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pref[mid] <= k) lo = mid;
-        else hi = mid - 1;
-      }
-      uint32_t v = s_w[lo];
-      for (int r = k - s_pref[lo]; r > 0; --r) v &= v - 1u;  // drop the lower marked tiles
-      const int t = 32 * lo + __ffs((int)v) - 1;
-      metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
-      __syncthreads();
-    }
-  } else {
-    const int n = WL.count[par];
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-      const int t = WL.wl[i].y;
-      metro_update<MU_TX, MU_TY, true>(J, t % J.tiles_x, t / J.tiles_x);
-      __syncthreads();
-    }
   }
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) WL.count[par ^ 1] = 0;
@@ -2155,7 +2142,8 @@ extern "C" int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *s
   const bool presample = J.SEL.scheme == 0 && !J.SEL.metropolis_time && J.SEL.nsc > PRESAMPLE_MIN;
   const bool fused = !inst->pbc && !run->log;
   // the presampling launch completes the step itself without PBC or log; it then also marks the
-  // step's tiles in a bitmap (grids of at most BITMAP_MAX_WORDS * 32 tiles), which the update reads
+  // step's tiles in a bitmap (grids of at most BITMAP_MAX_WORDS * 32 tiles) and its last workgroup
+  // lists them for the update
   const int nwords = (int)tile_words(nt);
   const bool bitmap = presample && fused && nwords <= BITMAP_MAX_WORDS;
   WorkList WL{(int2 *)wlp, (int32_t *)(wlp + align256(8 * (size_t)nt))};
