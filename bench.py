@@ -50,9 +50,9 @@ PERIOD = 1024
 # size; the commands and counter values are in profiles/neel_team_pinwheel256_pmc.md
 NEEL_TEAM_TRAFFIC_PER_LAUNCH = 94.8e6
 NEEL_TEAM_TRAFFIC_SOURCE = "profile figure (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, profiles/neel_team_pinwheel256_pmc.md)"
-# HBM bytes per device step of the 2048x2048 Metropolis workload (presample + tiles + update), a
-# profile figure from the same two PMC passes: profiles/metro_pmc_oop2048_ipsq64.md
-METRO2048_TRAFFIC_PER_STEP = 30.6e6
+# HBM bytes per device step of the 2048x2048 Metropolis workload (presample with the tile bitmap +
+# update), a profile figure from the same two PMC passes: profiles/metro_pmc_oop2048_ipsq64.md
+METRO2048_TRAFFIC_PER_STEP = 24.05e6
 METRO2048_TRAFFIC_SOURCE = "profile figure per device step (rocprofv3 PMC, profiles/metro_pmc_oop2048_ipsq64.md)"
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("all", "neel", "metro64", "metro2048", "batched", "neel_batched", "served")
