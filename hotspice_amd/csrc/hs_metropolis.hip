@@ -1243,8 +1243,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
   // the step is completed here rather than by k_metro_select_pre: without PBC (that selection
   // lists the tiles itself) and without a step log (which wants the ordered list)
   const bool fuse = !I.pbc && !R.log;
-  // tile bitmap mode (single instance, fused): the accepted switches mark the tiles they reach,
-  // and this launch also runs the selection when the step is not completed here
+  // tile bitmap mode (single instance, fused): the accepted switches mark the tiles they reach.
+  // The launch then also owns the selection's retries: an attempt without a valid sample leaves
+  // the step to the next launch, which draws the next attempt (ws.info[5])
   const bool mark = fuse && WL.bits != nullptr;
   if (fuse && threadIdx.x < HS_MAX_UC) s_site[threadIdx.x] = 0;
   // the previous step's mark goes first (before this workgroup's ticket, so before the last
@@ -1266,15 +1267,18 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
     return;
   }
   const uint64_t step = (uint64_t)S.step;
+  const int attempt = mark ? ws.info[5] : 0;  // the selection's attempt index (core.py:487)
   int ox, oy;
-  grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, 0ull), hs_bits53(I.seed, step, 2u, 1ull), ox, oy);
+  grid_offsets(I, SEL, hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 0)),
+               hs_bits53(I.seed, step, 2u, (uint64_t)(4 * attempt + 1)), ox, oy);
   const int pidx = hs_pmod(oy, I.uy) * I.ux + hs_pmod(ox, I.ux);
   const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
   const int j = blk * PRESAMPLE_THREADS + threadIdx.x;
   int inb = 0, val = 0, nacc = 0;
   if (fuse) __syncthreads();  // s_site cleared
   if (j < SEL.nsc) {
-    const int pick = (int)floor(hs_u53(hs_bits53(I.seed, step, 3u, (uint64_t)j), 0) * cnt);
+    const int pick =
+        (int)floor(hs_u53(hs_bits53(I.seed, step, 3u | ((uint32_t)attempt << 8), (uint64_t)j), 0) * cnt);
     bool ok;
     const int cell = grid_cell(I, SEL, ox, oy, j, SEL.pat_cells[p0 + pick], ok);
     const bool v = ok && (I.flags[cell] & HS_FLAG_OCC);
@@ -1350,12 +1354,21 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
   __syncthreads();
   if (tid < 8 + HS_MAX_UC) ws.fz[tid] = 0;  // zero for the next step
   if (tot_inb <= 0 || tot_val <= 0) {
-    // attempt 0 has no valid sample: with the tile bitmap there is no tile-listing launch to run
-    // the selection, so this workgroup runs it (redrawing attempt 0, then the next ones, and
-    // listing the tiles of its switches in the worklist), with the draws of the selection kernels
-    if (mark) {
-      __syncthreads();
-      metro_select<PRESELECT_BATCH, false, false, PRESAMPLE_THREADS>(J, WL, par, 0);
+    // This attempt has no valid sample. Without the bitmap, the selection launch after this one
+    // redraws the step. With it, the step (its counters, state and worklist) is left as it is and
+    // the next launch draws the next attempt, as the selection's loop would (the same draws);
+    // a call then completes one step fewer. No sample in range at all (the selection's
+    // _select_single, core.py:493) cannot happen on a grid of more than 1024 supercells.
+    if (mark && tid == 0) {
+      *ws.acc_n = 0;
+      if (tot_inb <= 0 || attempt + 1 >= MAX_ATTEMPTS) {
+        hs_state_t o = S;
+        o.err = 2;
+        o.done = 1;
+        J.st[0] = o;
+      } else {
+        ws.info[5] = attempt + 1;
+      }
     }
     return;
   }
@@ -1392,6 +1405,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
     ws.info[1] = oy;
     ws.info[2] = 0;
     ws.info[4] = 1;
+    ws.info[5] = 0;  // the next step starts at attempt 0
     *ws.acc_n = tot_acc;
     hs_state_t o = S;
     o.attempts += tot_val;
@@ -1576,6 +1590,7 @@ __global__ void k_metro_clear(WorkList WL, MetroWS ws) {
   if (threadIdx.x == 0) {
     ws.info[3] = 0;
     ws.info[4] = 0;
+    ws.info[5] = 0;
   }
 }
 
