@@ -210,3 +210,22 @@ def test_batched_refreshes_equal_alone():
                 np.testing.assert_array_equal(x, y)
             else:
                 assert x == y
+
+
+def test_refresh_entry_point_refusals():
+    """hs_refresh_fields refuses an instance without a shadow and a `which` that leaves out a
+    component the shadow holds (E_perp of an in-plane instance)."""
+    import ctypes as C
+    from hotspice_amd import _lib
+    mm, _ = _build("neel_pinwheel_perp", seed=9)
+    d = mm._dev
+    scr = d.recompute_scratch()
+    with pytest.raises(_lib.HotspiceError, match="covers every field component"):
+        _lib.check(d.lib.hs_refresh_fields(C.byref(d.inst), 1, d.ptr(scr), scr.numel(), d.stream()))
+    with pytest.raises(_lib.HotspiceError, match="scratch"):
+        _lib.check(d.lib.hs_refresh_fields(C.byref(d.inst), 3, d.ptr(scr), 16, d.stream()))
+    inst = _lib.Instance.from_buffer_copy(bytes(d.inst))
+    inst.C64 = None
+    with pytest.raises(_lib.HotspiceError, match="no FP64 shadow"):
+        _lib.check(d.lib.hs_refresh_fields(C.byref(inst), 3, d.ptr(scr), scr.numel(), d.stream()))
+    assert int(d.lib.hs_refresh_scratch_bytes(C.byref(d.inst))) == 12 * mm.nx * mm.ny + 8 * mm.ny + 4
