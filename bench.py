@@ -642,8 +642,8 @@ def _secondary(line):
 def bench_all(args):
     """The plain Neel line (`value`), then the other configurations of the metric, each timed in
     this run with its own defaults: plain 2048x2048 Metropolis, the 64-instance batch with its
-    plain-path comparison, and the served mix at the offered 200 jobs/s and at half the rate the
-    device sustained there."""
+    plain-path comparison, 256 batched Neel instances with theirs, and the served mix at the
+    offered 200 jobs/s and at half the rate the device sustained there."""
     t0 = time.perf_counter()
 
     def progress(what):
@@ -662,6 +662,8 @@ def bench_all(args):
     progress("plain Metropolis 64x64 (float64)")
     sec["batched"] = _secondary(bench_batched(sub))
     progress("batched")
+    sec["neel_batched"] = _secondary(bench_neel_batched(sub))
+    progress("batched Neel")
     served = bench_served(sub, rate=args.rate, count=args.jobs)
     sec["served"] = _secondary(served)
     progress("served")
