@@ -357,7 +357,10 @@ int hs_neel_run(const hs_instance_t *inst, const hs_run_t *run, hs_state_t *stat
  * energies.py:310-366 with its convolution-or-sum branch rule, ZeemanEnergy / ExchangeEnergy
  * updates), looped as Magnets.progress. ws: device scratch of hs_metropolis_ws_bytes(inst, sel)
  * bytes. state: device hs_state_t[2] (copy 0 is the state). log_cells/log_count: optional
- * per-step accepted cells (capacity sel->nsc per step, at least 1) and counts. Non-blocking.
+ * per-step accepted cells (capacity sel->nsc per step, at least 1) and counts. A call launches
+ * run->max_steps steps and completes at most that many (state.steps_done): the stop predicate
+ * can end it earlier, and a presampled step without PBC or log whose selection needs another
+ * attempt (core.py:487) takes the next launch for it. Non-blocking.
  */
 int hs_metropolis_run(const hs_instance_t *inst, const hs_select_t *sel, const hs_run_t *run,
                       hs_state_t *state, void *ws, int32_t *log_cells, int32_t *log_count,
