@@ -271,15 +271,20 @@ def _lower_than_vec(x, y, rtol=1e-10):
 
 
 def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk, sts):
-    """The pending test and the chunk sizes of neel_chunk, over all instances at once."""
+    """The pending test and the chunk sizes of neel_chunk, over all instances at once. A launch in
+    which every instance runs all its steps (t_max infinite, MC budget lasting past them) is
+    queued without reading the states back: each step is one attempt, so the host advances the
+    attempt counts itself (t does not enter the pending test then), and the states are read once
+    a launch may stop early or the call ends."""
     first = True
     n = np.array([m.n for m in mags], dtype=float)
     t0 = np.array([s[0] for s in starts], dtype=float)
     a0 = np.array([s[1] for s in starts], dtype=np.int64)
     every = np.array([m.recompute_every or 0 for m in mags], dtype=np.int64)
+    t = np.array([st.t for st in sts], dtype=float)
+    att = np.array([st.attempts for st in sts], dtype=np.int64)
+    unread = False  # launches queued since the states were last read
     while True:
-        t = np.array([st.t for st in sts], dtype=float)
-        att = np.array([st.attempts for st in sts], dtype=np.int64)
         pend = _lower_than_vec(t - t0, t_max) & _lower_than_vec(att / n - a0 / n, mc)
         idx = np.flatnonzero(pend)
         if idx.size == 0:
@@ -304,10 +309,35 @@ def _progress_many_neel(mags, starts, t_max, mc, f0, nbl, chunk, sts):
         else:
             full = None
         # the first launch opens every pending instance's progress call (origin = starts)
+        if full is not None and all(full):
+            queued = {id(m) for m in enqueue_neel(items, f0, begin=first, full=full)}
+            first = False
+            unread = True
+            for i, k in zip(idx.tolist(), steps.tolist()):
+                att[i] += k
+                m = mags[i]
+                if id(m) in queued:
+                    m._recompute_done()
+                else:
+                    m._since_recompute += k
+            continue
         after = launch_neel(items, f0, begin=first, full=full)
         first = False
         for i, st in zip(idx.tolist(), after):
             sts[i] = st
+        if unread and idx.size < len(mags):  # the others advanced in the queued launches
+            in_launch = set(idx.tolist())
+            fresh = read_states(mags)
+            _raise_first({i: int(st.err) for i, st in enumerate(fresh) if st.err}, "Neel")
+            for i in range(len(mags)):
+                if i not in in_launch:
+                    sts[i] = fresh[i]
+        unread = False
+        t = np.array([st.t for st in sts], dtype=float)
+        att = np.array([st.attempts for st in sts], dtype=np.int64)
+    if unread:
+        sts = read_states(mags)
+        _raise_first({i: int(st.err) for i, st in enumerate(sts) if st.err}, "Neel")
     return _results(mags, starts, sts)
 
 
