@@ -1097,9 +1097,14 @@ class Magnets(ABC):
         if scheme == Scheme.NEEL:
             if kwargs.keys() - {'attempt_freq'}:
                 raise TypeError(f"unexpected Neel keywords {sorted(kwargs.keys() - {'attempt_freq'})}")
+            # A launch that runs all its steps (t_max infinite, the MC budget lasting past them: one
+            # attempt per step) is queued without reading the state back; the host advances the
+            # attempt count, and the state is read after a launch that may stop early, or at the end.
+            t_cur, att = st.t, st.attempts
+            unread = False
             while True:
-                done_mc = (st.attempts - a0)
-                if not (lower_than(st.t - t0, t_max) and lower_than(st.attempts / self.n - a0 / self.n, MCsteps_max)):
+                done_mc = (att - a0)
+                if not (lower_than(t_cur - t0, t_max) and lower_than(att / self.n - a0 / self.n, MCsteps_max)):
                     break
                 if math.isfinite(MCsteps_max):
                     chunk = max(1, int(math.ceil(MCsteps_max * self.n - done_mc)) + 1)
@@ -1116,11 +1121,25 @@ class Magnets(ABC):
                     self._set_progress_origin(t0, a0, st0)
                 self._dev.neel_run(run, persistent=team, nblocks=self._neel_nblocks())
                 begin = False
+                full = (math.isinf(t_max) and t_max > 0 and math.isfinite(MCsteps_max)
+                        and chunk <= math.ceil(MCsteps_max * self.n - done_mc))
+                if full:
+                    att += chunk
+                    unread = True
+                    self._since_recompute += chunk
+                    self._maybe_recompute()
+                    continue
+                st = self._state()
+                unread = False
+                if st.err:
+                    raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
+                t_cur, att = st.t, st.attempts
+                self._since_recompute += int(st.steps_done)
+                self._maybe_recompute(st)
+            if unread:
                 st = self._state()
                 if st.err:
                     raise _lib.HotspiceError(f"device Neel step failed (err={st.err})")
-                self._since_recompute += int(st.steps_done)
-                self._maybe_recompute(st)
         else:
             r = kwargs.get('r', 0)
             Q = kwargs.get('Q', 0.05)
