@@ -53,8 +53,12 @@ NEEL_TEAM_TRAFFIC_PER_LAUNCH = 94.8e6
 NEEL_TEAM_TRAFFIC_SOURCE = "profile figure (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, profiles/neel_team_pinwheel256_pmc.md)"
 # HBM bytes per device step of the 2048x2048 Metropolis workload (presample with the tile bitmap +
 # update), a profile figure from the same two PMC passes: profiles/metro_pmc_oop2048_ipsq64.md
+# HBM bytes per device step of the 2048x2048 workload's first two sweeps from its random state, from a
+# rocprofv3 PMC profile; once the grid has settled, steps accept far fewer switches and move less
 METRO2048_TRAFFIC_PER_STEP = 24.05e6
-METRO2048_TRAFFIC_SOURCE = "profile figure per device step (rocprofv3 PMC, profiles/metro_pmc_oop2048_ipsq64.md)"
+METRO2048_TRAFFIC_SOURCE = ("profile figure per device step of the first 2 sweeps from the random state (rocprofv3 "
+                            "PMC, profiles/metro_pmc_oop2048_ipsq64.md); not applied to a run whose steps accept "
+                            "fewer switches")
 METRIC = "switch attempts/s (plain vs batched/served) at 1e-5 rel E err; p50/p99 job lat"
 WORKLOADS = ("all", "neel", "metro64", "metro2048", "batched", "neel_batched", "served")
 
@@ -275,14 +279,14 @@ def bench_metro(args, large):
     from hotspice_amd.workloads import oop_large, oop_metropolis, oop_metropolis_inputs, kB
     a = 200e-9
     if large:
-        mm, kw, sweeps, warm = oop_large(), dict(r=20 * a), 4, 0.2
+        mm, kw, sweeps, warm = oop_large(), dict(r=20 * a), 200, 0.2
         mm.recompute_every = max(1, int(16 * mm.n / 2850))  # every ~16 sweeps (about 2850 samples per step)
         desc = ("plain Metropolis: OOP_Square 2048x2048 (4.19M magnets), a=200 nm, moment 1.1e-15, T=350 K, "
                 "E_B N(30 kT, 5%), dipolar cutoff 20a (41x41 window, REDUCED_KERNEL_SIZE=20) + exchange J=0.5 kT, "
                 "grid r=20a, FP32+Kahan, FP64 recompute every ~16 sweeps")
         kwin = 20
     else:
-        mm, kw, sweeps, warm = oop_metropolis(fp64=True), dict(r=8 * a), 200, 1
+        mm, kw, sweeps, warm = oop_metropolis(fp64=True), dict(r=8 * a), 1000, 1
         desc = ("plain Metropolis: OOP_Square 64x64 (4096 magnets), a=200 nm, moment 1.1e-15, T=300 K, "
                 "E_B N(20 kT, 5%), full 127x127 dipolar kernel, grid r=8a, float64 fields, increments and "
                 "acceptance (Magnets(fp64_fields=True))")
@@ -295,8 +299,7 @@ def bench_metro(args, large):
     step0 = mm._state().step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        mm.progress(t_max=math.inf, MCsteps_max=1.0, **kw)
+    mm.progress(t_max=math.inf, MCsteps_max=float(steps), **kw)  # one run of `steps` MC sweeps
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     attempts, accepted = mm.attempted_switches - a0, mm.switches - s0
@@ -317,11 +320,12 @@ def bench_metro(args, large):
     line = base_line(args, attempts / wall, steps, 0, wall, dtype,
                      {"workload": desc, "attempts": attempts, "accepted": accepted, "device_steps": nsteps,
                       "rel_E_err_vs_fp64": err, "window_cells_per_accept": nt_per_acc, "parallelism": "single GPU"})
-    kname = ("k_metro_presample + k_metro_tiles + k_metro_update_one (per progress call)" if large
-             else "k_metro_persist_one_f64 (per progress call)")
+    kname = ("k_metro_presample + k_metro_update_one (per library call of the progress call)" if large
+             else "k_metro_persist_one_f64 (per library call of the progress call)")
     line["roofline"] = roofline(kname, launches, bytes_total)
     if large:
-        line["roofline"]["traffic"] = METRO2048_TRAFFIC_PER_STEP * nsteps / len(launches)
+        line["roofline"]["traffic"] = None
+        line["roofline"]["traffic_first_sweeps_per_step"] = METRO2048_TRAFFIC_PER_STEP
         line["roofline"]["traffic_source"] = METRO2048_TRAFFIC_SOURCE
     if not args.no_cpu:
         if large:
