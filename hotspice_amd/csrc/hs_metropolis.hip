@@ -1275,6 +1275,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
   const int p0 = SEL.pat_off[pidx], cnt = SEL.pat_off[pidx + 1] - p0;
   const int j = blk * PRESAMPLE_THREADS + threadIdx.x;
   int inb = 0, val = 0, nacc = 0;
+  int c = -1;
+  bool acc = false;
+  float mmu = 0.f;
   if (fuse) __syncthreads();  // s_site cleared
   if (j < SEL.nsc) {
     const int pick =
@@ -1282,11 +1285,9 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
     bool ok;
     const int cell = grid_cell(I, SEL, ox, oy, j, SEL.pat_cells[p0 + pick], ok);
     const bool v = ok && (I.flags[cell] & HS_FLAG_OCC);
-    const int c = v ? cell : -1;
+    c = v ? cell : -1;
     inb = ok;
     val = v;
-    bool acc = false;
-    float mmu = 0.f;
     if (c >= 0) {
       const float E = I.E[c], Ec = I.Ec[c], beta = I.beta[c];
       const int mc = (int)I.m[c];
@@ -1294,9 +1295,6 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
       mmu = (float)(-mc) * muv;
       acc = metro_accepts(I, step, c, E, Ec, beta);
     }
-    ws.cand[j] = c;
-    ws.accf[j] = acc ? 1 : 0;
-    ws.cmmu[j] = mmu;
     nacc = acc ? 1 : 0;
     if (acc && fuse) atomicAdd(&s_site[hs_site(I, c / I.nx, c % I.nx)], 1);
     if (acc && mark) {
@@ -1322,8 +1320,20 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
   // in-grid, valid and accepted samples of the workgroup, 21 bits each (at most 256)
   const long long sums = block_sum_i64(((long long)inb << 42) + ((long long)val << 21) + nacc, s_l);
   const int b_inb = (int)(sums >> 42), b_val = (int)((sums >> 21) & 0x1fffff), b_acc = (int)(sums & 0x1fffff);
-  if (threadIdx.x == 0) ws.part[blk] = ((long long)b_inb << 32) + b_val;
-  if (!fuse) return;
+  // the samples and the workgroup's counts, read by later launches only (with the listing fused,
+  // they are stored after the ticket, so that the wait before it covers the counts and marks alone)
+  auto store_samples = [&]() {
+    if (j < SEL.nsc) {
+      ws.cand[j] = c;
+      ws.accf[j] = acc ? 1 : 0;
+      ws.cmmu[j] = mmu;
+    }
+    if (threadIdx.x == 0) ws.part[blk] = ((long long)b_inb << 32) + b_val;
+  };
+  if (!fuse) {
+    store_samples();
+    return;
+  }
   // Fused listing: the workgroups add their counts to the job's counters; the last one to finish
   // (ticket) completes the step as the selection workgroup would (truncation flags, offsets,
   // counters and state), and k_metro_select_pre then finds the step done. The update takes its
@@ -1345,6 +1355,7 @@ __device__ __forceinline__ void metro_presample(const MetroJob &J, int blk, Work
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0) s_last = atomicAdd(&ws.fz[0], 1) == (SEL.nsc + PRESAMPLE_THREADS - 1) / PRESAMPLE_THREADS - 1;
+  store_samples();
   __syncthreads();
   if (!s_last) return;
   const int tot_inb = __hip_atomic_load(&ws.fz[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
